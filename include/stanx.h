@@ -1,0 +1,188 @@
+/*
+ * stanx.h -- C-ABI of the MI355X (gfx950) GLM log-density/gradient engine
+ * and the NUTS sampler that drives it.
+ *
+ * The entry points replace, on the GPU, the calls the reference makes at
+ * the model boundary (paths relative to the stan-dev/stan 2.35.0 tree):
+ *
+ *   sx_glm_logp_grad       stan::model::log_prob_grad<propto,jacobian>
+ *                          (src/stan/model/log_prob_grad.hpp:28-50, 68-89) and
+ *                          stan::model::gradient (src/stan/model/gradient.hpp:14-35)
+ *                          for a model whose log_prob is a
+ *                          bernoulli_logit_glm / normal_id_glm likelihood
+ *                          (Stan Math, reached through
+ *                          src/stan/model/model_functional.hpp:21) plus priors.
+ *   sx_glm_logp            model_base::log_prob / log_prob_propto (double)
+ *                          (src/stan/model/model_base.hpp:191-302, 404-544;
+ *                          src/stan/model/log_prob_propto.hpp:32-52)
+ *   sx_glm_num_params      prob_grad::num_params_r (src/stan/model/prob_grad.hpp:57)
+ *   sx_glm_logp_grad_batch many chains at once; the reference runs chains on
+ *                          separate threads instead
+ *                          (src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:387-401)
+ *   sx_hmc_nuts_diag_e_adapt
+ *                          stan::services::sample::hmc_nuts_diag_e_adapt
+ *                          (src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:57-112),
+ *                          same arguments and return codes, draws returned
+ *                          as a table instead of through writer callbacks.
+ *   sx_hmc_nuts_diag_e_adapt_cb
+ *                          the same service for a model given as a C callback.
+ *
+ * Conventions
+ *   - Host buffers are caller-owned.  X (row-major N x K doubles) and y are
+ *     copied to the device once, at create; the context owns device memory.
+ *   - Every call is synchronous on the context's own HIP stream.  Calls on
+ *     different contexts may run concurrently from different threads.
+ *   - Return value: SX_OK, or an SX_E* status; sx_last_error() holds a
+ *     thread-local message.  SX_EDOMAIN is the reference's std::domain_error
+ *     (proposal rejected, src/stan/mcmc/hmc/hamiltonians/base_hamiltonian.hpp:61-70);
+ *     any other error is fatal (std::runtime_error).
+ *   - Service entry points return the reference's error codes
+ *     (src/stan/services/error_codes.hpp:7-18): 0 OK, 70 SOFTWARE, 78 CONFIG.
+ */
+#ifndef STANX_H
+#define STANX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SX_OK 0
+#define SX_EDOMAIN 1
+#define SX_EINVAL 2
+#define SX_EDEVICE 3
+#define SX_ENOMEM 4
+
+/* families: theta layouts (unconstrained)
+ *   SX_LOGIT      : [alpha, beta_1..beta_K]                      D = K+1
+ *   SX_NORMAL     : [alpha, beta_1..beta_K, u], sigma = exp(u)   D = K+2
+ *   SX_HIER_LOGIT : [alpha, beta_1..K, v_0..v_K, z_{1,0..K} .. z_{J,0..K}],
+ *                   tau_k = exp(v_k)                     D = 2K+2+J(K+1)  */
+#define SX_LOGIT 0
+#define SX_NORMAL 1
+#define SX_HIER_LOGIT 2
+
+/* storage/arithmetic precision of the bulk X pass */
+#define SX_FP64 0   /* FP64 X, FP64 arithmetic: the plain drop-in           */
+#define SX_FP32 1   /* FP32 X, FP32 products, FP64 block reductions         */
+#define SX_BF16X2 2 /* X as BF16 hi+lo on MFMA, FP32 accumulate (batched)   */
+
+/* service error codes (src/stan/services/error_codes.hpp:7-18) */
+#define SX_CODE_OK 0
+#define SX_CODE_SOFTWARE 70
+#define SX_CODE_CONFIG 78
+
+typedef struct sx_glm sx_glm; /* opaque */
+
+typedef struct sx_prior {
+  double alpha_sd; /* alpha ~ normal(0, alpha_sd); <= 0 means flat   */
+  double beta_sd;  /* beta  ~ normal(0, beta_sd)                     */
+  double sigma_sd; /* sigma ~ half-normal(0, sigma_sd)  (SX_NORMAL)  */
+  double tau_sd;   /* tau   ~ half-normal(0, tau_sd)    (SX_HIER_LOGIT) */
+} sx_prior;
+
+/* y: const int32_t* with values in {0,1} for SX_LOGIT / SX_HIER_LOGIT,
+ * const double* for SX_NORMAL.  group_offsets: J+1 CSR row offsets for
+ * SX_HIER_LOGIT (rows of one group are contiguous), NULL otherwise.
+ * Replaces the data constructor of a generated model (the model owns its
+ * data: src/stan/model/model_base.hpp:36-56).                          */
+int sx_glm_create(sx_glm** out, int family, const double* X, const void* y,
+                  int64_t N, int64_t K, const int64_t* group_offsets,
+                  int64_t J, int precision, const sx_prior* prior,
+                  int device);
+void sx_glm_destroy(sx_glm* m);
+int sx_glm_num_params(const sx_glm* m, int64_t* D);
+/* family the context was created with (SX_LOGIT, ...), or -1 */
+int sx_glm_family(const sx_glm* m);
+const char* sx_last_error(void);
+
+/* log density (propto / jacobian as in log_prob<propto, jacobian>) and
+ * its gradient at theta (D doubles, host).  grad may be NULL.          */
+int sx_glm_logp_grad(sx_glm* m, const double* theta, int propto,
+                     int jacobian, double* lp, double* grad);
+int sx_glm_logp(sx_glm* m, const double* theta, int propto, int jacobian,
+                double* lp);
+
+/* C chains at once: theta is C x D (row-major), lp C, grad C x D.      */
+int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C,
+                           int propto, int jacobian, double* lp,
+                           double* grad);
+
+/* Device-resident timing of the gradient pass: runs `iters` evaluations
+ * at theta (already copied to the device) after `warmup` untimed ones,
+ * bracketed by HIP events on the context's stream.  Reports the mean
+ * wall time per evaluation (ms) and the mean duration of the bulk X-pass
+ * kernel alone (ms).  With C > 1 the batched pass is timed.           */
+int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C,
+                          int iters, int warmup, double* ms_per_eval,
+                          double* ms_kernel);
+
+/* Tuning knobs of the current kernel configuration (for reports):
+ * lanes per row, slots per lane, blocks, threads per block.           */
+int sx_glm_kernel_config(const sx_glm* m, int* lanes_per_row,
+                         int* slots_per_lane, int* blocks, int* threads);
+
+/* ---------------- sampler service ---------------- */
+/* Arguments of hmc_nuts_diag_e_adapt in the reference's order
+ * (src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:58-68).          */
+typedef struct sx_nuts_config {
+  uint32_t random_seed;
+  uint32_t chain;
+  double init_radius;
+  int32_t num_warmup;
+  int32_t num_samples;
+  int32_t num_thin;
+  int32_t save_warmup;
+  int32_t refresh;
+  double stepsize;
+  double stepsize_jitter;
+  int32_t max_depth;
+  double delta;
+  double gamma;
+  double kappa;
+  double t0;
+  uint32_t init_buffer;
+  uint32_t term_buffer;
+  uint32_t window;
+} sx_nuts_config;
+
+/* Outputs of one run.  draws: rows x (7 + D) doubles with columns
+ *   lp__, accept_stat__, stepsize__, treedepth__, n_leapfrog__,
+ *   divergent__, energy__, then the D unconstrained parameters
+ * (src/stan/mcmc/base_nuts.hpp:206-220, src/stan/mcmc/sample.hpp:42-45).
+ * Capacity must be (save_warmup ? num_warmup : 0) + num_samples rows
+ * (before thinning).                                                   */
+typedef struct sx_nuts_result {
+  double* draws;
+  int64_t rows;
+  double stepsize;      /* adapted step size */
+  double* inv_metric;   /* D doubles: adapted diagonal inverse metric */
+  int64_t n_grad;       /* gradient evaluations, incl. init and heuristics */
+  double warmup_seconds;
+  double sampling_seconds;
+} sx_nuts_result;
+
+/* init / init_inv_metric: D doubles each, or NULL (random inits within
+ * init_radius; unit metric).                                           */
+int sx_hmc_nuts_diag_e_adapt(sx_glm* m, const sx_nuts_config* cfg,
+                             const double* init,
+                             const double* init_inv_metric,
+                             sx_nuts_result* out);
+
+/* Model callback: value and gradient of the log density (propto=true,
+ * jacobian=true) at q.  Return 0, 1 for a domain error, anything else for
+ * a fatal error.                                                        */
+typedef int (*sx_lpgrad_fn)(void* ctx, const double* q, int D, double* lp,
+                            double* grad);
+int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D,
+                                const sx_nuts_config* cfg,
+                                const double* init,
+                                const double* init_inv_metric,
+                                sx_nuts_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

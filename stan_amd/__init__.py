@@ -1,0 +1,247 @@
+"""stan_amd: MI355X-native GLM log-density/gradient engine + NUTS sampler.
+
+Python side of the C-ABI in ``include/stanx.h``: plain ctypes plumbing over
+``stan_amd/libstanx.so`` (HIP kernels for gfx950 + the C++ sampler
+services).  Importing this package never falls back to CPU code: if the
+library is missing, ``lib()`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstanx.so")
+
+SX_OK, SX_EDOMAIN, SX_EINVAL, SX_EDEVICE, SX_ENOMEM = 0, 1, 2, 3, 4
+LOGIT, NORMAL, HIER_LOGIT = 0, 1, 2
+FP64, FP32, BF16X2 = 0, 1, 2
+
+# every entry point declared in include/stanx.h
+EXPORTED = [
+    "sx_glm_create", "sx_glm_destroy", "sx_glm_num_params", "sx_glm_family",
+    "sx_last_error", "sx_glm_logp_grad", "sx_glm_logp", "sx_glm_logp_grad_batch",
+    "sx_glm_time_logp_grad", "sx_glm_kernel_config", "sx_hmc_nuts_diag_e_adapt",
+    "sx_hmc_nuts_diag_e_adapt_cb",
+]
+
+_lib = None
+
+
+class StanxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class DomainError(StanxError, ValueError):
+    """The reference's std::domain_error: the density cannot be evaluated."""
+
+
+class Prior(C.Structure):
+    _fields_ = [("alpha_sd", C.c_double), ("beta_sd", C.c_double),
+                ("sigma_sd", C.c_double), ("tau_sd", C.c_double)]
+
+
+class NutsConfig(C.Structure):
+    _fields_ = [("random_seed", C.c_uint32), ("chain", C.c_uint32),
+                ("init_radius", C.c_double), ("num_warmup", C.c_int32),
+                ("num_samples", C.c_int32), ("num_thin", C.c_int32),
+                ("save_warmup", C.c_int32), ("refresh", C.c_int32),
+                ("stepsize", C.c_double), ("stepsize_jitter", C.c_double),
+                ("max_depth", C.c_int32), ("delta", C.c_double), ("gamma", C.c_double),
+                ("kappa", C.c_double), ("t0", C.c_double), ("init_buffer", C.c_uint32),
+                ("term_buffer", C.c_uint32), ("window", C.c_uint32)]
+
+
+class NutsResult(C.Structure):
+    _fields_ = [("draws", C.c_void_p), ("rows", C.c_int64), ("stepsize", C.c_double),
+                ("inv_metric", C.c_void_p), ("n_grad", C.c_int64),
+                ("warmup_seconds", C.c_double), ("sampling_seconds", C.c_double)]
+
+
+LPGRAD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int,
+                        C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def lib():
+    """Load libstanx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, i64, dp = C.c_void_p, C.c_int64, C.POINTER(C.c_double)
+        L.sx_glm_create.argtypes = [C.POINTER(vp), C.c_int, vp, vp, i64, i64, vp, i64,
+                                    C.c_int, C.POINTER(Prior), C.c_int]
+        L.sx_glm_destroy.argtypes = [vp]
+        L.sx_glm_destroy.restype = None
+        L.sx_glm_num_params.argtypes = [vp, C.POINTER(i64)]
+        L.sx_glm_family.argtypes = [vp]
+        L.sx_last_error.restype = C.c_char_p
+        L.sx_glm_logp_grad.argtypes = [vp, vp, C.c_int, C.c_int, dp, vp]
+        L.sx_glm_logp.argtypes = [vp, vp, C.c_int, C.c_int, dp]
+        L.sx_glm_logp_grad_batch.argtypes = [vp, vp, i64, C.c_int, C.c_int, vp, vp]
+        L.sx_glm_time_logp_grad.argtypes = [vp, vp, i64, C.c_int, C.c_int, dp, dp]
+        L.sx_glm_kernel_config.argtypes = [vp] + [C.POINTER(C.c_int)] * 4
+        L.sx_hmc_nuts_diag_e_adapt.argtypes = [vp, C.POINTER(NutsConfig), vp, vp,
+                                               C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_adapt_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
+                                                  vp, vp, C.POINTER(NutsResult)]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().sx_last_error().decode(errors="replace")
+
+
+def _check(rc):
+    if rc == SX_OK:
+        return
+    msg = last_error()
+    if rc == SX_EDOMAIN:
+        raise DomainError(rc, msg)
+    raise StanxError(rc, msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class GlmModel:
+    """A GLM log density with its data resident on the GPU.
+
+    family: LOGIT, NORMAL (HIER_LOGIT needs ``offsets``).  X is N x K,
+    y int {0,1} (logistic) or float (normal).  Theta layouts are those of
+    include/stanx.h.
+    """
+
+    def __init__(self, family, X, y, offsets=None, precision=FP64,
+                 prior=(5.0, 2.0, 2.0, 1.0), device=0):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        if X.ndim != 2:
+            raise ValueError("X must be N x K")
+        N, K = X.shape
+        if family == NORMAL:
+            y = np.ascontiguousarray(y, dtype=np.float64)
+        else:
+            y = np.ascontiguousarray(y, dtype=np.int32)
+        if y.shape != (N,):
+            raise ValueError("y must have N entries")
+        off = None
+        J = 0
+        if offsets is not None:
+            off = np.ascontiguousarray(offsets, dtype=np.int64)
+            J = len(off) - 1
+        self._h = C.c_void_p()
+        self.family, self.N, self.K, self.J = family, N, K, J
+        self.prior = Prior(*prior)
+        _check(lib().sx_glm_create(C.byref(self._h), family, _ptr(X), _ptr(y), N, K,
+                                   _ptr(off), J, precision, C.byref(self.prior), device))
+        D = C.c_int64()
+        _check(lib().sx_glm_num_params(self._h, C.byref(D)))
+        self.D = D.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sx_glm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def log_prob_grad(self, theta, propto=True, jacobian=True):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        g = np.zeros(self.D)
+        lp = C.c_double()
+        _check(lib().sx_glm_logp_grad(self._h, _ptr(theta), int(propto), int(jacobian),
+                                      C.byref(lp), _ptr(g)))
+        return lp.value, g
+
+    def log_prob(self, theta, propto=False, jacobian=True):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        lp = C.c_double()
+        _check(lib().sx_glm_logp(self._h, _ptr(theta), int(propto), int(jacobian),
+                                 C.byref(lp)))
+        return lp.value
+
+    def log_prob_grad_batch(self, theta, propto=True, jacobian=True):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        Cn = theta.shape[0]
+        lp = np.zeros(Cn)
+        g = np.zeros((Cn, self.D))
+        rc = lib().sx_glm_logp_grad_batch(self._h, _ptr(theta), Cn, int(propto),
+                                          int(jacobian), _ptr(lp), _ptr(g))
+        if rc not in (SX_OK, SX_EDOMAIN):
+            _check(rc)
+        return lp, g
+
+    def time_log_prob_grad(self, theta, iters=100, warmup=10, chains=1):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        ms, kms = C.c_double(), C.c_double()
+        _check(lib().sx_glm_time_logp_grad(self._h, _ptr(theta), chains, iters, warmup,
+                                           C.byref(ms), C.byref(kms)))
+        return ms.value, kms.value
+
+    def kernel_config(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().sx_glm_kernel_config(self._h, *[C.byref(x) for x in v]))
+        return dict(lanes_per_row=v[0].value, slots_per_lane=v[1].value,
+                    blocks=v[2].value, threads=v[3].value)
+
+
+def nuts_config(**kw):
+    """Defaults of the reference service (src/stan/services/sample/defaults.hpp;
+    delta 0.8 as in CmdStan)."""
+    a = dict(random_seed=1234, chain=1, init_radius=2.0, num_warmup=1000,
+             num_samples=1000, num_thin=1, save_warmup=0, refresh=0, stepsize=1.0,
+             stepsize_jitter=0.0, max_depth=10, delta=0.8, gamma=0.05, kappa=0.75,
+             t0=10.0, init_buffer=75, term_buffer=50, window=25)
+    a.update(kw)
+    return NutsConfig(**a)
+
+
+def _run_service(call, D, cfg, init, inv_metric):
+    rows_cap = (cfg.num_warmup if cfg.save_warmup else 0) + cfg.num_samples
+    draws = np.zeros((max(rows_cap, 1), 7 + D))
+    minv = np.zeros(D)
+    res = NutsResult(_ptr(draws).value, 0, 0.0, _ptr(minv).value, 0, 0.0, 0.0)
+    init_a = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
+    minv_a = None if inv_metric is None else np.ascontiguousarray(inv_metric,
+                                                                   dtype=np.float64)
+    rc = call(init_a, minv_a, res)
+    return dict(rc=rc, draws=draws[: res.rows], stepsize=res.stepsize, inv_metric=minv,
+                n_grad=res.n_grad, warmup_seconds=res.warmup_seconds,
+                sampling_seconds=res.sampling_seconds,
+                error=last_error() if rc else "")
+
+
+def hmc_nuts_diag_e_adapt(model, init=None, inv_metric=None, **kw):
+    """Adaptive diag_e NUTS on a GlmModel (one chain, FP64 gradient on the GPU).
+
+    Returns a dict with rc (0 / 70 / 78), draws (rows x (7 + D): lp__,
+    accept_stat__, stepsize__, treedepth__, n_leapfrog__, divergent__, energy__,
+    constrained parameters), stepsize, inv_metric, n_grad, timings."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt(model.handle, C.byref(cfg), _ptr(i),
+                                                       _ptr(m), C.byref(r)),
+        model.D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
+    """The same service for a model given as a C callback (function pointer
+    with the sx_lpgrad_fn signature)."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, C.byref(cfg),
+                                                          _ptr(i), _ptr(m), C.byref(r)),
+        D, cfg, init, inv_metric)

@@ -1,0 +1,101 @@
+// C-ABI of the sampler service: runs the C++ adaptive diag_e NUTS
+// (stanx/services.hpp) on a GPU-backed GLM model or on a model given as
+// a C callback, and returns the draws as a table.
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/stanx.h"
+#include "../include/stanx/callbacks.hpp"
+#include "../include/stanx/gpu_glm_model.hpp"
+#include "../include/stanx/model.hpp"
+#include "../include/stanx/services.hpp"
+
+namespace sx {
+void set_error(const std::string& msg);
+}
+
+namespace {
+
+template <class Model>
+int run_service(const Model& model, const sx_nuts_config* cfg, const double* init,
+                const double* init_inv_metric, sx_nuts_result* out) {
+  if (!cfg || !out || !out->draws) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt: NULL argument");
+    return SX_CODE_CONFIG;
+  }
+  const size_t D = model.num_params_r();
+  std::vector<double> init_v, minv_v;
+  if (init) init_v.assign(init, init + D);
+  if (init_inv_metric) minv_v.assign(init_inv_metric, init_inv_metric + D);
+  stanx::callbacks::interrupt interrupt;
+  stanx::callbacks::counting_logger logger;
+  stanx::callbacks::writer init_writer, diag_writer;
+  stanx::callbacks::table_writer sample_writer;
+  double eps = 0.0, warm = 0.0, samp = 0.0;
+  std::vector<double> minv_out;
+  long long n_grad = 0;
+  int rc = SX_CODE_SOFTWARE;
+  try {
+    rc = stanx::services::sample::hmc_nuts_diag_e_adapt(
+        model, init ? &init_v : nullptr, init_inv_metric ? &minv_v : nullptr,
+        cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
+        cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
+        cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma,
+        cfg->kappa, cfg->t0, cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt,
+        logger, init_writer, sample_writer, diag_writer, &eps, &minv_out, &n_grad, &warm,
+        &samp);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+  if (!logger.errors.empty()) sx::set_error(logger.errors.back());
+  // rows carry: lp__, accept_stat__, 5 sampler params, then constrained
+  // params from write_array; report unconstrained q for the table instead
+  out->rows = static_cast<int64_t>(sample_writer.rows.size());
+  const size_t ncol = 7 + D;
+  for (size_t i = 0; i < sample_writer.rows.size(); ++i) {
+    const auto& r = sample_writer.rows[i];
+    std::memcpy(out->draws + i * ncol, r.data(), sizeof(double) * std::min(r.size(), ncol));
+  }
+  out->stepsize = eps;
+  if (out->inv_metric && minv_out.size() == D)
+    std::memcpy(out->inv_metric, minv_out.data(), sizeof(double) * D);
+  out->n_grad = n_grad;
+  out->warmup_seconds = warm;
+  out->sampling_seconds = samp;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sx_hmc_nuts_diag_e_adapt(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                             const double* init_inv_metric, sx_nuts_result* out) {
+  if (!m) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt: NULL model");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config* cfg,
+                                const double* init, const double* init_inv_metric,
+                                sx_nuts_result* out) {
+  if (!f || D < 0) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt_cb: invalid model");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// Internal definitions shared by the HIP kernels and the C-ABI layer.
+#ifndef STANX_INTERNAL_HPP
+#define STANX_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/stanx.h"
+
+namespace sx {
+
+void set_error(const std::string& msg);
+
+#define SX_HIP(call)                                                     \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      ::sx::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return SX_EDEVICE;                                                 \
+    }                                                                    \
+  } while (0)
+
+// Geometry of the lane-slot tiled X layout used by the bulk passes.
+// A wave-tile holds R = 64 / L rows; each row is split across L lanes and
+// each lane holds S = ceil(K / L) columns ("slots").  Element (row, col)
+// of the tile t lives at  t*S*64 + (col / L)*64 + (row % R)*L + col % L,
+// so slot j of all 64 lanes is one contiguous 512-byte (FP64) segment.
+struct tile_geom {
+  int L = 1;       // lanes per row (power of two, <= 64)
+  int lgL = 0;     // log2(L)
+  int R = 64;      // rows per tile
+  int S = 0;       // slots per lane
+  int smax = 8;    // compiled register capacity (>= S)
+  int64_t T = 0;   // number of tiles
+};
+
+tile_geom make_geom(int64_t N, int64_t K, int L, int smax);
+
+// Arguments of the FP64 bulk pass (K1 logistic, K2 normal).
+struct pass_args {
+  const double* X;      // tiled
+  const void* y;        // int8 (logistic) or double (normal)
+  const double* theta;  // D doubles on the device
+  double* partials;     // blocks x E
+  unsigned* counter;    // ticket for the last-block reduction
+  double* out;          // lp, grad[D], status
+  int64_t N;
+  int64_t T;
+  int K, L, lgL, S, D, E;
+  int propto, jacobian;
+  double alpha_sd, beta_sd, sigma_sd;
+};
+
+hipError_t launch_pass_f64(int family, const tile_geom& g, const pass_args& a,
+                           int blocks, int threads, hipStream_t s);
+hipError_t launch_tile_x_f64(const double* Xrm, double* Xt, int64_t N,
+                             int64_t K, const tile_geom& g, hipStream_t s);
+int max_blocks_per_cu(int family, int smax, int threads);
+
+}  // namespace sx
+
+struct sx_glm {
+  int family = 0, precision = 0, device = 0;
+  int64_t N = 0, K = 0, J = 0, D = 0;
+  sx_prior prior{};
+  hipStream_t stream = nullptr;
+  sx::tile_geom geom;
+  double* d_X = nullptr;
+  void* d_y = nullptr;
+  double* d_theta = nullptr;
+  double* d_partials = nullptr;
+  unsigned* d_counter = nullptr;
+  double* d_out = nullptr;
+  double* h_theta = nullptr;  // pinned
+  double* h_out = nullptr;    // pinned
+  int blocks = 0, threads = 256;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+#endif

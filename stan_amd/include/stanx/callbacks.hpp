@@ -1,0 +1,75 @@
+// Output and control callbacks of the sampler services, mirroring the
+// reference's interfaces (src/stan/callbacks/writer.hpp:16-61,
+// logger.hpp:24-97, interrupt.hpp:14-28).
+#ifndef STANX_CALLBACKS_HPP
+#define STANX_CALLBACKS_HPP
+
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace stanx {
+namespace callbacks {
+
+class writer {
+ public:
+  virtual ~writer() {}
+  virtual void operator()(const std::vector<std::string>& names) {}
+  virtual void operator()(const std::vector<double>& state) {}
+  virtual void operator()() {}
+  virtual void operator()(const std::string& message) {}
+};
+
+class logger {
+ public:
+  virtual ~logger() {}
+  virtual void debug(const std::string& message) {}
+  virtual void debug(const std::stringstream& message) {}
+  virtual void info(const std::string& message) {}
+  virtual void info(const std::stringstream& message) {}
+  virtual void warn(const std::string& message) {}
+  virtual void warn(const std::stringstream& message) {}
+  virtual void error(const std::string& message) {}
+  virtual void error(const std::stringstream& message) {}
+  virtual void fatal(const std::string& message) {}
+  virtual void fatal(const std::stringstream& message) {}
+};
+
+class interrupt {
+ public:
+  virtual ~interrupt() {}
+  virtual void operator()() {}
+};
+
+// Collects every numeric row and every message (tests and the C-ABI use it).
+class table_writer : public writer {
+ public:
+  std::vector<std::string> names;
+  std::vector<std::vector<double>> rows;
+  std::vector<std::string> messages;
+  void operator()(const std::vector<std::string>& n) override { names = n; }
+  void operator()(const std::vector<double>& s) override { rows.push_back(s); }
+  void operator()(const std::string& m) override { messages.push_back(m); }
+};
+
+class counting_logger : public logger {
+ public:
+  int n_info = 0, n_warn = 0, n_error = 0;
+  std::vector<std::string> errors;
+  void info(const std::string&) override { ++n_info; }
+  void info(const std::stringstream&) override { ++n_info; }
+  void warn(const std::string&) override { ++n_warn; }
+  void warn(const std::stringstream&) override { ++n_warn; }
+  void error(const std::string& m) override {
+    ++n_error;
+    errors.push_back(m);
+  }
+  void error(const std::stringstream& m) override {
+    ++n_error;
+    errors.push_back(m.str());
+  }
+};
+
+}  // namespace callbacks
+}  // namespace stanx
+#endif

@@ -1,0 +1,445 @@
+// Multinomial NUTS with a diagonal Euclidean metric, written as an
+// explicit-stack (iterative) tree builder so that the same state machine
+// can run one leapfrog step at a time; the GPU batched sampler uses the
+// same frame layout per chain.  Semantics follow the recursive reference
+// exactly, including the draw order of the RNG:
+//   base_hmc                  src/stan/mcmc/hmc/base_hmc.hpp:28-215
+//   base_nuts::transition     src/stan/mcmc/hmc/nuts/base_nuts.hpp:78-204
+//   base_nuts::build_tree     src/stan/mcmc/hmc/nuts/base_nuts.hpp:247-352
+//   compute_criterion         src/stan/mcmc/hmc/nuts/base_nuts.hpp:222-226
+// Per merge of two half-subtrees: a log-sum-exp of their weights, a
+// uniform progressive sample (one uniform draw whenever the final half
+// does not dominate), and three U-turn checks (merged, left join, right
+// join).  At the top level the sample is biased-progressive.
+#ifndef STANX_MCMC_BASE_NUTS_HPP
+#define STANX_MCMC_BASE_NUTS_HPP
+
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../callbacks.hpp"
+#include "../philox.hpp"
+#include "hamiltonian.hpp"
+
+namespace stanx {
+namespace mcmc {
+
+// log(1 + exp(a)) and log(exp(a) + exp(b)) with the -inf short cut of
+// Stan Math's log_sum_exp(a, b).
+inline double log1p_exp(double a) {
+  return a > 0.0 ? a + std::log1p(std::exp(-a)) : std::log1p(std::exp(a));
+}
+inline double log_sum_exp(double a, double b) {
+  const double inf = std::numeric_limits<double>::infinity();
+  if (a == -inf) return b;
+  if (a == inf && b == inf) return inf;
+  if (a > b) return a + log1p_exp(b - a);
+  return b + log1p_exp(a - b);
+}
+inline double dot(const vec_t& a, const vec_t& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+// draw returned by a transition (src/stan/mcmc/sample.hpp:12-56)
+class sample {
+ public:
+  sample(const vec_t& q, double log_prob, double stat)
+      : cont_params_(q), log_prob_(log_prob), accept_stat_(stat) {}
+  const vec_t& cont_params() const { return cont_params_; }
+  double log_prob() const { return log_prob_; }
+  double accept_stat() const { return accept_stat_; }
+  static void get_sample_param_names(std::vector<std::string>& names) {
+    names.push_back("lp__");
+    names.push_back("accept_stat__");
+  }
+  void get_sample_params(std::vector<double>& values) const {
+    values.push_back(log_prob_);
+    values.push_back(accept_stat_);
+  }
+
+ private:
+  vec_t cont_params_;
+  double log_prob_, accept_stat_;
+};
+
+template <class Model>
+class base_nuts {
+ public:
+  using hamiltonian_t = diag_e_metric<Model>;
+
+  base_nuts(const Model& model, rng_t& rng)
+      : z_(static_cast<int>(model.num_params_r())),
+        hamiltonian_(model),
+        rng_(rng),
+        nom_epsilon_(0.1),
+        epsilon_(0.1),
+        epsilon_jitter_(0.0),
+        depth_(0),
+        max_depth_(5),
+        max_deltaH_(1000),
+        n_leapfrog_(0),
+        divergent_(false),
+        energy_(0) {}
+  virtual ~base_nuts() {}
+
+  // ---- base_hmc surface ----
+  diag_e_point& z() { return z_; }
+  const diag_e_point& z() const { return z_; }
+  hamiltonian_t& hamiltonian() { return hamiltonian_; }
+  void set_metric(const vec_t& inv_e_metric) { z_.set_metric(inv_e_metric); }
+  void set_nominal_stepsize(double e) {
+    if (e > 0) nom_epsilon_ = e;
+  }
+  double get_nominal_stepsize() const { return nom_epsilon_; }
+  double get_current_stepsize() const { return epsilon_; }
+  void set_stepsize_jitter(double j) {
+    if (j > 0 && j < 1) epsilon_jitter_ = j;
+  }
+  double get_stepsize_jitter() const { return epsilon_jitter_; }
+  void set_max_depth(int d) {
+    if (d > 0) max_depth_ = d;
+  }
+  void set_max_delta(double d) { max_deltaH_ = d; }
+  int get_max_depth() const { return max_depth_; }
+  double get_max_delta() const { return max_deltaH_; }
+  int depth() const { return depth_; }
+  int n_leapfrog() const { return n_leapfrog_; }
+  bool divergent() const { return divergent_; }
+  double energy() const { return energy_; }
+
+  void seed(const vec_t& q) { z_.q = q; }
+  void init_hamiltonian(callbacks::logger& logger) { hamiltonian_.init(z_, logger); }
+
+  void sample_stepsize() {
+    epsilon_ = nom_epsilon_;
+    if (epsilon_jitter_)
+      epsilon_ *= 1.0 + epsilon_jitter_ * (2.0 * rng_.uniform() - 1.0);
+  }
+
+  // step-size heuristic (base_hmc.hpp:93-161)
+  void init_stepsize(callbacks::logger& logger) {
+    ps_point z_init(static_cast<int>(z_.q.size()));
+    z_init.assign(z_);
+    if (z_.q.size() == 0) {
+      nom_epsilon_ = std::numeric_limits<double>::quiet_NaN();
+      return;
+    }
+    if (nom_epsilon_ == 0 || nom_epsilon_ > 1e7 || std::isnan(nom_epsilon_))
+      return;
+    hamiltonian_.sample_p(z_, rng_);
+    hamiltonian_.init(z_, logger);
+    double H0 = hamiltonian_.H(z_);
+    integrator_.evolve(z_, hamiltonian_, nom_epsilon_, logger);
+    double h = hamiltonian_.H(z_);
+    if (std::isnan(h)) h = std::numeric_limits<double>::infinity();
+    double delta_H = H0 - h;
+    const int direction = delta_H > std::log(0.8) ? 1 : -1;
+    while (1) {
+      z_.assign(z_init);
+      hamiltonian_.sample_p(z_, rng_);
+      hamiltonian_.init(z_, logger);
+      double H0b = hamiltonian_.H(z_);
+      integrator_.evolve(z_, hamiltonian_, nom_epsilon_, logger);
+      double hb = hamiltonian_.H(z_);
+      if (std::isnan(hb)) hb = std::numeric_limits<double>::infinity();
+      double dH = H0b - hb;
+      if ((direction == 1) && !(dH > std::log(0.8)))
+        break;
+      else if ((direction == -1) && !(dH < std::log(0.8)))
+        break;
+      else
+        nom_epsilon_ = direction == 1 ? 2.0 * nom_epsilon_ : 0.5 * nom_epsilon_;
+      if (nom_epsilon_ > 1e7)
+        throw std::runtime_error(
+            "Posterior is improper. Please check your model.");
+      if (nom_epsilon_ == 0)
+        throw std::runtime_error(
+            "No acceptably small step size could be found. Perhaps the "
+            "posterior is not continuous?");
+    }
+    z_.assign(z_init);
+  }
+
+  // ---- NUTS transition (base_nuts.hpp:78-204) ----
+  sample transition(const sample& init_sample, callbacks::logger& logger) {
+    const int D = static_cast<int>(z_.q.size());
+    sample_stepsize();
+    seed(init_sample.cont_params());
+    hamiltonian_.sample_p(z_, rng_);
+    hamiltonian_.init(z_, logger);
+    ensure_frames(D);
+
+    ps_point& z_fwd = top_.z_fwd;
+    ps_point& z_bck = top_.z_bck;
+    ps_point& z_sample = top_.z_sample;
+    ps_point& z_propose = top_.z_propose;
+    z_fwd.assign(z_);
+    z_bck.assign(z_fwd);
+    z_sample.assign(z_fwd);
+    z_propose.assign(z_fwd);
+
+    vec_t& p_fwd_fwd = top_.p_fwd_fwd;
+    vec_t& p_sharp_fwd_fwd = top_.p_sharp_fwd_fwd;
+    vec_t& p_fwd_bck = top_.p_fwd_bck;
+    vec_t& p_sharp_fwd_bck = top_.p_sharp_fwd_bck;
+    vec_t& p_bck_fwd = top_.p_bck_fwd;
+    vec_t& p_sharp_bck_fwd = top_.p_sharp_bck_fwd;
+    vec_t& p_bck_bck = top_.p_bck_bck;
+    vec_t& p_sharp_bck_bck = top_.p_sharp_bck_bck;
+    vec_t& rho = top_.rho;
+    vec_t& rho_fwd = top_.rho_fwd;
+    vec_t& rho_bck = top_.rho_bck;
+    vec_t& rho_ext = top_.rho_ext;
+
+    p_fwd_fwd = z_.p;
+    hamiltonian_.dtau_dp(z_, p_sharp_fwd_fwd);
+    p_fwd_bck = z_.p;
+    p_sharp_fwd_bck = p_sharp_fwd_fwd;
+    p_bck_fwd = z_.p;
+    p_sharp_bck_fwd = p_sharp_fwd_fwd;
+    p_bck_bck = z_.p;
+    p_sharp_bck_bck = p_sharp_fwd_fwd;
+    rho = z_.p;
+
+    double log_sum_weight = 0;
+    double H0 = hamiltonian_.H(z_);
+    int n_leapfrog = 0;
+    double sum_metro_prob = 0;
+    depth_ = 0;
+    divergent_ = false;
+
+    while (depth_ < max_depth_) {
+      std::fill(rho_fwd.begin(), rho_fwd.end(), 0.0);
+      std::fill(rho_bck.begin(), rho_bck.end(), 0.0);
+      bool valid_subtree = false;
+      double log_sum_weight_subtree = -std::numeric_limits<double>::infinity();
+      if (rng_.uniform() > 0.5) {
+        z_.assign(z_fwd);
+        rho_bck = rho;
+        p_bck_fwd = p_fwd_fwd;
+        p_sharp_bck_fwd = p_sharp_fwd_fwd;
+        valid_subtree = build_tree(depth_, z_propose, p_sharp_fwd_bck,
+                                   p_sharp_fwd_fwd, rho_fwd, p_fwd_bck,
+                                   p_fwd_fwd, H0, 1, n_leapfrog,
+                                   log_sum_weight_subtree, sum_metro_prob,
+                                   logger);
+        z_fwd.assign(z_);
+      } else {
+        z_.assign(z_bck);
+        rho_fwd = rho;
+        p_fwd_bck = p_bck_bck;
+        p_sharp_fwd_bck = p_sharp_bck_bck;
+        valid_subtree = build_tree(depth_, z_propose, p_sharp_bck_fwd,
+                                   p_sharp_bck_bck, rho_bck, p_bck_fwd,
+                                   p_bck_bck, H0, -1, n_leapfrog,
+                                   log_sum_weight_subtree, sum_metro_prob,
+                                   logger);
+        z_bck.assign(z_);
+      }
+      if (!valid_subtree) break;
+      ++depth_;
+      if (log_sum_weight_subtree > log_sum_weight) {
+        z_sample.assign(z_propose);
+      } else {
+        const double accept_prob = std::exp(log_sum_weight_subtree - log_sum_weight);
+        if (rng_.uniform() < accept_prob) z_sample.assign(z_propose);
+      }
+      log_sum_weight = log_sum_exp(log_sum_weight, log_sum_weight_subtree);
+      for (int i = 0; i < D; ++i) rho[i] = rho_bck[i] + rho_fwd[i];
+      bool persist = compute_criterion(p_sharp_bck_bck, p_sharp_fwd_fwd, rho);
+      for (int i = 0; i < D; ++i) rho_ext[i] = rho_bck[i] + p_fwd_bck[i];
+      persist &= compute_criterion(p_sharp_bck_bck, p_sharp_fwd_bck, rho_ext);
+      for (int i = 0; i < D; ++i) rho_ext[i] = rho_fwd[i] + p_bck_fwd[i];
+      persist &= compute_criterion(p_sharp_bck_fwd, p_sharp_fwd_fwd, rho_ext);
+      if (!persist) break;
+    }
+    n_leapfrog_ = n_leapfrog;
+    const double accept_prob = sum_metro_prob / static_cast<double>(n_leapfrog);
+    z_.assign(z_sample);
+    energy_ = hamiltonian_.H(z_);
+    return sample(z_.q, -z_.V, accept_prob);
+  }
+
+  static bool compute_criterion(const vec_t& p_sharp_minus,
+                                const vec_t& p_sharp_plus, const vec_t& rho) {
+    return dot(p_sharp_plus, rho) > 0 && dot(p_sharp_minus, rho) > 0;
+  }
+
+  // build_tree with the reference's argument semantics, evaluated with an
+  // explicit frame stack instead of recursion.
+  bool build_tree(int depth, ps_point& z_propose, vec_t& p_sharp_beg,
+                  vec_t& p_sharp_end, vec_t& rho, vec_t& p_beg, vec_t& p_end,
+                  double H0, double sign, int& n_leapfrog,
+                  double& log_sum_weight, double& sum_metro_prob,
+                  callbacks::logger& logger) {
+    const int D = static_cast<int>(z_.q.size());
+    ensure_frames(D);
+    if (static_cast<int>(frames_.size()) < depth + 1)
+      frames_.resize(depth + 1, frame(D));
+    // dest[l]: where the subtree of depth l being built writes its results
+    std::vector<dest>& dst = dests_;
+    dst.assign(depth + 1, dest());
+    dst[depth] = dest{&z_propose, &p_sharp_beg, &p_sharp_end, &rho,
+                      &p_beg,     &p_end,       &log_sum_weight};
+    for (int l = depth; l >= 1; --l) start_frame(l, dst);
+
+    while (true) {
+      // leaf: one leapfrog step writes a depth-0 subtree into dst[0]
+      bool valid = leaf(dst[0], H0, sign, n_leapfrog, sum_metro_prob, logger);
+      int l = 1;
+      bool descended = false;
+      while (l <= depth) {
+        frame& f = frames_[l];
+        if (!valid) return false;
+        if (f.stage == 0) {
+          // initial half done: start the final half from the current point
+          f.stage = 1;
+          f.z_propose_final.assign(z_);
+          f.log_sum_weight_final = -std::numeric_limits<double>::infinity();
+          std::fill(f.rho_final.begin(), f.rho_final.end(), 0.0);
+          dst[l - 1] = dest{&f.z_propose_final, &f.p_sharp_final_beg,
+                            dst[l].p_sharp_end, &f.rho_final,
+                            &f.p_final_beg,     dst[l].p_end,
+                            &f.log_sum_weight_final};
+          for (int m = l - 1; m >= 1; --m) start_frame(m, dst);
+          descended = true;
+          break;
+        }
+        // final half done: merge the two halves of level l
+        valid = merge(f, dst[l]);
+        ++l;
+      }
+      if (!descended) return valid;
+    }
+  }
+
+ protected:
+  struct dest {
+    ps_point* z_propose = nullptr;
+    vec_t* p_sharp_beg = nullptr;
+    vec_t* p_sharp_end = nullptr;
+    vec_t* rho = nullptr;
+    vec_t* p_beg = nullptr;
+    vec_t* p_end = nullptr;
+    double* log_sum_weight = nullptr;
+  };
+  struct frame {
+    explicit frame(int D)
+        : stage(0),
+          p_init_end(D),
+          p_sharp_init_end(D),
+          rho_init(D),
+          z_propose_final(D),
+          p_final_beg(D),
+          p_sharp_final_beg(D),
+          rho_final(D),
+          rho_subtree(D) {}
+    int stage;
+    double log_sum_weight_init = 0, log_sum_weight_final = 0;
+    vec_t p_init_end, p_sharp_init_end, rho_init;
+    ps_point z_propose_final;
+    vec_t p_final_beg, p_sharp_final_beg, rho_final, rho_subtree;
+  };
+  struct top_state {
+    explicit top_state(int D)
+        : z_fwd(D), z_bck(D), z_sample(D), z_propose(D), p_fwd_fwd(D),
+          p_sharp_fwd_fwd(D), p_fwd_bck(D), p_sharp_fwd_bck(D), p_bck_fwd(D),
+          p_sharp_bck_fwd(D), p_bck_bck(D), p_sharp_bck_bck(D), rho(D),
+          rho_fwd(D), rho_bck(D), rho_ext(D) {}
+    ps_point z_fwd, z_bck, z_sample, z_propose;
+    vec_t p_fwd_fwd, p_sharp_fwd_fwd, p_fwd_bck, p_sharp_fwd_bck, p_bck_fwd,
+        p_sharp_bck_fwd, p_bck_bck, p_sharp_bck_bck, rho, rho_fwd, rho_bck,
+        rho_ext;
+  };
+
+  void ensure_frames(int D) {
+    if (top_.rho.size() != static_cast<size_t>(D)) top_ = top_state(D);
+    const size_t want = static_cast<size_t>(max_depth_ + 1);
+    if (frames_.empty() || frames_[0].rho_init.size() != static_cast<size_t>(D))
+      frames_.assign(want, frame(D));
+    else if (frames_.size() < want)
+      frames_.resize(want, frame(D));
+  }
+
+  // begin building the initial half of level l (depth l-1 subtree)
+  void start_frame(int l, std::vector<dest>& dst) {
+    frame& f = frames_[l];
+    f.stage = 0;
+    f.log_sum_weight_init = -std::numeric_limits<double>::infinity();
+    std::fill(f.rho_init.begin(), f.rho_init.end(), 0.0);
+    dst[l - 1] = dest{dst[l].z_propose, dst[l].p_sharp_beg, &f.p_sharp_init_end,
+                      &f.rho_init,      dst[l].p_beg,       &f.p_init_end,
+                      &f.log_sum_weight_init};
+  }
+
+  // base case of build_tree (base_nuts.hpp:253-282)
+  bool leaf(const dest& d, double H0, double sign, int& n_leapfrog,
+            double& sum_metro_prob, callbacks::logger& logger) {
+    integrator_.evolve(z_, hamiltonian_, sign * epsilon_, logger);
+    ++n_leapfrog;
+    double h = hamiltonian_.H(z_);
+    if (std::isnan(h)) h = std::numeric_limits<double>::infinity();
+    if ((h - H0) > max_deltaH_) divergent_ = true;
+    *d.log_sum_weight = log_sum_exp(*d.log_sum_weight, H0 - h);
+    if (H0 - h > 0)
+      sum_metro_prob += 1;
+    else
+      sum_metro_prob += std::exp(H0 - h);
+    d.z_propose->assign(z_);
+    hamiltonian_.dtau_dp(z_, *d.p_sharp_beg);
+    *d.p_sharp_end = *d.p_sharp_beg;
+    vec_t& rho = *d.rho;
+    for (size_t i = 0; i < rho.size(); ++i) rho[i] += z_.p[i];
+    *d.p_beg = z_.p;
+    *d.p_end = *d.p_beg;
+    return !divergent_;
+  }
+
+  // merge of the two halves of a subtree (base_nuts.hpp:321-351)
+  bool merge(frame& f, const dest& d) {
+    const size_t D = f.rho_init.size();
+    const double log_sum_weight_subtree =
+        log_sum_exp(f.log_sum_weight_init, f.log_sum_weight_final);
+    *d.log_sum_weight = log_sum_exp(*d.log_sum_weight, log_sum_weight_subtree);
+    if (f.log_sum_weight_final > log_sum_weight_subtree) {
+      d.z_propose->assign(f.z_propose_final);
+    } else {
+      const double accept_prob =
+          std::exp(f.log_sum_weight_final - log_sum_weight_subtree);
+      if (rng_.uniform() < accept_prob) d.z_propose->assign(f.z_propose_final);
+    }
+    vec_t& rs = f.rho_subtree;
+    vec_t& rho = *d.rho;
+    for (size_t i = 0; i < D; ++i) rs[i] = f.rho_init[i] + f.rho_final[i];
+    for (size_t i = 0; i < D; ++i) rho[i] += rs[i];
+    bool persist = compute_criterion(*d.p_sharp_beg, *d.p_sharp_end, rs);
+    for (size_t i = 0; i < D; ++i) rs[i] = f.rho_init[i] + f.p_final_beg[i];
+    persist &= compute_criterion(*d.p_sharp_beg, f.p_sharp_final_beg, rs);
+    for (size_t i = 0; i < D; ++i) rs[i] = f.rho_final[i] + f.p_init_end[i];
+    persist &= compute_criterion(f.p_sharp_init_end, *d.p_sharp_end, rs);
+    return persist;
+  }
+
+  diag_e_point z_;
+  expl_leapfrog<hamiltonian_t> integrator_;
+  hamiltonian_t hamiltonian_;
+  rng_t& rng_;
+  double nom_epsilon_, epsilon_, epsilon_jitter_;
+  int depth_, max_depth_;
+  double max_deltaH_;
+  int n_leapfrog_;
+  bool divergent_;
+  double energy_;
+  std::vector<frame> frames_;
+  std::vector<dest> dests_;
+  top_state top_{0};
+};
+
+}  // namespace mcmc
+}  // namespace stanx
+#endif
