@@ -92,6 +92,31 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y,
                   int64_t N, int64_t K, const int64_t* group_offsets,
                   int64_t J, int precision, const sx_prior* prior,
                   int device);
+/* Kernel tuning of the bulk X pass.  Zero fields mean "choose":
+ *   lanes_per_row  lanes sharing one row of X (power of two <= 64); fixes
+ *                  the tiled layout of X in HBM
+ *   slots_cap      compiled register capacity per lane (multiple of 4, <= 32)
+ *   prefetch       1: double-buffer tiles in registers; 0: single buffer;
+ *                  -1: choose
+ *   blocks         workgroups per launch
+ *   autotune       1: time the candidate configurations on the device at
+ *                  create and keep the fastest (default for N*K >= 2^22)
+ * No counterpart in the reference (its CPU GLM has no tunables).         */
+typedef struct sx_tuning {
+  int lanes_per_row;
+  int slots_cap;
+  int prefetch;
+  int blocks;
+  int autotune;
+} sx_tuning;
+
+/* sx_glm_create with explicit tuning (NULL = defaults). */
+int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y,
+                     int64_t N, int64_t K, const int64_t* group_offsets,
+                     int64_t J, int precision, const sx_prior* prior,
+                     int device, const sx_tuning* tuning);
+/* Text log of the autotuner's measurements and the chosen configuration. */
+const char* sx_glm_tuning_log(const sx_glm* m);
 void sx_glm_destroy(sx_glm* m);
 int sx_glm_num_params(const sx_glm* m, int64_t* D);
 /* family the context was created with (SX_LOGIT, ...), or -1 */
@@ -110,11 +135,12 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C,
                            int propto, int jacobian, double* lp,
                            double* grad);
 
-/* Device-resident timing of the gradient pass: runs `iters` evaluations
- * at theta (already copied to the device) after `warmup` untimed ones,
- * bracketed by HIP events on the context's stream.  Reports the mean
- * wall time per evaluation (ms) and the mean duration of the bulk X-pass
- * kernel alone (ms).  With C > 1 the batched pass is timed.           */
+/* Device-resident timing of the gradient evaluation: `iters` evaluations
+ * at theta (copied to the device once) after `warmup` untimed ones,
+ * bracketed by HIP events on the context's stream.  ms_per_eval: mean
+ * time of a full evaluation (bulk X pass + cross-block reduction).
+ * ms_kernel: mean duration of the bulk X pass alone, from a second loop
+ * that launches only that kernel.  C must be 1.                       */
 int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C,
                           int iters, int warmup, double* ms_per_eval,
                           double* ms_kernel);
@@ -122,7 +148,8 @@ int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C,
 /* Tuning knobs of the current kernel configuration (for reports):
  * lanes per row, slots per lane, blocks, threads per block.           */
 int sx_glm_kernel_config(const sx_glm* m, int* lanes_per_row,
-                         int* slots_per_lane, int* blocks, int* threads);
+                         int* slots_per_lane, int* blocks, int* threads,
+                         int* prefetch);
 
 /* ---------------- sampler service ---------------- */
 /* Arguments of hmc_nuts_diag_e_adapt in the reference's order

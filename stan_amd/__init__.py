@@ -19,7 +19,7 @@ FP64, FP32, BF16X2 = 0, 1, 2
 
 # every entry point declared in include/stanx.h
 EXPORTED = [
-    "sx_glm_create", "sx_glm_destroy", "sx_glm_num_params", "sx_glm_family",
+    "sx_glm_create", "sx_glm_create_ex", "sx_glm_tuning_log", "sx_glm_destroy", "sx_glm_num_params", "sx_glm_family",
     "sx_last_error", "sx_glm_logp_grad", "sx_glm_logp", "sx_glm_logp_grad_batch",
     "sx_glm_time_logp_grad", "sx_glm_kernel_config", "sx_hmc_nuts_diag_e_adapt",
     "sx_hmc_nuts_diag_e_adapt_cb",
@@ -41,6 +41,11 @@ class DomainError(StanxError, ValueError):
 class Prior(C.Structure):
     _fields_ = [("alpha_sd", C.c_double), ("beta_sd", C.c_double),
                 ("sigma_sd", C.c_double), ("tau_sd", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lanes_per_row", C.c_int), ("slots_cap", C.c_int), ("prefetch", C.c_int),
+                ("blocks", C.c_int), ("autotune", C.c_int)]
 
 
 class NutsConfig(C.Structure):
@@ -74,6 +79,10 @@ def lib():
         vp, i64, dp = C.c_void_p, C.c_int64, C.POINTER(C.c_double)
         L.sx_glm_create.argtypes = [C.POINTER(vp), C.c_int, vp, vp, i64, i64, vp, i64,
                                     C.c_int, C.POINTER(Prior), C.c_int]
+        L.sx_glm_create_ex.argtypes = [C.POINTER(vp), C.c_int, vp, vp, i64, i64, vp, i64,
+                                       C.c_int, C.POINTER(Prior), C.c_int, C.POINTER(Tuning)]
+        L.sx_glm_tuning_log.argtypes = [vp]
+        L.sx_glm_tuning_log.restype = C.c_char_p
         L.sx_glm_destroy.argtypes = [vp]
         L.sx_glm_destroy.restype = None
         L.sx_glm_num_params.argtypes = [vp, C.POINTER(i64)]
@@ -83,7 +92,7 @@ def lib():
         L.sx_glm_logp.argtypes = [vp, vp, C.c_int, C.c_int, dp]
         L.sx_glm_logp_grad_batch.argtypes = [vp, vp, i64, C.c_int, C.c_int, vp, vp]
         L.sx_glm_time_logp_grad.argtypes = [vp, vp, i64, C.c_int, C.c_int, dp, dp]
-        L.sx_glm_kernel_config.argtypes = [vp] + [C.POINTER(C.c_int)] * 4
+        L.sx_glm_kernel_config.argtypes = [vp] + [C.POINTER(C.c_int)] * 5
         L.sx_hmc_nuts_diag_e_adapt.argtypes = [vp, C.POINTER(NutsConfig), vp, vp,
                                                C.POINTER(NutsResult)]
         L.sx_hmc_nuts_diag_e_adapt_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
@@ -118,7 +127,7 @@ class GlmModel:
     """
 
     def __init__(self, family, X, y, offsets=None, precision=FP64,
-                 prior=(5.0, 2.0, 2.0, 1.0), device=0):
+                 prior=(5.0, 2.0, 2.0, 1.0), device=0, tuning=None):
         X = np.ascontiguousarray(X, dtype=np.float64)
         if X.ndim != 2:
             raise ValueError("X must be N x K")
@@ -137,8 +146,12 @@ class GlmModel:
         self._h = C.c_void_p()
         self.family, self.N, self.K, self.J = family, N, K, J
         self.prior = Prior(*prior)
-        _check(lib().sx_glm_create(C.byref(self._h), family, _ptr(X), _ptr(y), N, K,
-                                   _ptr(off), J, precision, C.byref(self.prior), device))
+        tn = None if tuning is None else Tuning(**{**dict(lanes_per_row=0, slots_cap=0,
+                                                             prefetch=-1, blocks=0,
+                                                             autotune=0), **tuning})
+        _check(lib().sx_glm_create_ex(C.byref(self._h), family, _ptr(X), _ptr(y), N, K,
+                                      _ptr(off), J, precision, C.byref(self.prior), device,
+                                      None if tn is None else C.byref(tn)))
         D = C.c_int64()
         _check(lib().sx_glm_num_params(self._h, C.byref(D)))
         self.D = D.value
@@ -192,10 +205,13 @@ class GlmModel:
         return ms.value, kms.value
 
     def kernel_config(self):
-        v = [C.c_int() for _ in range(4)]
+        v = [C.c_int() for _ in range(5)]
         _check(lib().sx_glm_kernel_config(self._h, *[C.byref(x) for x in v]))
         return dict(lanes_per_row=v[0].value, slots_per_lane=v[1].value,
-                    blocks=v[2].value, threads=v[3].value)
+                    blocks=v[2].value, threads=v[3].value, prefetch=v[4].value)
+
+    def tuning_log(self):
+        return lib().sx_glm_tuning_log(self._h).decode()
 
 
 def nuts_config(**kw):

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -26,45 +27,110 @@ using sx::set_error;
 
 namespace {
 
-int choose_geom(sx_glm* m, int L_req, int smax_req) {
-  int L = L_req, smax = smax_req;
-  if (L <= 0) {
-    // widest register tile that still fits: fewest lanes per row with
-    // ceil(K / L) <= 32
-    L = 1;
-    while (L < 64 && (m->K + L - 1) / L > 32) L <<= 1;
-  }
-  const int64_t S = (m->K + L - 1) / L;
-  if (smax <= 0) {
-    smax = 4;
-    while (smax < S) smax <<= 1;
-  }
-  if (S > smax || smax > 32 || (L & (L - 1)) != 0 || L > 64) {
-    set_error("unsupported tile configuration for K=" + std::to_string(m->K));
-    return SX_EINVAL;
-  }
-  m->geom = sx::make_geom(m->N, m->K, L, smax);
-  return SX_OK;
+// compiled register capacities are the multiples of 4 up to 32
+int smax_for(int64_t S) {
+  int smax = 4;
+  while (smax < S) smax += 4;
+  return smax;
 }
 
-int set_blocks(sx_glm* m, int blocks_req) {
+// lanes per row: fewest lanes whose register tile fits 32 slots
+int default_lanes(int64_t K) {
+  int L = 1;
+  while (L < 64 && (K + L - 1) / L > 32) L <<= 1;
+  return L;
+}
+
+int device_cus(int* cus) {
   int dev = 0;
   SX_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   SX_HIP(hipGetDeviceProperties(&prop, dev));
-  const int per_cu = sx::max_blocks_per_cu(m->family, m->geom.smax, m->threads);
-  int blocks = blocks_req > 0 ? blocks_req : prop.multiProcessorCount * per_cu;
-  const int64_t waves_needed = m->geom.T;  // one tile per wave at least
-  const int64_t max_blocks = std::max<int64_t>(1, (waves_needed + 3) / 4);
-  blocks = static_cast<int>(std::min<int64_t>(blocks, max_blocks));
-  m->blocks = std::max(1, blocks);
+  *cus = prop.multiProcessorCount;
   return SX_OK;
 }
 
-int alloc_work(sx_glm* m) {
-  const int E = static_cast<int>(m->K) + 3;
-  if (m->d_partials) (void)hipFree(m->d_partials);
-  SX_HIP(hipMalloc(&m->d_partials, sizeof(double) * static_cast<size_t>(m->blocks) * E));
+int clamp_blocks(const sx_glm* m, int blocks) {
+  const int64_t max_blocks = std::max<int64_t>(1, (m->geom.T + 3) / 4);
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(blocks, max_blocks)));
+}
+
+int retile(sx_glm* m, const double* d_rowmajor, int L, int smax, int prefetch) {
+  sx::tile_geom g = sx::make_geom(m->N, m->K, L, smax);
+  g.prefetch = prefetch;
+  const size_t need = static_cast<size_t>(g.T) * g.S * 64;
+  const size_t have = static_cast<size_t>(m->geom.T) * m->geom.S * 64;
+  if (need != have || (!m->d_X && need)) {
+    if (m->d_X) (void)hipFree(m->d_X);
+    m->d_X = nullptr;
+    if (need) SX_HIP(hipMalloc(&m->d_X, need * sizeof(double)));
+  }
+  if (need) SX_HIP(sx::launch_tile_x_f64(d_rowmajor, m->d_X, m->N, m->K, g, m->stream));
+  m->geom = g;
+  return SX_OK;
+}
+
+}  // namespace
+
+// forward declarations used by the autotuner
+namespace {
+int enqueue_pass(sx_glm* m, int propto, int jacobian, bool reduce = true);
+}
+
+namespace {
+
+// Times candidate layouts / launch shapes on the device and keeps the
+// fastest.  X is re-tiled from the row-major staging copy per layout.
+int autotune(sx_glm* m, const double* d_rowmajor, int cus) {
+  std::string log;
+  std::vector<double> zeros(static_cast<size_t>(m->D), 0.0);
+  SX_HIP(hipMemcpy(m->d_theta, zeros.data(), zeros.size() * sizeof(double),
+                   hipMemcpyHostToDevice));
+  struct cand { int L, smax, pf, blocks; double ms; };
+  cand best{0, 0, 0, 0, 1e30};
+  for (int L = 1; L <= 64; L <<= 1) {
+    const int64_t S = (m->K + L - 1) / L;
+    if (S > 32) continue;
+    const double waste = static_cast<double>(S * L - m->K) / std::max<int64_t>(1, m->K);
+    if (waste > 0.15 && L != default_lanes(m->K)) continue;
+    const int smax = smax_for(S);
+    for (int pf = 0; pf <= 1; ++pf) {
+      int rc = retile(m, d_rowmajor, L, smax, pf);
+      if (rc) return rc;
+      for (int mult : {1, 2, 3, 4, 6, 8}) {
+        m->blocks = clamp_blocks(m, mult * cus);
+        if (mult > 1 && m->blocks < mult * cus) break;
+        for (int w = 0; w < 2; ++w) {
+          rc = enqueue_pass(m, 1, 1);
+          if (rc) return rc;
+        }
+        SX_HIP(hipEventRecord(m->ev0, m->stream));
+        const int reps = 5;
+        for (int i = 0; i < reps; ++i) {
+          rc = enqueue_pass(m, 1, 1);
+          if (rc) return rc;
+        }
+        SX_HIP(hipEventRecord(m->ev1, m->stream));
+        SX_HIP(hipEventSynchronize(m->ev1));
+        float ms = 0.f;
+        SX_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        const double per = ms / reps;
+        char line[160];
+        std::snprintf(line, sizeof line, "L=%d S=%d smax=%d prefetch=%d blocks=%d: %.4f ms\n",
+                      L, static_cast<int>(S), smax, pf, m->blocks, per);
+        log += line;
+        if (per < best.ms) best = cand{L, smax, pf, m->blocks, per};
+      }
+    }
+  }
+  int rc = retile(m, d_rowmajor, best.L, best.smax, best.pf);
+  if (rc) return rc;
+  m->blocks = best.blocks;
+  char line[160];
+  std::snprintf(line, sizeof line, "chosen: L=%d smax=%d prefetch=%d blocks=%d (%.4f ms)\n",
+                best.L, best.smax, best.pf, best.blocks, best.ms);
+  log += line;
+  m->tuning_log = log;
   return SX_OK;
 }
 
@@ -77,6 +143,13 @@ const char* sx_last_error(void) { return sx::g_last_error.c_str(); }
 int sx_glm_create(sx_glm** out, int family, const double* X, const void* y, int64_t N,
                   int64_t K, const int64_t* group_offsets, int64_t J, int precision,
                   const sx_prior* prior, int device) {
+  return sx_glm_create_ex(out, family, X, y, N, K, group_offsets, J, precision, prior, device,
+                          nullptr);
+}
+
+int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, int64_t N,
+                     int64_t K, const int64_t* group_offsets, int64_t J, int precision,
+                     const sx_prior* prior, int device, const sx_tuning* tuning) {
   if (!out) {
     set_error("sx_glm_create: out is NULL");
     return SX_EINVAL;
@@ -95,7 +168,15 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y, int6
     return SX_EINVAL;
   }
   (void)group_offsets;
-  (void)J;
+  sx_tuning tune{0, 0, -1, 0, -1};
+  if (tuning) tune = *tuning;
+  if (tune.lanes_per_row < 0 || tune.lanes_per_row > 64 ||
+      (tune.lanes_per_row & (tune.lanes_per_row - 1)) != 0 ||
+      tune.slots_cap < 0 || tune.slots_cap > 32 || tune.slots_cap % 4 != 0 ||
+      tune.prefetch < -1 || tune.prefetch > 1 || tune.blocks < 0) {
+    set_error("sx_glm_create: invalid tuning");
+    return SX_EINVAL;
+  }
   // data checks the reference's GLM functions perform (y in {0,1},
   // finite y and X) happen once here instead of per evaluation
   if (family == SX_LOGIT) {
@@ -126,6 +207,9 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y, int6
     return SX_EINVAL;
   }
   SX_HIP(hipSetDevice(device));
+  int cus = 0;
+  int rc = device_cus(&cus);
+  if (rc) return rc;
   sx_glm* m = new sx_glm();
   m->family = family;
   m->precision = precision;
@@ -134,13 +218,7 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y, int6
   m->K = K;
   m->J = J;
   m->D = family == SX_LOGIT ? K + 1 : K + 2;
-  if (prior) m->prior = *prior;
-  else m->prior = sx_prior{5.0, 2.0, 2.0, 1.0};
-  int rc = choose_geom(m, 0, 0);
-  if (rc) {
-    delete m;
-    return rc;
-  }
+  m->prior = prior ? *prior : sx_prior{5.0, 2.0, 2.0, 1.0};
   auto fail = [&](int code) {
     sx_glm_destroy(m);
     return code;
@@ -149,28 +227,6 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y, int6
     set_error("hipStreamCreate failed");
     delete m;
     return SX_EDEVICE;
-  }
-  const sx::tile_geom& g = m->geom;
-  const size_t xt_elems = static_cast<size_t>(g.T) * g.S * 64;
-  if (xt_elems && hipMalloc(&m->d_X, xt_elems * sizeof(double)) != hipSuccess) {
-    set_error("hipMalloc(X) failed");
-    return fail(SX_ENOMEM);
-  }
-  if (xt_elems) {
-    double* tmp = nullptr;
-    if (hipMalloc(&tmp, static_cast<size_t>(N * K) * sizeof(double)) != hipSuccess) {
-      set_error("hipMalloc(staging X) failed");
-      return fail(SX_ENOMEM);
-    }
-    hipError_t e = hipMemcpy(tmp, X, static_cast<size_t>(N * K) * sizeof(double),
-                             hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = sx::launch_tile_x_f64(tmp, m->d_X, N, K, g, m->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess) {
-      set_error(std::string("X upload: ") + hipGetErrorString(e));
-      return fail(SX_EDEVICE);
-    }
   }
   if (family == SX_LOGIT) {
     std::vector<int8_t> y8(static_cast<size_t>(std::max<int64_t>(N, 1)));
@@ -187,16 +243,55 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y, int6
   }
   if (hipMalloc(&m->d_theta, sizeof(double) * (m->D + 1)) != hipSuccess) return fail(SX_ENOMEM);
   if (hipMalloc(&m->d_out, sizeof(double) * (m->D + 2)) != hipSuccess) return fail(SX_ENOMEM);
-  if (hipMalloc(&m->d_counter, sizeof(unsigned)) != hipSuccess) return fail(SX_ENOMEM);
-  if (hipMemset(m->d_counter, 0, sizeof(unsigned)) != hipSuccess) return fail(SX_EDEVICE);
   if (hipHostMalloc(&m->h_theta, sizeof(double) * (m->D + 1)) != hipSuccess) return fail(SX_ENOMEM);
   if (hipHostMalloc(&m->h_out, sizeof(double) * (m->D + 2)) != hipSuccess) return fail(SX_ENOMEM);
   if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
     return fail(SX_EDEVICE);
-  if ((rc = set_blocks(m, 0)) || (rc = alloc_work(m))) return fail(rc);
+  const int max_blocks = 8 * cus;
+  const size_t E = static_cast<size_t>(K) + 3;
+  if (hipMalloc(&m->d_partials, sizeof(double) * max_blocks * E) != hipSuccess)
+    return fail(SX_ENOMEM);
+  m->max_blocks = max_blocks;
+
+  // row-major staging copy on the device, tiled into the kernel layout
+  double* staging = nullptr;
+  if (N * K > 0) {
+    if (hipMalloc(&staging, static_cast<size_t>(N * K) * sizeof(double)) != hipSuccess) {
+      set_error("hipMalloc(staging X) failed");
+      return fail(SX_ENOMEM);
+    }
+    if (hipMemcpy(staging, X, static_cast<size_t>(N * K) * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(staging);
+      set_error("X upload failed");
+      return fail(SX_EDEVICE);
+    }
+  }
+  const bool do_tune =
+      tune.autotune == 1 || (tune.autotune == -1 && tuning == nullptr && N * K >= (1 << 22));
+  if (do_tune && N * K > 0) {
+    rc = autotune(m, staging, cus);
+  } else {
+    const int L = tune.lanes_per_row > 0 ? tune.lanes_per_row : default_lanes(K);
+    const int64_t S = (K + L - 1) / L;
+    const int smax = tune.slots_cap > 0 ? tune.slots_cap : smax_for(S);
+    if (S > smax) {
+      set_error("sx_glm_create: K does not fit lanes_per_row x slots_cap");
+      rc = SX_EINVAL;
+    } else {
+      rc = retile(m, staging, L, smax, tune.prefetch == 1 ? 1 : 0);
+      m->blocks = clamp_blocks(m, tune.blocks > 0 ? std::min(tune.blocks, max_blocks)
+                                                  : 2 * cus);
+    }
+  }
+  if (rc == SX_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = SX_EDEVICE;
+  if (staging) (void)hipFree(staging);
+  if (rc) return fail(rc);
   *out = m;
   return SX_OK;
 }
+
+const char* sx_glm_tuning_log(const sx_glm* m) { return m ? m->tuning_log.c_str() : ""; }
 
 void sx_glm_destroy(sx_glm* m) {
   if (!m) return;
@@ -206,7 +301,6 @@ void sx_glm_destroy(sx_glm* m) {
   (void)hipFree(m->d_y);
   (void)hipFree(m->d_theta);
   (void)hipFree(m->d_partials);
-  (void)hipFree(m->d_counter);
   (void)hipFree(m->d_out);
   if (m->h_theta) (void)hipHostFree(m->h_theta);
   if (m->h_out) (void)hipHostFree(m->h_out);
@@ -231,13 +325,12 @@ int sx_glm_family(const sx_glm* m) { return m ? m->family : -1; }
 
 namespace {
 
-int enqueue_pass(sx_glm* m, int propto, int jacobian) {
+int enqueue_pass(sx_glm* m, int propto, int jacobian, bool reduce) {
   sx::pass_args a;
   a.X = m->d_X;
   a.y = m->d_y;
   a.theta = m->d_theta;
   a.partials = m->d_partials;
-  a.counter = m->d_counter;
   a.out = m->d_out;
   a.N = m->N;
   a.T = m->geom.T;
@@ -252,7 +345,11 @@ int enqueue_pass(sx_glm* m, int propto, int jacobian) {
   a.alpha_sd = m->prior.alpha_sd;
   a.beta_sd = m->prior.beta_sd;
   a.sigma_sd = m->prior.sigma_sd;
-  SX_HIP(sx::launch_pass_f64(m->family, m->geom, a, m->blocks, m->threads, m->stream));
+  const double half_log_2pi = 0.91893853320467274178032973640562;
+  a.alpha_const = a.alpha_sd > 0 ? -std::log(a.alpha_sd) - half_log_2pi : 0.0;
+  a.beta_const = a.beta_sd > 0 ? -std::log(a.beta_sd) - half_log_2pi : 0.0;
+  a.sigma_const = a.sigma_sd > 0 ? -std::log(a.sigma_sd) - half_log_2pi : 0.0;
+  SX_HIP(sx::launch_pass_f64(m->family, m->geom, a, m->blocks, m->threads, m->stream, reduce));
   return SX_OK;
 }
 
@@ -335,6 +432,7 @@ int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C, int iters, 
     if (rc) return rc;
   }
   SX_HIP(hipStreamSynchronize(m->stream));
+  // full evaluations: bulk pass + reduction/finalize launches
   SX_HIP(hipEventRecord(m->ev0, m->stream));
   for (int i = 0; i < iters; ++i) {
     int rc = enqueue_pass(m, 1, 1);
@@ -344,14 +442,22 @@ int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C, int iters, 
   SX_HIP(hipEventSynchronize(m->ev1));
   float ms = 0.f;
   SX_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-  // the pass is a single kernel: per-launch time is the kernel time
   if (ms_per_eval) *ms_per_eval = ms / iters;
+  // the bulk X pass alone (its partials are left unreduced)
+  SX_HIP(hipEventRecord(m->ev0, m->stream));
+  for (int i = 0; i < iters; ++i) {
+    int rc = enqueue_pass(m, 1, 1, false);
+    if (rc) return rc;
+  }
+  SX_HIP(hipEventRecord(m->ev1, m->stream));
+  SX_HIP(hipEventSynchronize(m->ev1));
+  SX_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
   if (ms_kernel) *ms_kernel = ms / iters;
   return SX_OK;
 }
 
 int sx_glm_kernel_config(const sx_glm* m, int* lanes_per_row, int* slots_per_lane,
-                         int* blocks, int* threads) {
+                         int* blocks, int* threads, int* prefetch) {
   if (!m) {
     set_error("NULL model");
     return SX_EINVAL;
@@ -360,6 +466,7 @@ int sx_glm_kernel_config(const sx_glm* m, int* lanes_per_row, int* slots_per_lan
   if (slots_per_lane) *slots_per_lane = m->geom.S;
   if (blocks) *blocks = m->blocks;
   if (threads) *threads = m->threads;
+  if (prefetch) *prefetch = m->geom.prefetch;
   return SX_OK;
 }
 

@@ -10,10 +10,9 @@
 // each wave loads a 64-row-block tile of X into registers (one 512-byte
 // coalesced load per slot), forms eta with an L-lane shuffle reduction,
 // evaluates the link, and accumulates x * r into per-lane FP64 registers.
-// Block partials are summed in a fixed order by the last block to finish
-// (ticket counter), so results are bitwise reproducible, and the same
-// block adds the priors and the log-Jacobian and writes lp and the
-// gradient.  This replaces the two GEMV passes over X that the reference's
+// Block partials are summed in a fixed order by a second one-block launch,
+// so results are bitwise reproducible, and that launch adds the priors and
+// the log-Jacobian and writes lp and the gradient.  This replaces the two GEMV passes over X that the reference's
 // CPU GLM performs per gradient (Stan Math bernoulli_logit_glm_lpmf /
 // normal_id_glm_lpdf, reached through src/stan/model/gradient.hpp:21-35).
 #include <hip/hip_runtime.h>
@@ -21,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "../include/stanx/sx_math.hpp"
 #include "sx_internal.hpp"
 
 namespace sx {
@@ -28,14 +28,6 @@ namespace sx {
 namespace {
 
 constexpr double kHalfLog2Pi = 0.91893853320467274178032973640562;
-
-__device__ __forceinline__ double log1p_exp_d(double x) {
-  return x > 0.0 ? x + log1p(exp(-x)) : log1p(exp(x));
-}
-
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Sum of v over the 64 lanes (xor butterfly): every lane gets the total.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -46,7 +38,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Adds priors, Jacobian and scaling to the reduced statistics st[0..E) and
 // writes out[0] = lp, out[1..D] = gradient, out[D+1] = status (0 ok,
-// 1 domain error).  Runs on the first wave of the last block.
+// 1 domain error).  Runs on the first wave of the reduction launch.
 template <int FAM>
 __device__ void finalize(const pass_args& a, const double* st) {
   const int lane = threadIdx.x & 63;
@@ -65,7 +57,7 @@ __device__ void finalize(const pass_args& a, const double* st) {
     if (a.beta_sd > 0) {
       const double z = b / a.beta_sd;
       lp_prior += -0.5 * z * z;
-      if (!a.propto) lp_prior += -log(a.beta_sd) - kHalfLog2Pi;
+      if (!a.propto) lp_prior += a.beta_const;
       g += -b / (a.beta_sd * a.beta_sd);
     }
     a.out[2 + k] = g;
@@ -88,7 +80,7 @@ __device__ void finalize(const pass_args& a, const double* st) {
     if (a.sigma_sd > 0) {
       const double z = sigma / a.sigma_sd;
       lp += -0.5 * z * z;
-      if (!a.propto) lp += -log(a.sigma_sd) - kHalfLog2Pi;
+      if (!a.propto) lp += a.sigma_const;
       gu += -z * z;
     }
     if (a.jacobian) {
@@ -100,7 +92,7 @@ __device__ void finalize(const pass_args& a, const double* st) {
   if (a.alpha_sd > 0) {
     const double z = alpha / a.alpha_sd;
     lp += -0.5 * z * z;
-    if (!a.propto) lp += -log(a.alpha_sd) - kHalfLog2Pi;
+    if (!a.propto) lp += a.alpha_const;
     ga += -alpha / (a.alpha_sd * a.alpha_sd);
   }
   lp += lp_prior;
@@ -110,12 +102,52 @@ __device__ void finalize(const pass_args& a, const double* st) {
   a.out[a.D + 1] = bad > 0 ? 1.0 : 0.0;
 }
 
-template <int SMAX, int FAM>
+// Buffer-resource loads: lanes whose offset is past `bytes` read zero and
+// generate no memory request, so a tile's SMAX slot loads need no
+// branches when the layout only holds S <= SMAX slots.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base,
+                                                            uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0),
+                                           static_cast<int>(bytes), 0x00020000);
+}
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                          uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
+
+template <int SMAX>
+__device__ __forceinline__ void load_tile(const pass_args& a, int64_t t, int lane,
+                                          double (&x)[SMAX]) {
+  const __amdgpu_buffer_rsrc_t rs =
+      make_rsrc(a.X + t * static_cast<int64_t>(a.S) * 64, static_cast<uint32_t>(a.S) * 512u);
+#pragma unroll
+  for (int j = 0; j < SMAX; ++j) x[j] = buf_f64(rs, lane * 8u, j * 512u);
+}
+
+// y of this lane's row (zero past N); issued before the tile's X loads so
+// that waiting for it leaves the X loads in flight
+template <int FAM>
+__device__ __forceinline__ double load_y(const pass_args& a, int64_t t, int R, int r) {
+  const int64_t row0 = t * R;
+  const int64_t rows = a.N - row0 < R ? a.N - row0 : R;
+  if (FAM == SX_LOGIT) {
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(static_cast<const int8_t*>(a.y) + row0, static_cast<uint32_t>(rows));
+    return static_cast<double>(__builtin_amdgcn_raw_buffer_load_b8(rs, r, 0, 0));
+  } else {
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(static_cast<const double*>(a.y) + row0, static_cast<uint32_t>(rows) * 8u);
+    return buf_f64(rs, r * 8u, 0);
+  }
+}
+
+template <int SMAX, int FAM, int PF>
 __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
   extern __shared__ double lds[];
-  __shared__ int is_last;
   const int lane = threadIdx.x & 63;
-  const int wib = threadIdx.x >> 6;
+  // the wave index is uniform across the wave: say so, so that tile
+  // indices and buffer descriptors live in scalar registers
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
   const int L = a.L;
   const int q = lane & (L - 1);
@@ -123,9 +155,9 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
   const int R = 64 >> a.lgL;
   const int S = a.S;
 
-  // beta staged in LDS, padded to S*L entries (zero beyond K)
+  // beta staged in LDS, padded to SMAX*L entries (zero beyond K)
   double* sbeta = lds + nw * a.E;
-  for (int i = threadIdx.x; i < S * L; i += blockDim.x)
+  for (int i = threadIdx.x; i < SMAX * L; i += blockDim.x)
     sbeta[i] = i < a.K ? a.theta[1 + i] : 0.0;
   __syncthreads();
   const double alpha = a.theta[0];
@@ -135,17 +167,38 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
   double s0 = 0.0, s1 = 0.0, bad = 0.0;
 
   const int64_t W = static_cast<int64_t>(gridDim.x) * nw;
-  const int64_t tile_stride = static_cast<int64_t>(S) * 64;
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * nw + wib; t < a.T; t += W) {
-    const double* base = a.X + t * tile_stride + lane;
-    double xv[SMAX];
+  int64_t t = static_cast<int64_t>(blockIdx.x) * nw + wib;
+  double xv[SMAX];
+  double yv = 0.0;
+  if (PF && t < a.T) {
+    yv = load_y<FAM>(a, t, R, r);
+    load_tile<SMAX>(a, t, lane, xv);
+  }
+  for (; t < a.T; t += W) {
+    double xn[SMAX];
+    double yn = 0.0;
+    if (PF) {
+      // the next tile's loads are in flight during this tile's arithmetic
+      const int64_t tn = t + W;
+      if (tn < a.T) {
+        yn = load_y<FAM>(a, tn, R, r);
+        load_tile<SMAX>(a, tn, lane, xn);
+      } else {
 #pragma unroll
-    for (int j = 0; j < SMAX; ++j)
-      xv[j] = (j < S) ? __builtin_nontemporal_load(base + j * 64) : 0.0;
+        for (int j = 0; j < SMAX; ++j) xn[j] = 0.0;
+      }
+    } else {
+      yv = load_y<FAM>(a, t, R, r);
+      load_tile<SMAX>(a, t, lane, xv);
+    }
+    // beta is re-read from LDS every tile: an opaque zero offset keeps the
+    // compiler from hoisting SMAX loop-invariant values into registers
+    int zoff = 0;
+    asm volatile("" : "+s"(zoff));
+    const double* sb = sbeta + zoff + q;
     double e = 0.0;
 #pragma unroll
-    for (int j = 0; j < SMAX; ++j)
-      if (j < S) e = fma(xv[j], sbeta[j * L + q], e);
+    for (int j = 0; j < SMAX; ++j) e = fma(xv[j], sb[j * L], e);
     for (int off = 1; off < L; off <<= 1) e += __shfl_xor(e, off, 64);
     const int64_t row = t * R + r;
     const bool valid = row < a.N;
@@ -153,13 +206,9 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
     double rr = 0.0, term = 0.0;
     if (valid) {
       if (FAM == SX_LOGIT) {
-        const int yv = static_cast<const int8_t*>(a.y)[row];
-        const double s = yv ? 1.0 : -1.0;
-        const double se = s * e;
-        rr = s / (1.0 + exp(se));
-        term = -log1p_exp_d(-se);
+        stanx::math::sx_logit_row(e, yv != 0.0 ? 1.0 : -1.0, &term, &rr);
       } else {
-        const double d = static_cast<const double*>(a.y)[row] - e;
+        const double d = yv - e;
         rr = d;
         term = d * d;
       }
@@ -171,12 +220,18 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
     }
 #pragma unroll
     for (int j = 0; j < SMAX; ++j) acc[j] = fma(xv[j], rr, acc[j]);
+    if (PF) {
+#pragma unroll
+      for (int j = 0; j < SMAX; ++j) xv[j] = xn[j];
+      yv = yn;
+    }
   }
 
   // reduce the per-lane column partials over the R row-lanes of each column
   for (int off = L; off < 64; off <<= 1) {
 #pragma unroll
-    for (int j = 0; j < SMAX; ++j) acc[j] += __shfl_xor(acc[j], off, 64);
+    for (int j = 0; j < SMAX; ++j)
+      if (j < S) acc[j] += __shfl_xor(acc[j], off, 64);
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
@@ -195,38 +250,59 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
     lds[wib * E + a.K + 2] = bad;
   }
   __syncthreads();
+  // block partial, stored entry-major: partials[i * gridDim.x + block]
+  const int nb = gridDim.x;
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
     double v = 0.0;
     for (int w = 0; w < nw; ++w) v += lds[w * E + i];
-    a.partials[static_cast<int64_t>(blockIdx.x) * E + i] = v;
+    a.partials[static_cast<int64_t>(i) * nb + blockIdx.x] = v;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned ticket = atomicAdd(a.counter, 1u);
-    is_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // fixed-order sum of the block partials; four interleaved chains keep
-  // several loads in flight per thread
-  for (int i = threadIdx.x; i < E; i += blockDim.x) {
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    const int nb = gridDim.x;
-    int b = 0;
-    for (; b + 3 < nb; b += 4) {
-      v0 += load_coherent(a.partials + static_cast<int64_t>(b) * E + i);
-      v1 += load_coherent(a.partials + static_cast<int64_t>(b + 1) * E + i);
-      v2 += load_coherent(a.partials + static_cast<int64_t>(b + 2) * E + i);
-      v3 += load_coherent(a.partials + static_cast<int64_t>(b + 3) * E + i);
+}
+
+// Cross-block reduction of the bulk pass's partials (fixed order, so the
+// result is bitwise reproducible for a given launch shape) followed by the
+// priors, Jacobian and output.  A separate launch: the kernel boundary
+// makes every block's partials visible without per-block cache
+// write-backs.  One block of 1024 threads; wave w sums entries w, w+16, ...
+template <int FAM>
+__global__ __launch_bounds__(1024) void glm_reduce_f64(pass_args a, int nb) {
+  extern __shared__ double st[];
+  const int lane = threadIdx.x & 63;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  const int E = a.E;
+  // Each wave takes 8 entries per round; every lane issues all of its
+  // loads (8 entries x 4 blocks per step) before any add: clamped indices
+  // and 0/1 weights keep the loop free of branches.
+  for (int i0 = wib * 8; i0 < E; i0 += nw * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = 0.0;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+      double x[8][4];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = min(i0 + u, E - 1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int b = min(b0 + lane + 64 * k, nb - 1);
+          x[u][k] = a.partials[static_cast<int64_t>(i) * nb + b];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          v[u] += (b0 + lane + 64 * k < nb) ? x[u][k] : 0.0;
     }
-    for (; b < nb; ++b) v0 += load_coherent(a.partials + static_cast<int64_t>(b) * E + i);
-    lds[i] = (v0 + v1) + (v2 + v3);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double w = wave_sum(v[u]);
+      if (lane == 0 && i0 + u < E) st[i0 + u] = w;
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) *a.counter = 0u;
-  if (wib == 0) finalize<FAM>(a, lds);
+  if (wib == 0) finalize<FAM>(a, st);
 }
 
 __global__ void tile_x_f64(const double* __restrict__ Xrm, double* __restrict__ Xt,
@@ -242,34 +318,33 @@ __global__ void tile_x_f64(const double* __restrict__ Xrm, double* __restrict__ 
   Xt[idx] = (row < N && col < K) ? Xrm[row * K + col] : 0.0;
 }
 
-template <int SMAX, int FAM>
-hipError_t launch_one(const pass_args& a, int blocks, int threads, hipStream_t s) {
+template <int SMAX, int FAM, int PF>
+hipError_t launch_one(const pass_args& a, int blocks, int threads, hipStream_t s,
+                      bool reduce) {
   const size_t lds =
-      (static_cast<size_t>(threads / 64) * a.E + static_cast<size_t>(a.S) * a.L) * sizeof(double);
-  hipLaunchKernelGGL((glm_pass_f64<SMAX, FAM>), dim3(blocks), dim3(threads), lds, s, a);
+      (static_cast<size_t>(threads / 64) * a.E + static_cast<size_t>(SMAX) * a.L) * sizeof(double);
+  hipLaunchKernelGGL((glm_pass_f64<SMAX, FAM, PF>), dim3(blocks), dim3(threads), lds, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !reduce) return e;
+  hipLaunchKernelGGL((glm_reduce_f64<FAM>), dim3(1), dim3(1024), a.E * sizeof(double), s, a,
+                     blocks);
   return hipGetLastError();
 }
 
-template <int FAM>
-hipError_t launch_fam(int smax, const pass_args& a, int blocks, int threads,
-                      hipStream_t s) {
+template <int FAM, int PF>
+hipError_t launch_fam(int smax, const pass_args& a, int blocks, int threads, hipStream_t s,
+                      bool reduce) {
   switch (smax) {
-    case 4: return launch_one<4, FAM>(a, blocks, threads, s);
-    case 8: return launch_one<8, FAM>(a, blocks, threads, s);
-    case 16: return launch_one<16, FAM>(a, blocks, threads, s);
-    case 32: return launch_one<32, FAM>(a, blocks, threads, s);
+    case 4: return launch_one<4, FAM, PF>(a, blocks, threads, s, reduce);
+    case 8: return launch_one<8, FAM, PF>(a, blocks, threads, s, reduce);
+    case 12: return launch_one<12, FAM, PF>(a, blocks, threads, s, reduce);
+    case 16: return launch_one<16, FAM, PF>(a, blocks, threads, s, reduce);
+    case 20: return launch_one<20, FAM, PF>(a, blocks, threads, s, reduce);
+    case 24: return launch_one<24, FAM, PF>(a, blocks, threads, s, reduce);
+    case 28: return launch_one<28, FAM, PF>(a, blocks, threads, s, reduce);
+    case 32: return launch_one<32, FAM, PF>(a, blocks, threads, s, reduce);
   }
   return hipErrorInvalidValue;
-}
-
-template <int SMAX, int FAM>
-int occ_one(int threads) {
-  int nb = 0;
-  const size_t lds = (static_cast<size_t>(threads / 64) * 515 + 1024) * sizeof(double);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, glm_pass_f64<SMAX, FAM>, threads,
-                                                   lds) != hipSuccess)
-    return 1;
-  return nb > 0 ? nb : 1;
 }
 
 }  // namespace
@@ -287,9 +362,13 @@ tile_geom make_geom(int64_t N, int64_t K, int L, int smax) {
 }
 
 hipError_t launch_pass_f64(int family, const tile_geom& g, const pass_args& a,
-                           int blocks, int threads, hipStream_t s) {
-  if (family == SX_LOGIT) return launch_fam<SX_LOGIT>(g.smax, a, blocks, threads, s);
-  if (family == SX_NORMAL) return launch_fam<SX_NORMAL>(g.smax, a, blocks, threads, s);
+                           int blocks, int threads, hipStream_t s, bool reduce) {
+  if (family == SX_LOGIT)
+    return g.prefetch ? launch_fam<SX_LOGIT, 1>(g.smax, a, blocks, threads, s, reduce)
+                      : launch_fam<SX_LOGIT, 0>(g.smax, a, blocks, threads, s, reduce);
+  if (family == SX_NORMAL)
+    return g.prefetch ? launch_fam<SX_NORMAL, 1>(g.smax, a, blocks, threads, s, reduce)
+                      : launch_fam<SX_NORMAL, 0>(g.smax, a, blocks, threads, s, reduce);
   return hipErrorInvalidValue;
 }
 
@@ -302,23 +381,6 @@ hipError_t launch_tile_x_f64(const double* Xrm, double* Xt, int64_t N, int64_t K
   hipLaunchKernelGGL(tile_x_f64, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, s,
                      Xrm, Xt, N, K, g.L, g.lgL, g.S, total);
   return hipGetLastError();
-}
-
-int max_blocks_per_cu(int family, int smax, int threads) {
-  if (family == SX_NORMAL) {
-    switch (smax) {
-      case 4: return occ_one<4, SX_NORMAL>(threads);
-      case 8: return occ_one<8, SX_NORMAL>(threads);
-      case 16: return occ_one<16, SX_NORMAL>(threads);
-      default: return occ_one<32, SX_NORMAL>(threads);
-    }
-  }
-  switch (smax) {
-    case 4: return occ_one<4, SX_LOGIT>(threads);
-    case 8: return occ_one<8, SX_LOGIT>(threads);
-    case 16: return occ_one<16, SX_LOGIT>(threads);
-    default: return occ_one<32, SX_LOGIT>(threads);
-  }
 }
 
 }  // namespace sx

@@ -33,6 +33,7 @@ struct tile_geom {
   int R = 64;      // rows per tile
   int S = 0;       // slots per lane
   int smax = 8;    // compiled register capacity (>= S)
+  int prefetch = 0;  // 1: next tile's loads issued before this tile's math
   int64_t T = 0;   // number of tiles
 };
 
@@ -44,20 +45,22 @@ struct pass_args {
   const void* y;        // int8 (logistic) or double (normal)
   const double* theta;  // D doubles on the device
   double* partials;     // blocks x E
-  unsigned* counter;    // ticket for the last-block reduction
   double* out;          // lp, grad[D], status
   int64_t N;
   int64_t T;
   int K, L, lgL, S, D, E;
   int propto, jacobian;
   double alpha_sd, beta_sd, sigma_sd;
+  // normalising constants -log(sd) - log(2 pi)/2 of the normal priors
+  double alpha_const, beta_const, sigma_const;
 };
 
+// Bulk X pass; with reduce, also the cross-block reduction + finalize
+// launch that writes out[].
 hipError_t launch_pass_f64(int family, const tile_geom& g, const pass_args& a,
-                           int blocks, int threads, hipStream_t s);
+                           int blocks, int threads, hipStream_t s, bool reduce = true);
 hipError_t launch_tile_x_f64(const double* Xrm, double* Xt, int64_t N,
                              int64_t K, const tile_geom& g, hipStream_t s);
-int max_blocks_per_cu(int family, int smax, int threads);
 
 }  // namespace sx
 
@@ -71,11 +74,11 @@ struct sx_glm {
   void* d_y = nullptr;
   double* d_theta = nullptr;
   double* d_partials = nullptr;
-  unsigned* d_counter = nullptr;
   double* d_out = nullptr;
   double* h_theta = nullptr;  // pinned
   double* h_out = nullptr;    // pinned
-  int blocks = 0, threads = 256;
+  int blocks = 0, threads = 256, max_blocks = 0;
+  std::string tuning_log;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 

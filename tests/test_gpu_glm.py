@@ -61,6 +61,40 @@ def test_normal_parity_shapes(N, K):
         m.close()
 
 
+@pytest.mark.parametrize("family", [S.LOGIT, S.NORMAL])
+@pytest.mark.parametrize("lanes,cap", [(1, 32), (2, 24), (4, 28), (4, 12), (8, 16), (8, 8),
+                                       (16, 4), (32, 4), (64, 4), (2, 32), (1, 20)])
+@pytest.mark.parametrize("prefetch", [0, 1])
+def test_every_kernel_variant(family, lanes, cap, prefetch):
+    """Each layout / register capacity / prefetch variant, with an odd block
+    count so that waves own different numbers of tiles."""
+    K = 45 if lanes * cap >= 45 else lanes * cap
+    N = 3001
+    _say(f"variant family={family} L={lanes} cap={cap} pf={prefetch} K={K}")
+    d = W.logistic(N, K, 0.3, 0.5, seed=K) if family == S.LOGIT else W.normal(N, K, seed=K)
+    m = S.GlmModel(family, d["X"], d["y"], prior=d["prior"],
+                   tuning=dict(lanes_per_row=lanes, slots_cap=cap, prefetch=prefetch,
+                               blocks=7))
+    cfg = m.kernel_config()
+    assert cfg["lanes_per_row"] == lanes and cfg["prefetch"] == prefetch
+    ref = O.GlmData(family, d["X"], d["y"], prior=d["prior"])
+    for th in W.random_thetas(m.D, 2, seed=lanes + cap) * 0.5:
+        _check(m, ref, th)
+    m.close()
+
+
+def test_autotuned_configuration_parity():
+    d = W.logistic(60000, 100, 0.1, 0.5, seed=9)
+    m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"])
+    log = m.tuning_log()
+    _say(log)
+    assert "chosen:" in log
+    ref = O.GlmData(S.LOGIT, d["X"], d["y"], prior=d["prior"])
+    for th in W.random_thetas(m.D, 3, seed=1):
+        _check(m, ref, th)
+    m.close()
+
+
 def test_empty_data_gives_prior_only():
     X = np.zeros((0, 3))
     m = S.GlmModel(S.LOGIT, X, np.zeros(0, dtype=np.int32), prior=(5.0, 2.0, 2.0, 1.0))
