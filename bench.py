@@ -19,6 +19,12 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the bulk logistic pass at N=1e6, K=100 with the
+# lanes_per_row=4 layout, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
+# each scaled by the same counter on a plain copy of known size (fetch x2,
+# write x1).  Summary and commands: profiles/glm_pass_f64_logistic_n1e6_k100_pmc.md
+PMC_TRAFFIC_BYTES = {(4, 25): 783350 * 1024.0 + 3298 * 1024.0}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -102,8 +108,10 @@ def main():
 
     alg_bytes = 8.0 * N * K + 1.0 * N + 8.0 * m.D
     achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
+    kc = m.kernel_config()
+    traffic = PMC_TRAFFIC_BYTES.get((kc["lanes_per_row"], kc["slots_per_lane"]))
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS,
-                    unit="GB/s", frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic,
                     kernel="glm_pass_f64 (logistic)", bytes_per_launch=alg_bytes,
                     kernel_ms=ms_kernel, kernel_config=m.kernel_config())
 
