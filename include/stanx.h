@@ -209,6 +209,36 @@ int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D,
                                 const double* init_inv_metric,
                                 sx_nuts_result* out);
 
+/* ---------------- batched device-resident sampler ---------------- */
+/* `chains` chains of adaptive diag_e NUTS on the device, chain ids
+ * cfg->chain, cfg->chain + 1, ... (the reference's multi-chain service,
+ * src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:328-407, runs one
+ * thread per chain instead).  Every chain draws from its own Philox stream
+ * {seed, chain id}, the stream of the single-chain service.
+ * grad_precision: SX_FP64 (the batched FP64 pass) or SX_BF16X2 (MFMA;
+ * needs a context created with that precision).  inits: chains x D or
+ * NULL; inv_metric: D or NULL.                                          */
+typedef struct sx_nuts_batch sx_nuts_batch;
+int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m,
+                         const sx_nuts_config* cfg, int64_t chains,
+                         int grad_precision, const double* inits,
+                         const double* inv_metric);
+/* Advance all chains by up to max_steps lockstep gradient steps (stops
+ * early once every chain has finished).                                 */
+int sx_nuts_batch_run(sx_nuts_batch* b, int64_t max_steps,
+                      int64_t* steps_run, int64_t* chains_done);
+/* Rows of draws each chain can hold ((save_warmup ? warmup : 0) + samples,
+ * thinned).                                                             */
+int64_t sx_nuts_batch_rows_cap(const sx_nuts_batch* b);
+/* draws: chains x rows_cap x (7 + D), columns as in sx_nuts_result with
+ * constrained parameters; rows, status (0 / 70 / 78, -1 still running),
+ * stepsize, n_grad: per chain; inv_metric: chains x D.  Any output may be
+ * NULL.                                                                 */
+int sx_nuts_batch_results(sx_nuts_batch* b, double* draws, int64_t* rows,
+                          int* status, double* stepsize, double* inv_metric,
+                          int64_t* n_grad, double* seconds, int64_t* steps);
+void sx_nuts_batch_destroy(sx_nuts_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,8 @@ EXPORTED = [
     "sx_glm_create", "sx_glm_create_ex", "sx_glm_tuning_log", "sx_glm_destroy", "sx_glm_num_params", "sx_glm_family",
     "sx_last_error", "sx_glm_logp_grad", "sx_glm_logp", "sx_glm_logp_grad_batch",
     "sx_glm_time_logp_grad", "sx_glm_kernel_config", "sx_hmc_nuts_diag_e_adapt",
-    "sx_hmc_nuts_diag_e_adapt_cb",
+    "sx_hmc_nuts_diag_e_adapt_cb", "sx_nuts_batch_create", "sx_nuts_batch_run",
+    "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
 ]
 
 _lib = None
@@ -97,6 +98,14 @@ def lib():
                                                C.POINTER(NutsResult)]
         L.sx_hmc_nuts_diag_e_adapt_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                                   vp, vp, C.POINTER(NutsResult)]
+        L.sx_nuts_batch_create.argtypes = [C.POINTER(vp), vp, C.POINTER(NutsConfig), i64,
+                                           C.c_int, vp, vp]
+        L.sx_nuts_batch_run.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64)]
+        L.sx_nuts_batch_rows_cap.argtypes = [vp]
+        L.sx_nuts_batch_rows_cap.restype = i64
+        L.sx_nuts_batch_results.argtypes = [vp, vp, vp, vp, vp, vp, vp, dp, C.POINTER(i64)]
+        L.sx_nuts_batch_destroy.argtypes = [vp]
+        L.sx_nuts_batch_destroy.restype = None
         _lib = L
     return _lib
 
@@ -261,3 +270,65 @@ def hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
         lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, C.byref(cfg),
                                                           _ptr(i), _ptr(m), C.byref(r)),
         D, cfg, init, inv_metric)
+
+
+class NutsBatch:
+    """Many chains of adaptive diag_e NUTS resident on the device (one
+    GlmModel).  Chain ids are chain, chain + 1, ...; each chain draws from
+    the Philox stream of the single-chain service with the same id."""
+
+    def __init__(self, model, chains, grad_precision=FP64, inits=None, inv_metric=None,
+                 **kw):
+        self.model = model
+        self.C = int(chains)
+        self.cfg = nuts_config(**kw)
+        self._h = C.c_void_p()
+        init_a = None if inits is None else np.ascontiguousarray(inits, dtype=np.float64)
+        minv_a = None if inv_metric is None else np.ascontiguousarray(inv_metric,
+                                                                       dtype=np.float64)
+        _check(lib().sx_nuts_batch_create(C.byref(self._h), model.handle, C.byref(self.cfg),
+                                          self.C, grad_precision, _ptr(init_a), _ptr(minv_a)))
+        self.rows_cap = lib().sx_nuts_batch_rows_cap(self._h)
+
+    def run(self, max_steps=1 << 40):
+        steps, done = C.c_int64(), C.c_int64()
+        _check(lib().sx_nuts_batch_run(self._h, int(max_steps), C.byref(steps), C.byref(done)))
+        return steps.value, done.value
+
+    def results(self):
+        D = self.model.D
+        draws = np.zeros((self.C, max(self.rows_cap, 1), 7 + D))
+        rows = np.zeros(self.C, dtype=np.int64)
+        status = np.zeros(self.C, dtype=np.int32)
+        eps = np.zeros(self.C)
+        minv = np.zeros((self.C, D))
+        ng = np.zeros(self.C, dtype=np.int64)
+        secs = C.c_double()
+        steps = C.c_int64()
+        _check(lib().sx_nuts_batch_results(self._h, _ptr(draws), _ptr(rows), _ptr(status),
+                                           _ptr(eps), _ptr(minv), _ptr(ng), C.byref(secs),
+                                           C.byref(steps)))
+        return dict(draws=draws[:, : self.rows_cap], rows=rows, status=status, stepsize=eps,
+                    inv_metric=minv, n_grad=ng, seconds=secs.value, steps=steps.value)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sx_nuts_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hmc_nuts_diag_e_adapt_batch(model, chains, grad_precision=FP64, inits=None,
+                                inv_metric=None, **kw):
+    """Run `chains` device-resident chains to completion; see NutsBatch."""
+    b = NutsBatch(model, chains, grad_precision, inits, inv_metric, **kw)
+    try:
+        b.run()
+        return b.results()
+    finally:
+        b.close()

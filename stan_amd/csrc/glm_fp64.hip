@@ -144,6 +144,9 @@ __device__ __forceinline__ double load_y(const pass_args& a, int64_t t, int R, i
 template <int SMAX, int FAM, int PF>
 __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
   extern __shared__ double lds[];
+  // blockIdx.y selects the chain of a batched evaluation
+  a.theta += static_cast<int64_t>(blockIdx.y) * a.theta_stride;
+  a.partials += static_cast<int64_t>(blockIdx.y) * a.E * gridDim.x;
   const int lane = threadIdx.x & 63;
   // the wave index is uniform across the wave: say so, so that tile
   // indices and buffer descriptors live in scalar registers
@@ -267,6 +270,9 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
 template <int FAM>
 __global__ __launch_bounds__(1024) void glm_reduce_f64(pass_args a, int nb) {
   extern __shared__ double st[];
+  a.theta += static_cast<int64_t>(blockIdx.x) * a.theta_stride;
+  a.partials += static_cast<int64_t>(blockIdx.x) * a.E * nb;
+  a.out += static_cast<int64_t>(blockIdx.x) * a.out_stride;
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
@@ -323,11 +329,12 @@ hipError_t launch_one(const pass_args& a, int blocks, int threads, hipStream_t s
                       bool reduce) {
   const size_t lds =
       (static_cast<size_t>(threads / 64) * a.E + static_cast<size_t>(SMAX) * a.L) * sizeof(double);
-  hipLaunchKernelGGL((glm_pass_f64<SMAX, FAM, PF>), dim3(blocks), dim3(threads), lds, s, a);
+  hipLaunchKernelGGL((glm_pass_f64<SMAX, FAM, PF>), dim3(blocks, a.chains), dim3(threads), lds,
+                     s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !reduce) return e;
-  hipLaunchKernelGGL((glm_reduce_f64<FAM>), dim3(1), dim3(1024), a.E * sizeof(double), s, a,
-                     blocks);
+  hipLaunchKernelGGL((glm_reduce_f64<FAM>), dim3(a.chains), dim3(1024), a.E * sizeof(double),
+                     s, a, blocks);
   return hipGetLastError();
 }
 

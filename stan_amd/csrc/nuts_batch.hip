@@ -1,0 +1,770 @@
+// Device-resident adaptive diag_e NUTS for many chains at once.
+//
+// Every chain runs the same iterative state machine as the host sampler
+// (stanx/mcmc/base_nuts.hpp), suspended whenever it needs a gradient: one
+// launch of nuts_step advances each chain (one wave per chain) from the
+// gradient it just received to the point where it needs the next one,
+// writes that point to theta[chain], and the batched gradient launches
+// evaluate all chains together.  Chains therefore move in lockstep one
+// gradient at a time but are otherwise independent: each owns its trees,
+// adaptation and Philox stream (key {seed, chain id}, the host sampler's
+// stream), so a chain's draws do not depend on which chains run beside it.
+//
+// Semantics followed (paths relative to the stan-dev/stan 2.35.0 tree):
+//   initialisation        src/stan/services/util/initialize.hpp:68-227
+//   step-size heuristic   src/stan/mcmc/hmc/base_hmc.hpp:93-161
+//   transition            src/stan/mcmc/hmc/nuts/base_nuts.hpp:78-204
+//   build_tree            src/stan/mcmc/hmc/nuts/base_nuts.hpp:247-352
+//   leapfrog              src/stan/mcmc/hmc/integrators/expl_leapfrog.hpp:16-32
+//   adaptation            src/stan/mcmc/hmc/nuts/adapt_diag_e_nuts.hpp:25-48,
+//                         src/stan/mcmc/stepsize_adaptation.hpp:55-73,
+//                         src/stan/mcmc/windowed_adaptation.hpp:80-110,
+//                         src/stan/mcmc/var_adaptation.hpp:17-46
+//   service loop          src/stan/services/util/run_adaptive_sampler.hpp:46-103
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../include/stanx/philox.hpp"
+#include "nuts_batch.hpp"
+
+#pragma clang fp contract(off)
+
+namespace sx {
+namespace nb {
+
+namespace {
+
+constexpr double kInf = __builtin_huge_val();
+constexpr double kLog08 = -0.22314355131420976;  // log(0.8)
+
+// ---- vector slots of a chain ----
+enum : int {
+  ZQ = 0, ZP, ZG,          // current point
+  FQ, FP, FG,              // forward end of the trajectory
+  BQ, BP, BG,              // backward end
+  SQ, PQ,                  // sample, top-level proposal (positions)
+  PFB, PBF,                // p at the backward end of the forward subtree / forward end of the backward subtree
+  RHO, RHOF, RHOB, TMP,
+  IQ, IP, IG,              // z_init of the step-size heuristic
+  MINV, WM, WM2,           // metric and Welford accumulators
+  LVL0                     // per level l >= 1: PIE, RI, ZPF, PFBG, RF
+};
+static_assert(MINV == kSlotMinv, "metric slot");
+__device__ __forceinline__ int PIE(int l) { return LVL0 + 5 * (l - 1); }
+__device__ __forceinline__ int RI(int l) { return LVL0 + 5 * (l - 1) + 1; }
+__device__ __forceinline__ int ZPF(int l) { return LVL0 + 5 * (l - 1) + 2; }
+__device__ __forceinline__ int PFBG(int l) { return LVL0 + 5 * (l - 1) + 3; }
+__device__ __forceinline__ int RF(int l) { return LVL0 + 5 * (l - 1) + 4; }
+
+// scalar slots of proposal-like points (position stored in a vector slot)
+// id -1: top-level proposal PQ, 0: sample SQ, l >= 1: ZPF(l)
+struct dest {
+  int zprop;  // proposal id (-1 top, l >= 1 frame)
+  int pbeg, pend, rho;
+  int lsw;    // -1 top subtree accumulator, 2l: lsw_init[l], 2l+1: lsw_final[l]
+};
+
+struct ctx {
+  const batch_args* a;
+  chain_state* s;
+  double* v;  // chain vector base
+  int c, lane;
+  uint32_t chain_id;
+  __device__ double* slot(int i) const { return v + static_cast<int64_t>(i) * a->Dpad; }
+};
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// ---- vector helpers (lanes stride over D) ----
+__device__ void vcopy(const ctx& k, int dst, int src) {
+  double* d = k.slot(dst);
+  const double* s = k.slot(src);
+  for (int i = k.lane; i < k.a->D; i += 64) d[i] = s[i];
+}
+__device__ void vzero(const ctx& k, int dst) {
+  double* d = k.slot(dst);
+  for (int i = k.lane; i < k.a->D; i += 64) d[i] = 0.0;
+}
+// sum_i p_sharp_i * rho_i with p_sharp = minv * p
+__device__ double dot_sharp(const ctx& k, int p, int rho) {
+  const double* m = k.slot(MINV);
+  const double* pp = k.slot(p);
+  const double* r = k.slot(rho);
+  double acc = 0.0;
+  for (int i = k.lane; i < k.a->D; i += 64) acc += (m[i] * pp[i]) * r[i];
+  return wave_sum(acc);
+}
+// sum_i p_sharp_i * (r1_i + r2_i)
+__device__ double dot_sharp2(const ctx& k, int p, int r1, int r2) {
+  const double* m = k.slot(MINV);
+  const double* pp = k.slot(p);
+  const double* a = k.slot(r1);
+  const double* b = k.slot(r2);
+  double acc = 0.0;
+  for (int i = k.lane; i < k.a->D; i += 64) acc += (m[i] * pp[i]) * (a[i] + b[i]);
+  return wave_sum(acc);
+}
+__device__ double kinetic(const ctx& k, int p) {
+  const double* m = k.slot(MINV);
+  const double* pp = k.slot(p);
+  double acc = 0.0;
+  for (int i = k.lane; i < k.a->D; i += 64) acc += pp[i] * (m[i] * pp[i]);
+  return 0.5 * wave_sum(acc);
+}
+
+// ---- RNG: the chain's Philox stream, one variate per counter value ----
+// Called in wave-uniform control flow: every lane draws the same value,
+// then lane 0 advances the counter.
+__device__ double uniform(const ctx& k) {
+  const uint64_t ctr = k.s->rng;
+  const double u = stanx::philox_uniform(k.a->seed, k.chain_id, ctr);
+  __syncthreads();
+  if (k.lane == 0) k.s->rng = ctr + 1;
+  __syncthreads();
+  return u;
+}
+// diag_e_metric::sample_p: p_i = z_i / sqrt(minv_i), variate i of the call
+// from counter base + i
+__device__ void sample_p(const ctx& k) {
+  double* p = k.slot(ZP);
+  const double* m = k.slot(MINV);
+  const uint64_t base = k.s->rng;
+  for (int i = k.lane; i < k.a->D; i += 64)
+    p[i] = stanx::philox_normal(k.a->seed, k.chain_id, base + i) / sqrt(m[i]);
+  __syncthreads();
+  if (k.lane == 0) k.s->rng = base + k.a->D;
+  __syncthreads();
+}
+
+__device__ double log1p_exp(double a) { return a > 0.0 ? a + log1p(exp(-a)) : log1p(exp(a)); }
+__device__ double log_sum_exp(double a, double b) {
+  if (a == -kInf) return b;
+  if (a == kInf && b == kInf) return kInf;
+  if (a > b) return a + log1p_exp(b - a);
+  return b + log1p_exp(a - b);
+}
+
+__device__ double& lsw_ref(const ctx& k, int id) {
+  if (id < 0) return k.s->lsw_sub;
+  return (id & 1) ? k.s->lsw_final[id >> 1] : k.s->lsw_init[id >> 1];
+}
+__device__ int prop_slot(int id) { return id < 0 ? PQ : (id == 0 ? SQ : ZPF(id)); }
+__device__ double& prop_V(const ctx& k, int id) {
+  return id < 0 ? k.s->V_prop : (id == 0 ? k.s->V_samp : k.s->V_zpf[id]);
+}
+__device__ double& prop_T(const ctx& k, int id) {
+  return id < 0 ? k.s->T_prop : (id == 0 ? k.s->T_samp : k.s->T_zpf[id]);
+}
+__device__ void copy_prop(const ctx& k, int dst, int src) {
+  vcopy(k, prop_slot(dst), prop_slot(src));
+  if (k.lane == 0) {
+    prop_V(k, dst) = prop_V(k, src);
+    prop_T(k, dst) = prop_T(k, src);
+  }
+}
+
+// ---- point copies (q, p, g, V) ----
+__device__ void copy_point(const ctx& k, int dq, int sq, double& dV, double sV) {
+  vcopy(k, dq, sq);
+  vcopy(k, dq + 1, sq + 1);
+  vcopy(k, dq + 2, sq + 2);
+  if (k.lane == 0) dV = sV;
+}
+
+// ---- gradient request / consumption ----
+__device__ void request_gradient(const ctx& k, int next_phase) {
+  double* th = k.a->theta + static_cast<int64_t>(k.c) * k.a->D;
+  const double* q = k.slot(ZQ);
+  for (int i = k.lane; i < k.a->D; i += 64) th[i] = q[i];
+  if (k.lane == 0) {
+    k.s->phase = next_phase;
+    k.s->pending = 1;
+    k.s->n_grad++;
+  }
+}
+// base_hamiltonian::update_potential_gradient: V = -lp, g = -grad; on a
+// domain error V = +inf and g = -g_old (the reference flips the stale g)
+__device__ void consume_gradient(const ctx& k) {
+  const double* o = k.a->out + static_cast<int64_t>(k.c) * k.a->out_stride;
+  const bool ok = o[k.a->D + 1] == 0.0 && isfinite(o[0]);
+  double* g = k.slot(ZG);
+  for (int i = k.lane; i < k.a->D; i += 64) g[i] = ok ? -o[1 + i] : -g[i];
+  if (k.lane == 0) {
+    k.s->V_z = ok ? -o[0] : kInf;
+    k.s->last_lp = o[0];
+    k.s->last_ok = ok ? 1 : 0;
+  }
+}
+
+// leapfrog halves (expl_leapfrog): begin = p -= e/2 g; q += e minv p
+__device__ void leapfrog_begin(const ctx& k, double eps) {
+  double* p = k.slot(ZP);
+  double* q = k.slot(ZQ);
+  const double* g = k.slot(ZG);
+  const double* m = k.slot(MINV);
+  const double he = 0.5 * eps;
+  for (int i = k.lane; i < k.a->D; i += 64) {
+    p[i] -= he * g[i];
+    q[i] += eps * (m[i] * p[i]);
+  }
+}
+__device__ void leapfrog_end(const ctx& k, double eps) {
+  double* p = k.slot(ZP);
+  const double* g = k.slot(ZG);
+  const double he = 0.5 * eps;
+  for (int i = k.lane; i < k.a->D; i += 64) p[i] -= he * g[i];
+}
+
+__device__ dest init_dest(const dest& d, int l) {
+  return dest{d.zprop, d.pbeg, PIE(l), RI(l), 2 * l};
+}
+__device__ dest final_dest(const dest& d, int l) {
+  return dest{l, PFBG(l), d.pend, RF(l), 2 * l + 1};
+}
+__device__ dest top_dest(const ctx& k) {
+  return k.s->fwd ? dest{-1, PFB, FP, RHOF, -1} : dest{-1, PBF, BP, RHOB, -1};
+}
+__device__ dest dest_of(const ctx& k, int level) {
+  dest d = top_dest(k);
+  for (int l = k.s->tree_depth; l > level; --l)
+    d = k.s->stage[l] == 0 ? init_dest(d, l) : final_dest(d, l);
+  return d;
+}
+__device__ void start_frame(const ctx& k, int l) {
+  if (k.lane == 0) {
+    k.s->stage[l] = 0;
+    k.s->lsw_init[l] = -kInf;
+  }
+  vzero(k, RI(l));
+}
+
+// base case of build_tree, after the leapfrog's gradient
+__device__ bool leaf(const ctx& k, const dest& d) {
+  chain_state* s = k.s;
+  const double T = kinetic(k, ZP);
+  double h = T + s->V_z;
+  if (isnan(h)) h = kInf;
+  const double H0 = s->H0;
+  if (k.lane == 0) {
+    s->n_leapfrog++;
+    if ((h - H0) > k.a->max_deltaH) s->divergent = 1;
+    double& lw = lsw_ref(k, d.lsw);
+    lw = log_sum_exp(lw, H0 - h);
+    if (H0 - h > 0)
+      s->smp += 1;
+    else
+      s->smp += exp(H0 - h);
+    prop_V(k, d.zprop) = s->V_z;
+    prop_T(k, d.zprop) = T;
+  }
+  vcopy(k, prop_slot(d.zprop), ZQ);
+  {
+    const double* p = k.slot(ZP);
+    double* r = k.slot(d.rho);
+    double* pb = k.slot(d.pbeg);
+    double* pe = k.slot(d.pend);
+    for (int i = k.lane; i < k.a->D; i += 64) {
+      r[i] += p[i];
+      pb[i] = p[i];
+      pe[i] = p[i];
+    }
+  }
+  __syncthreads();
+  return !s->divergent;
+}
+
+// merge of the two halves of level l into d (base_nuts.hpp:321-351)
+__device__ bool merge(const ctx& k, int l, const dest& d) {
+  chain_state* s = k.s;
+  const double lsw_sub = log_sum_exp(s->lsw_init[l], s->lsw_final[l]);
+  double& lw = lsw_ref(k, d.lsw);
+  const double new_lw = log_sum_exp(lw, lsw_sub);
+  bool take;
+  if (s->lsw_final[l] > lsw_sub) {
+    take = true;
+  } else {
+    const double accept_prob = exp(s->lsw_final[l] - lsw_sub);
+    take = uniform(k) < accept_prob;
+  }
+  __syncthreads();
+  if (k.lane == 0) lw = new_lw;
+  if (take) copy_prop(k, d.zprop, l);
+  // rho_subtree = rho_init + rho_final; rho(d) += rho_subtree
+  {
+    const double* ri = k.slot(RI(l));
+    const double* rf = k.slot(RF(l));
+    double* t = k.slot(TMP);
+    double* r = k.slot(d.rho);
+    for (int i = k.lane; i < k.a->D; i += 64) {
+      t[i] = ri[i] + rf[i];
+      r[i] += t[i];
+    }
+  }
+  bool persist = dot_sharp(k, d.pend, TMP) > 0 && dot_sharp(k, d.pbeg, TMP) > 0;
+  persist &= dot_sharp2(k, PFBG(l), RI(l), PFBG(l)) > 0 &&
+             dot_sharp2(k, d.pbeg, RI(l), PFBG(l)) > 0;
+  persist &= dot_sharp2(k, d.pend, RF(l), PIE(l)) > 0 &&
+             dot_sharp2(k, PIE(l), RF(l), PIE(l)) > 0;
+  __syncthreads();
+  return persist;
+}
+
+// dual averaging (stepsize_adaptation::learn_stepsize)
+__device__ void learn_stepsize(const ctx& k, double stat) {
+  chain_state* s = k.s;
+  const batch_args* a = k.a;
+  s->sa_counter += 1;
+  stat = stat > 1 ? 1 : stat;
+  const double eta = 1.0 / (s->sa_counter + a->t0);
+  s->s_bar = (1.0 - eta) * s->s_bar + eta * (a->delta - stat);
+  const double x = s->mu - s->s_bar * sqrt(s->sa_counter) / a->gamma;
+  const double x_eta = pow(s->sa_counter, -a->kappa);
+  s->x_bar = (1.0 - x_eta) * s->x_bar + x_eta * x;
+  s->nom_eps = exp(x);
+}
+
+// windowed variance adaptation; returns 1 update, 0 none, -1 non-finite
+__device__ int learn_variance(const ctx& k) {
+  chain_state* s = k.s;
+  const batch_args* a = k.a;
+  const unsigned nw = a->w_num_warmup, ib = a->w_init_buffer, tb = a->w_term_buffer;
+  const bool in_window = (s->w_counter >= ib) && (s->w_counter < nw - tb) && (s->w_counter != nw);
+  if (in_window) {
+    const double n = s->w_n + 1.0;
+    double* m = k.slot(WM);
+    double* m2 = k.slot(WM2);
+    const double* q = k.slot(ZQ);
+    for (int i = k.lane; i < a->D; i += 64) {
+      const double delta = q[i] - m[i];
+      m[i] += delta / n;
+      m2[i] += (q[i] - m[i]) * delta;
+    }
+    __syncthreads();
+    if (k.lane == 0) s->w_n = n;
+  }
+  __syncthreads();
+  const bool end = (s->w_counter == s->w_next) && (s->w_counter != nw);
+  if (!end) {
+    __syncthreads();
+    if (k.lane == 0) s->w_counter++;
+    return 0;
+  }
+  // compute_next_window
+  unsigned next = s->w_next, size = s->w_size;
+  if (!(next == nw - tb - 1)) {
+    size *= 2;
+    next = s->w_counter + size;
+    if (!(next == nw - tb - 1)) {
+      const unsigned boundary = next + 2 * size;
+      if (boundary >= nw - tb) next = nw - tb - 1;
+    }
+  }
+  const double n = s->w_n;
+  double* minv = k.slot(MINV);
+  double* m = k.slot(WM);
+  double* m2 = k.slot(WM2);
+  int finite = 1;
+  for (int i = k.lane; i < a->D; i += 64) {
+    double var = minv[i];
+    if (n > 1) var = m2[i] / (n - 1.0);
+    var = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0)) * 1.0;
+    if (!isfinite(var)) finite = 0;
+    minv[i] = var;
+    m[i] = 0.0;
+    m2[i] = 0.0;
+  }
+  const int all_finite = __all(finite);
+  __syncthreads();
+  if (k.lane == 0) {
+    s->w_next = next;
+    s->w_size = size;
+    s->w_n = 0;
+    s->w_counter++;
+  }
+  return all_finite ? 1 : -1;
+}
+
+__device__ void write_draw(const ctx& k) {
+  chain_state* s = k.s;
+  const batch_args* a = k.a;
+  if (s->rows >= a->rows_cap) return;
+  double* row = a->draws + (static_cast<int64_t>(k.c) * a->rows_cap + s->rows) * a->ncol;
+  if (k.lane == 0) {
+    row[0] = -s->V_samp;
+    row[1] = s->accept;
+    row[2] = s->eps;
+    row[3] = s->depth;
+    row[4] = s->n_leapfrog;
+    row[5] = s->divergent;
+    row[6] = s->T_samp + s->V_samp;
+  }
+  const double* q = k.slot(SQ);
+  for (int i = k.lane; i < a->D; i += 64) row[7 + i] = q[i];
+  __syncthreads();
+  if (k.lane == 0) s->rows++;
+}
+
+__device__ void finish(const ctx& k, int status) {
+  if (k.lane == 0) {
+    k.s->status = status;
+    k.s->phase = PH_DONE;
+    k.s->pending = 0;
+    atomicAdd(k.a->done_count, 1u);
+  }
+}
+
+__device__ void begin_stepsize_heuristic(const ctx& k, int ret_phase) {
+  chain_state* s = k.s;
+  if (s->nom_eps == 0 || s->nom_eps > 1e7 || isnan(s->nom_eps)) {
+    if (k.lane == 0) s->phase = ret_phase;
+    return;
+  }
+  copy_point(k, IQ, ZQ, s->V_init, s->V_z);
+  __syncthreads();
+  if (k.lane == 0) {
+    s->ss_ret = ret_phase;
+    s->ss_iter = 0;
+  }
+  sample_p(k);
+  __syncthreads();
+  request_gradient(k, PH_SS_A);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void nuts_step(batch_args a) {
+  __shared__ chain_state s;
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  {
+    const int* src = reinterpret_cast<const int*>(a.states + c);
+    int* dst = reinterpret_cast<int*>(&s);
+    for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += 64) dst[i] = src[i];
+  }
+  __syncthreads();
+  ctx k{&a, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane,
+        a.chain0 + static_cast<uint32_t>(c)};
+  if (s.phase == PH_DONE) return;
+  if (s.pending) {
+    consume_gradient(k);
+    __syncthreads();
+    if (lane == 0) s.pending = 0;
+  }
+  __syncthreads();
+  // advance until a gradient is requested or the chain finishes
+  for (int guard = 0; guard < 4096 && !s.pending && s.phase != PH_DONE; ++guard) {
+    const int ph = s.phase;
+    __syncthreads();
+    switch (ph) {
+      case PH_INIT_TRY: {
+        // uniform(-R, R) inits (random_var_context), or the given init
+        double* q = k.slot(ZQ);
+        if (a.inits) {
+          const double* src = a.inits + static_cast<int64_t>(c) * a.D;
+          for (int i = lane; i < a.D; i += 64) q[i] = src[i];
+        } else {
+          const uint64_t base = s.rng;
+          for (int i = lane; i < a.D; i += 64)
+            q[i] = a.init_radius == 0.0
+                       ? 0.0
+                       : -a.init_radius + stanx::philox_uniform(a.seed, k.chain_id, base + i) *
+                                              2.0 * a.init_radius;
+          __syncthreads();
+          if (lane == 0 && a.init_radius != 0.0) s.rng = base + a.D;
+        }
+        __syncthreads();
+        request_gradient(k, PH_INIT_CHECK);
+        break;
+      }
+      case PH_INIT_CHECK: {
+        double gs = 0.0;
+        const double* g = k.slot(ZG);
+        for (int i = lane; i < a.D; i += 64) gs += g[i];
+        gs = wave_sum(gs);
+        const bool ok = s.last_ok && isfinite(s.last_lp) && isfinite(gs);
+        __syncthreads();
+        if (ok) {
+          // run_adaptive_sampler: engage adaptation, step-size heuristic
+          if (lane == 0) s.adapt_flag = 1;
+          begin_stepsize_heuristic(k, PH_TRANS_BEGIN);
+        } else {
+          if (lane == 0) s.init_tries++;
+          __syncthreads();
+          if (s.init_tries >= a.max_init_tries) {
+            finish(k, 78);
+          } else if (lane == 0) {
+            s.phase = PH_INIT_TRY;
+          }
+        }
+        break;
+      }
+      case PH_SS_A: {  // heuristic: gradient at z with fresh p
+        const double T = kinetic(k, ZP);
+        if (lane == 0) s.ss_H0 = T + s.V_z;
+        leapfrog_begin(k, s.nom_eps);
+        __syncthreads();
+        request_gradient(k, PH_SS_B);
+        break;
+      }
+      case PH_SS_B: {
+        leapfrog_end(k, s.nom_eps);
+        __syncthreads();
+        const double T = kinetic(k, ZP);
+        double h = T + s.V_z;
+        if (isnan(h)) h = kInf;
+        const double dH = s.ss_H0 - h;
+        bool done = false, fail = false;
+        double eps = s.nom_eps;
+        int dir = s.ss_dir;
+        if (s.ss_iter == 0) {
+          dir = dH > kLog08 ? 1 : -1;
+        } else if ((dir == 1) && !(dH > kLog08)) {
+          done = true;
+        } else if ((dir == -1) && !(dH < kLog08)) {
+          done = true;
+        } else {
+          eps = dir == 1 ? 2.0 * eps : 0.5 * eps;
+          if (eps > 1e7 || eps == 0) fail = true;
+        }
+        __syncthreads();
+        if (lane == 0) {
+          s.ss_dir = dir;
+          s.ss_iter = 1;
+          s.nom_eps = eps;
+        }
+        // restore z_init
+        copy_point(k, ZQ, IQ, s.V_z, s.V_init);
+        __syncthreads();
+        if (fail) {
+          // the reference's initial heuristic failure is logged and the
+          // service returns OK without draws; later failures are SOFTWARE
+          finish(k, s.ss_ret == PH_TRANS_BEGIN ? 0 : 70);
+        } else if (done) {
+          if (lane == 0) s.phase = s.ss_ret;
+        } else {
+          sample_p(k);
+          __syncthreads();
+          request_gradient(k, PH_SS_A);
+        }
+        break;
+      }
+      case PH_POST_ADAPT: {
+        if (lane == 0) {
+          s.mu = log(10 * s.nom_eps);
+          s.sa_counter = 0;
+          s.s_bar = 0;
+          s.x_bar = 0;
+          s.iter++;
+          s.phase = PH_TRANS_BEGIN;
+        }
+        break;
+      }
+      case PH_TRANS_BEGIN: {
+        if (s.iter == a.num_warmup + a.num_samples) {
+          finish(k, 0);
+          break;
+        }
+        if (s.iter == a.num_warmup && s.adapt_flag) {
+          // disengage_adaptation: complete_adaptation sets eps = exp(x_bar)
+          if (lane == 0) {
+            s.adapt_flag = 0;
+            s.nom_eps = exp(s.x_bar);
+          }
+          __syncthreads();
+        }
+        // sample_stepsize, seed(q) (z.q already holds the last sample), sample_p
+        double eps = s.nom_eps;
+        if (a.jitter) eps *= 1.0 + a.jitter * (2.0 * uniform(k) - 1.0);
+        __syncthreads();
+        if (lane == 0) s.eps = eps;
+        sample_p(k);
+        __syncthreads();
+        request_gradient(k, PH_TRANS_INIT);
+        break;
+      }
+      case PH_TRANS_INIT: {
+        const double T = kinetic(k, ZP);
+        copy_point(k, FQ, ZQ, s.V_fwd, s.V_z);
+        copy_point(k, BQ, ZQ, s.V_bck, s.V_z);
+        vcopy(k, SQ, ZQ);
+        vcopy(k, PQ, ZQ);
+        vcopy(k, PFB, ZP);
+        vcopy(k, PBF, ZP);
+        vcopy(k, RHO, ZP);
+        __syncthreads();
+        if (lane == 0) {
+          s.V_samp = s.V_z;
+          s.T_samp = T;
+          s.V_prop = s.V_z;
+          s.T_prop = T;
+          s.lsw = 0;
+          s.H0 = T + s.V_z;
+          s.n_leapfrog = 0;
+          s.smp = 0;
+          s.depth = 0;
+          s.divergent = 0;
+          s.phase = PH_DOUBLING;
+        }
+        break;
+      }
+      case PH_DOUBLING: {
+        if (s.depth >= a.max_depth) {
+          if (lane == 0) s.phase = PH_TRANS_END;
+          break;
+        }
+        vzero(k, RHOF);
+        vzero(k, RHOB);
+        const bool fwd = uniform(k) > 0.5;
+        __syncthreads();
+        if (fwd) {
+          copy_point(k, ZQ, FQ, s.V_z, s.V_fwd);
+          vcopy(k, RHOB, RHO);
+          vcopy(k, PBF, FP);
+        } else {
+          copy_point(k, ZQ, BQ, s.V_z, s.V_bck);
+          vcopy(k, RHOF, RHO);
+          vcopy(k, PFB, BP);
+        }
+        __syncthreads();
+        if (lane == 0) {
+          s.fwd = fwd ? 1 : 0;
+          s.lsw_sub = -kInf;
+          s.tree_depth = s.depth;
+        }
+        __syncthreads();
+        for (int l = s.tree_depth; l >= 1; --l) start_frame(k, l);
+        __syncthreads();
+        leapfrog_begin(k, (s.fwd ? 1.0 : -1.0) * s.eps);
+        __syncthreads();
+        request_gradient(k, PH_LEAF);
+        break;
+      }
+      case PH_LEAF: {
+        const double eps = (s.fwd ? 1.0 : -1.0) * s.eps;
+        leapfrog_end(k, eps);
+        __syncthreads();
+        bool valid = leaf(k, dest_of(k, 0));
+        bool descended = false;
+        int l = 1;
+        while (l <= s.tree_depth) {
+          if (!valid) break;
+          if (s.stage[l] == 0) {
+            // initial half done: start the final half from the current point
+            const double T = kinetic(k, ZP);
+            vcopy(k, ZPF(l), ZQ);
+            vzero(k, RF(l));
+            __syncthreads();
+            if (lane == 0) {
+              s.stage[l] = 1;
+              s.V_zpf[l] = s.V_z;
+              s.T_zpf[l] = T;
+              s.lsw_final[l] = -kInf;
+            }
+            __syncthreads();
+            for (int m = l - 1; m >= 1; --m) start_frame(k, m);
+            __syncthreads();
+            leapfrog_begin(k, eps);
+            __syncthreads();
+            request_gradient(k, PH_LEAF);
+            descended = true;
+            break;
+          }
+          valid = merge(k, l, dest_of(k, l));
+          ++l;
+        }
+        if (descended) break;
+        // the subtree of depth tree_depth is complete (or invalid)
+        if (s.fwd)
+          copy_point(k, FQ, ZQ, s.V_fwd, s.V_z);
+        else
+          copy_point(k, BQ, ZQ, s.V_bck, s.V_z);
+        __syncthreads();
+        if (!valid) {
+          if (lane == 0) s.phase = PH_TRANS_END;
+          break;
+        }
+        bool take;
+        if (s.lsw_sub > s.lsw) {
+          take = true;
+        } else {
+          take = uniform(k) < exp(s.lsw_sub - s.lsw);
+        }
+        __syncthreads();
+        if (take) copy_prop(k, 0, -1);
+        {
+          const double* rb = k.slot(RHOB);
+          const double* rf = k.slot(RHOF);
+          double* r = k.slot(RHO);
+          for (int i = lane; i < a.D; i += 64) r[i] = rb[i] + rf[i];
+        }
+        __syncthreads();
+        bool persist = dot_sharp(k, FP, RHO) > 0 && dot_sharp(k, BP, RHO) > 0;
+        persist &= dot_sharp2(k, PFB, RHOB, PFB) > 0 && dot_sharp2(k, BP, RHOB, PFB) > 0;
+        persist &= dot_sharp2(k, FP, RHOF, PBF) > 0 && dot_sharp2(k, PBF, RHOF, PBF) > 0;
+        __syncthreads();
+        if (lane == 0) {
+          s.depth++;
+          s.lsw = log_sum_exp(s.lsw, s.lsw_sub);
+          s.phase = persist ? PH_DOUBLING : PH_TRANS_END;
+        }
+        break;
+      }
+      case PH_TRANS_END: {
+        // z = z_sample; record; adapt
+        vcopy(k, ZQ, SQ);
+        __syncthreads();
+        if (lane == 0) {
+          s.accept = s.smp / static_cast<double>(s.n_leapfrog);
+          s.V_z = s.V_samp;
+        }
+        __syncthreads();
+        const bool warm = s.iter < a.num_warmup;
+        const int m = warm ? s.iter : s.iter - a.num_warmup;
+        if ((warm ? a.save_warmup != 0 : true) && (m % a.num_thin) == 0) write_draw(k);
+        __syncthreads();
+        if (s.adapt_flag) {
+          if (lane == 0) learn_stepsize(k, s.accept);
+          __syncthreads();
+          const int upd = learn_variance(k);
+          __syncthreads();
+          if (upd < 0) {
+            finish(k, 70);
+          } else if (upd > 0) {
+            begin_stepsize_heuristic(k, PH_POST_ADAPT);
+          } else if (lane == 0) {
+            s.iter++;
+            s.phase = PH_TRANS_BEGIN;
+          }
+        } else if (lane == 0) {
+          s.iter++;
+          s.phase = PH_TRANS_BEGIN;
+        }
+        break;
+      }
+      default:
+        finish(k, 70);
+        break;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  {
+    const int* src = reinterpret_cast<const int*>(&s);
+    int* dst = reinterpret_cast<int*>(a.states + c);
+    for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += 64) dst[i] = src[i];
+  }
+}
+
+hipError_t launch_nuts_step(const batch_args& a, hipStream_t s) {
+  hipLaunchKernelGGL(nuts_step, dim3(a.C), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace nb
+}  // namespace sx

@@ -1,0 +1,66 @@
+// Shared definitions of the device-resident batched NUTS sampler.
+#ifndef STANX_NUTS_BATCH_HPP
+#define STANX_NUTS_BATCH_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sx {
+namespace nb {
+
+constexpr int kMaxDepth = 12;  // frames per chain (max_depth <= kMaxDepth)
+constexpr int kNumSlots = 23 + 5 * kMaxDepth;
+constexpr int kSlotMinv = 20;  // vector slot of the diagonal inverse metric
+
+enum phase : int {
+  PH_INIT_TRY = 0,
+  PH_INIT_CHECK,
+  PH_SS_A,
+  PH_SS_B,
+  PH_POST_ADAPT,
+  PH_TRANS_BEGIN,
+  PH_TRANS_INIT,
+  PH_DOUBLING,
+  PH_LEAF,
+  PH_TRANS_END,
+  PH_DONE
+};
+
+// Scalar state of one chain (copied to LDS by the step kernel).
+struct chain_state {
+  int phase, pending, status, iter, rows, init_tries, adapt_flag;
+  int depth, tree_depth, n_leapfrog, divergent, fwd;
+  int ss_ret, ss_iter, ss_dir, last_ok;
+  int stage[kMaxDepth + 1];
+  unsigned w_counter, w_next, w_size, pad0;
+  uint64_t rng;
+  int64_t n_grad;
+  double nom_eps, eps, H0, lsw, lsw_sub, smp, accept;
+  double V_z, V_fwd, V_bck, V_samp, T_samp, V_prop, T_prop, V_init, ss_H0, last_lp;
+  double sa_counter, s_bar, x_bar, mu, w_n;
+  double lsw_init[kMaxDepth + 1], lsw_final[kMaxDepth + 1];
+  double V_zpf[kMaxDepth + 1], T_zpf[kMaxDepth + 1];
+};
+
+struct batch_args {
+  chain_state* states;
+  double* vecs;           // C x nslot x Dpad
+  double* theta;          // C x D: the point each chain needs a gradient at
+  const double* out;      // C x out_stride: lp, grad[D], status
+  double* draws;          // C x rows_cap x ncol
+  const double* inits;    // C x D or null
+  unsigned* done_count;
+  int64_t out_stride, rows_cap;
+  int C, D, Dpad, nslot, ncol;
+  uint32_t seed, chain0;
+  int num_warmup, num_samples, num_thin, save_warmup, max_depth, max_init_tries;
+  double init_radius, jitter, delta, gamma, kappa, t0, max_deltaH;
+  unsigned w_num_warmup, w_init_buffer, w_term_buffer;
+};
+
+hipError_t launch_nuts_step(const batch_args& a, hipStream_t s);
+
+}  // namespace nb
+}  // namespace sx
+#endif

@@ -43,13 +43,16 @@ tile_geom make_geom(int64_t N, int64_t K, int L, int smax);
 struct pass_args {
   const double* X;      // tiled
   const void* y;        // int8 (logistic) or double (normal)
-  const double* theta;  // D doubles on the device
+  const double* theta;  // D doubles per chain on the device
   double* partials;     // blocks x E
   double* out;          // lp, grad[D], status
   int64_t N;
   int64_t T;
   int K, L, lgL, S, D, E;
   int propto, jacobian;
+  int chains = 1;              // batched evaluation: chains on grid.y
+  int64_t theta_stride = 0;    // doubles between chains' theta
+  int64_t out_stride = 0;      // doubles between chains' outputs (>= D+2)
   double alpha_sd, beta_sd, sigma_sd;
   // normalising constants -log(sd) - log(2 pi)/2 of the normal priors
   double alpha_const, beta_const, sigma_const;
