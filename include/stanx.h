@@ -67,7 +67,8 @@ extern "C" {
 /* storage/arithmetic precision of the bulk X pass */
 #define SX_FP64 0   /* FP64 X, FP64 arithmetic: the plain drop-in           */
 #define SX_FP32 1   /* FP32 X, FP32 products, FP64 block reductions         */
-#define SX_BF16X2 2 /* X as BF16 hi+lo on MFMA, FP32 accumulate (batched)   */
+#define SX_BF16X2 2 /* FP64 X for single evaluations, plus X as BF16 hi+lo
+                        for batched MFMA evaluations (K <= 256)        */
 
 /* service error codes (src/stan/services/error_codes.hpp:7-18) */
 #define SX_CODE_OK 0
@@ -130,7 +131,11 @@ int sx_glm_logp_grad(sx_glm* m, const double* theta, int propto,
 int sx_glm_logp(sx_glm* m, const double* theta, int propto, int jacobian,
                 double* lp);
 
-/* C chains at once: theta is C x D (row-major), lp C, grad C x D.      */
+/* C chains at once: theta is C x D (row-major), lp C, grad C x D.  On a
+ * context created with SX_BF16X2 all chains are evaluated together by the
+ * MFMA pass (X as BF16 hi+lo, FP32 accumulation, FP64 cross-block sums);
+ * otherwise chain by chain in FP64.  A chain whose density cannot be
+ * evaluated gets lp = -inf and the call returns SX_EDOMAIN.             */
 int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C,
                            int propto, int jacobian, double* lp,
                            double* grad);

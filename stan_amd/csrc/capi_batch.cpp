@@ -64,6 +64,8 @@ void window_params(unsigned num_warmup, unsigned init_buffer, unsigned term_buff
 }
 
 int enqueue_gradient(sx_nuts_batch* b) {
+  if (b->grad_precision == SX_BF16X2)
+    return sx::mx_enqueue(b->m, b->d_theta, static_cast<int>(b->C), b->d_out, 1, 1, b->stream);
   SX_HIP(sx::launch_pass_f64(b->m->family, b->m->geom, b->g, b->grad_blocks, b->m->threads,
                              b->stream, true));
   return SX_OK;
@@ -104,8 +106,8 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
     return SX_EINVAL;
   }
   *out = nullptr;
-  if (grad_precision != SX_FP64) {
-    set_error("sx_nuts_batch_create: gradient precision not available for this model");
+  if (grad_precision != SX_FP64 && !(grad_precision == SX_BF16X2 && m->precision == SX_BF16X2)) {
+    set_error("sx_nuts_batch_create: SX_BF16X2 gradients need a context created with SX_BF16X2");
     return SX_EINVAL;
   }
   const int max_depth = cfg->max_depth > 0 ? cfg->max_depth : 5;
@@ -221,9 +223,10 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
   a.w_term_buffer = tb;
   // batched FP64 gradient: every chain's pass on grid.y
   {
-    // blocks per chain depend on the data size only, so a chain's
-    // reduction order (and its draws) is the same in any batch
-    b->grad_blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(128, (m->geom.T + 31) / 32)));
+    // each chain's pass uses the launch shape of the single-chain pass: a
+    // chain's gradient is bitwise the plain path's, and independent of the
+    // other chains in the batch
+    b->grad_blocks = m->blocks;
     const size_t E = static_cast<size_t>(m->K) + 3;
     if (hipMalloc(&b->d_partials, sizeof(double) * E * b->grad_blocks * chains) != hipSuccess)
       return fail(SX_ENOMEM, "partials");
@@ -253,6 +256,11 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
     g.alpha_const = g.alpha_sd > 0 ? -std::log(g.alpha_sd) - h : 0.0;
     g.beta_const = g.beta_sd > 0 ? -std::log(g.beta_sd) - h : 0.0;
     g.sigma_const = g.sigma_sd > 0 ? -std::log(g.sigma_sd) - h : 0.0;
+  }
+  if (grad_precision == SX_BF16X2) {
+    // allocate the MFMA workspace outside the captured region
+    const int rcp = sx::mx_enqueue(m, b->d_theta, static_cast<int>(chains), b->d_out, 1, 1, b->stream);
+    if (rcp != SX_OK) return fail(rcp, "MFMA workspace");
   }
   // capture a fixed run of lockstep steps in a graph
   if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(SX_EDEVICE, "sync");

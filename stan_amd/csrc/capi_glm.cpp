@@ -159,8 +159,12 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     set_error("sx_glm_create: family not supported by this build");
     return SX_EINVAL;
   }
-  if (precision != SX_FP64) {
+  if (precision != SX_FP64 && precision != SX_BF16X2) {
     set_error("sx_glm_create: precision not supported for this family");
+    return SX_EINVAL;
+  }
+  if (precision == SX_BF16X2 && K > 256) {
+    set_error("sx_glm_create: BF16 hi+lo batched evaluation supports K <= 256");
     return SX_EINVAL;
   }
   if (N < 0 || K < 0 || K > 1024 || (N > 0 && (!X || !y))) {
@@ -284,6 +288,20 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
                                                   : 2 * cus);
     }
   }
+  if (rc == SX_OK && precision == SX_BF16X2) {
+    m->Np = (N + 31) / 32 * 32;
+    m->Kp = std::max<int64_t>(32, (K + 31) / 32 * 32);
+    const size_t elems = static_cast<size_t>(m->Np) * m->Kp;
+    if (hipMalloc(&m->d_X1h, elems * 2) != hipSuccess || hipMalloc(&m->d_X1l, elems * 2) != hipSuccess ||
+        hipMalloc(&m->d_X2h, elems * 2) != hipSuccess || hipMalloc(&m->d_X2l, elems * 2) != hipSuccess) {
+      set_error("hipMalloc(BF16 X) failed");
+      rc = SX_ENOMEM;
+    } else if (sx::launch_split_x(staging, m->d_X1h, m->d_X1l, m->d_X2h, m->d_X2l, N, K, m->Np,
+                                  m->Kp, m->stream) != hipSuccess) {
+      set_error("BF16 split of X failed");
+      rc = SX_EDEVICE;
+    }
+  }
   if (rc == SX_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = SX_EDEVICE;
   if (staging) (void)hipFree(staging);
   if (rc) return fail(rc);
@@ -302,6 +320,17 @@ void sx_glm_destroy(sx_glm* m) {
   (void)hipFree(m->d_theta);
   (void)hipFree(m->d_partials);
   (void)hipFree(m->d_out);
+  (void)hipFree(m->d_X1h);
+  (void)hipFree(m->d_X1l);
+  (void)hipFree(m->d_X2h);
+  (void)hipFree(m->d_X2l);
+  (void)hipFree(m->mx.Bh);
+  (void)hipFree(m->mx.Bl);
+  (void)hipFree(m->mx.alpha);
+  (void)hipFree(m->mx.gpart);
+  (void)hipFree(m->mx.spart);
+  (void)hipFree(m->d_btheta);
+  (void)hipFree(m->d_bout);
   if (m->h_theta) (void)hipHostFree(m->h_theta);
   if (m->h_out) (void)hipHostFree(m->h_out);
   if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -387,6 +416,102 @@ int eval(sx_glm* m, const double* theta, int propto, int jacobian, double* lp,
 
 }  // namespace
 
+namespace sx {
+
+// Batched MFMA evaluation of C chains: theta (device, C x D) -> out
+// (device, C x (D + 2)).  Row splits depend on N only, so each chain's
+// result is independent of the other chains in the batch.
+int mx_prepare(sx_glm* m, int C) {
+  if (!m->d_X1h) {
+    set_error("batched BF16 evaluation needs a context created with SX_BF16X2");
+    return SX_EINVAL;
+  }
+  const int Cp = (C + 63) / 64 * 64;
+  mx_workspace& w = m->mx;
+  const int64_t n_tiles = m->Np / 32;
+  int splits = static_cast<int>(std::min<int64_t>(512, std::max<int64_t>(8, ((n_tiles + 7) / 8 + 7) / 8 * 8)));
+  if (w.cap >= Cp && w.splits == splits) return SX_OK;
+  (void)hipFree(w.Bh);
+  (void)hipFree(w.Bl);
+  (void)hipFree(w.alpha);
+  (void)hipFree(w.gpart);
+  (void)hipFree(w.spart);
+  w = mx_workspace();
+  const size_t ck = static_cast<size_t>(Cp) * m->Kp;
+  SX_HIP(hipMalloc(&w.Bh, ck * 2));
+  SX_HIP(hipMalloc(&w.Bl, ck * 2));
+  SX_HIP(hipMalloc(&w.alpha, sizeof(float) * Cp));
+  SX_HIP(hipMalloc(&w.gpart, sizeof(float) * ck * splits));
+  SX_HIP(hipMalloc(&w.spart, sizeof(double) * 3 * static_cast<size_t>(Cp) * splits));
+  w.cap = Cp;
+  w.Cp = Cp;
+  w.splits = splits;
+  return SX_OK;
+}
+
+int mx_enqueue(sx_glm* m, const double* d_theta, int C, double* d_out, int propto, int jacobian,
+               hipStream_t stream) {
+  int rc = mx_prepare(m, C);
+  if (rc) return rc;
+  mx_args a{};
+  a.X1h = m->d_X1h;
+  a.X1l = m->d_X1l;
+  a.X2h = m->d_X2h;
+  a.X2l = m->d_X2l;
+  a.y = m->d_y;
+  a.Bh = m->mx.Bh;
+  a.Bl = m->mx.Bl;
+  a.alpha = m->mx.alpha;
+  a.gpart = m->mx.gpart;
+  a.spart = m->mx.spart;
+  a.N = m->N;
+  a.n_tiles = m->Np / 32;
+  a.Kp = static_cast<int>(m->Kp);
+  a.Cp = (C + 63) / 64 * 64;
+  a.splits = m->mx.splits;
+  a.tiles_per_split = (a.n_tiles + a.splits - 1) / a.splits;
+  pass_args pa{};
+  pa.theta = d_theta;
+  pa.out = d_out;
+  pa.N = m->N;
+  pa.K = static_cast<int>(m->K);
+  pa.D = static_cast<int>(m->D);
+  pa.E = static_cast<int>(m->K) + 3;
+  pa.propto = propto;
+  pa.jacobian = jacobian;
+  pa.chains = C;
+  pa.theta_stride = m->D;
+  pa.out_stride = m->D + 2;
+  pa.alpha_sd = m->prior.alpha_sd;
+  pa.beta_sd = m->prior.beta_sd;
+  pa.sigma_sd = m->prior.sigma_sd;
+  const double h = 0.91893853320467274178032973640562;
+  pa.alpha_const = pa.alpha_sd > 0 ? -std::log(pa.alpha_sd) - h : 0.0;
+  pa.beta_const = pa.beta_sd > 0 ? -std::log(pa.beta_sd) - h : 0.0;
+  pa.sigma_const = pa.sigma_sd > 0 ? -std::log(pa.sigma_sd) - h : 0.0;
+  SX_HIP(launch_mfma_pass(m->family, d_theta, m->D, C, a, pa, stream));
+  return SX_OK;
+}
+
+}  // namespace sx
+
+namespace {
+
+int ensure_batch_staging(sx_glm* m, int64_t C) {
+  if (m->bcap >= C) return SX_OK;
+  (void)hipFree(m->d_btheta);
+  (void)hipFree(m->d_bout);
+  m->d_btheta = nullptr;
+  m->d_bout = nullptr;
+  m->bcap = 0;
+  SX_HIP(hipMalloc(&m->d_btheta, sizeof(double) * C * m->D));
+  SX_HIP(hipMalloc(&m->d_bout, sizeof(double) * C * (m->D + 2)));
+  m->bcap = C;
+  return SX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int sx_glm_logp_grad(sx_glm* m, const double* theta, int propto, int jacobian, double* lp,
@@ -404,6 +529,34 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C, int propto
     set_error("sx_glm_logp_grad_batch: invalid argument");
     return SX_EINVAL;
   }
+  if (C == 0) return SX_OK;
+  if (m->precision == SX_BF16X2) {
+    SX_HIP(hipSetDevice(m->device));
+    int rc = ensure_batch_staging(m, C);
+    if (rc) return rc;
+    const int64_t D = m->D;
+    SX_HIP(hipMemcpyAsync(m->d_btheta, theta, sizeof(double) * C * D, hipMemcpyHostToDevice,
+                          m->stream));
+    rc = sx::mx_enqueue(m, m->d_btheta, static_cast<int>(C), m->d_bout, propto, jacobian,
+                        m->stream);
+    if (rc) return rc;
+    std::vector<double> out(static_cast<size_t>(C * (D + 2)));
+    SX_HIP(hipMemcpyAsync(out.data(), m->d_bout, sizeof(double) * out.size(),
+                          hipMemcpyDeviceToHost, m->stream));
+    SX_HIP(hipStreamSynchronize(m->stream));
+    int worst = SX_OK;
+    for (int64_t c = 0; c < C; ++c) {
+      const double* o = out.data() + c * (D + 2);
+      lp[c] = o[0];
+      if (grad) std::memcpy(grad + c * D, o + 1, sizeof(double) * D);
+      if (o[D + 1] != 0.0) {
+        lp[c] = -INFINITY;
+        worst = SX_EDOMAIN;
+        set_error("batched GLM: linear predictor or log density is not finite");
+      }
+    }
+    return worst;
+  }
   int worst = SX_OK;
   for (int64_t c = 0; c < C; ++c) {
     int rc = eval(m, theta + c * m->D, propto, jacobian, lp + c,
@@ -420,22 +573,33 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C, int propto
 
 int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C, int iters, int warmup,
                           double* ms_per_eval, double* ms_kernel) {
-  if (!m || !theta || iters <= 0 || C != 1) {
+  if (!m || !theta || iters <= 0 || C < 1 || (C > 1 && m->precision != SX_BF16X2)) {
     set_error("sx_glm_time_logp_grad: invalid argument");
     return SX_EINVAL;
   }
   SX_HIP(hipSetDevice(m->device));
   const size_t D = static_cast<size_t>(m->D);
-  SX_HIP(hipMemcpy(m->d_theta, theta, D * sizeof(double), hipMemcpyHostToDevice));
+  const bool batched = C > 1;
+  if (batched) {
+    int rc = ensure_batch_staging(m, C);
+    if (rc) return rc;
+    SX_HIP(hipMemcpy(m->d_btheta, theta, D * C * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    SX_HIP(hipMemcpy(m->d_theta, theta, D * sizeof(double), hipMemcpyHostToDevice));
+  }
+  auto one = [&](bool reduce) -> int {
+    if (batched) return sx::mx_enqueue(m, m->d_btheta, static_cast<int>(C), m->d_bout, 1, 1, m->stream);
+    return enqueue_pass(m, 1, 1, reduce);
+  };
   for (int i = 0; i < warmup; ++i) {
-    int rc = enqueue_pass(m, 1, 1);
+    int rc = one(true);
     if (rc) return rc;
   }
   SX_HIP(hipStreamSynchronize(m->stream));
-  // full evaluations: bulk pass + reduction/finalize launches
+  // full evaluations (for a single chain: bulk pass + cross-block reduction)
   SX_HIP(hipEventRecord(m->ev0, m->stream));
   for (int i = 0; i < iters; ++i) {
-    int rc = enqueue_pass(m, 1, 1);
+    int rc = one(true);
     if (rc) return rc;
   }
   SX_HIP(hipEventRecord(m->ev1, m->stream));
@@ -443,10 +607,14 @@ int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C, int iters, 
   float ms = 0.f;
   SX_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
   if (ms_per_eval) *ms_per_eval = ms / iters;
+  if (batched) {
+    if (ms_kernel) *ms_kernel = ms / iters;
+    return SX_OK;
+  }
   // the bulk X pass alone (its partials are left unreduced)
   SX_HIP(hipEventRecord(m->ev0, m->stream));
   for (int i = 0; i < iters; ++i) {
-    int rc = enqueue_pass(m, 1, 1, false);
+    int rc = one(false);
     if (rc) return rc;
   }
   SX_HIP(hipEventRecord(m->ev1, m->stream));

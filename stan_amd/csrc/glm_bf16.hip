@@ -1,0 +1,282 @@
+// Batched GLM log density + gradient for many chains on MFMA (gfx950),
+// X held as a BF16 hi+lo split with FP32 accumulation.
+//
+// For C chains with coefficients B (K x C) the forward pass is the dense
+// contraction H = X B and the adjoint is G = X^T R with R the residual
+// matrix: two GEMMs per evaluation, which is where MFMA pays (the single-
+// chain pass is a matrix-vector product and stays on the HBM roofline).
+// Each product of a split pair is formed as hi*hi + hi*lo + lo*hi (the
+// lo*lo term is below the split's own representation error) with
+// v_mfma_f32_16x16x32_bf16.
+//
+// The two contractions run over different indices of X (k forward, n in
+// the adjoint) and an MFMA operand holds 8 consecutive inner-index values
+// per lane, so X is stored twice, in HBM, as BF16 hi+lo:
+//   X1[n][k]            k-packed rows      (forward operand)
+//   X2[n/8][k][n % 8]   8-row-packed       (adjoint operand)
+// Workgroup = 4 waves x 16 chains; the grid's x dimension splits the rows,
+// y the chain tiles; row splits are multiples of 8 so the blocks that share
+// a row range (different chain tiles) land on the same XCD and share X in
+// its L2.  Per-block FP32 partials are summed across blocks in FP64 by a
+// second launch that also adds priors and Jacobian (shared finalize).
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "glm_finalize.hpp"
+#include "sx_internal.hpp"
+
+namespace sx {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
+
+__device__ __forceinline__ void split(float x, __bf16* hi, __bf16* lo) {
+  const __bf16 h = to_bf16(x);
+  *hi = h;
+  *lo = to_bf16(x - static_cast<float>(h));
+}
+
+__device__ __forceinline__ f32x4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+// FP32 Bernoulli-logit row: term = -log1p(exp(-s eta)), r = s inv_logit(-s eta)
+__device__ __forceinline__ void logit_row_f32(float eta, float s, float* term, float* r) {
+  const float se = s * eta;
+  const float t = __expf(-fabsf(se));
+  const float inv = 1.0f / (1.0f + t);
+  *term = -(fmaxf(-se, 0.0f) + log1pf(t));
+  *r = s * (se >= 0.0f ? t * inv : inv);
+}
+
+// Splits row-major FP64 X into the two BF16 hi+lo layouts (zero padded).
+__global__ void split_x(const double* __restrict__ Xrm, __bf16* X1h, __bf16* X1l, __bf16* X2h,
+                        __bf16* X2l, int64_t N, int64_t K, int64_t Kp, int64_t total) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int64_t n = idx / Kp;
+  const int64_t k = idx - n * Kp;
+  const float x = (n < N && k < K) ? static_cast<float>(Xrm[n * K + k]) : 0.0f;
+  __bf16 h, l;
+  split(x, &h, &l);
+  X1h[idx] = h;
+  X1l[idx] = l;
+  const int64_t j = ((n >> 3) * Kp + k) * 8 + (n & 7);
+  X2h[j] = h;
+  X2l[j] = l;
+}
+
+// theta (C x D, FP64) -> B hi/lo (Cp x Kp), alpha (Cp)
+__global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, int K, int Kp,
+                            __bf16* Bh, __bf16* Bl, float* alpha) {
+  const int c = blockIdx.x;
+  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+    const float b = (c < C && k < K) ? static_cast<float>(theta[c * D + 1 + k]) : 0.0f;
+    __bf16 h, l;
+    split(b, &h, &l);
+    Bh[static_cast<int64_t>(c) * Kp + k] = h;
+    Bl[static_cast<int64_t>(c) * Kp + k] = l;
+  }
+  if (threadIdx.x == 0) alpha[c] = c < C ? static_cast<float>(theta[c * D]) : 0.0f;
+}
+
+template <int KS, int FAM>
+__global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
+  __shared__ float rs[4][16][33];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lr = lane & 15, lg = lane >> 4;
+  const int Kp = a.Kp;
+  const int cb = blockIdx.y * 64 + 16 * w;  // first chain of this wave
+  // forward A operands: B^T rows (chains cb + lr), k = 32 s + 8 lg + j
+  bf16x8 bh[KS], bl[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int64_t off = static_cast<int64_t>(cb + lr) * Kp + 32 * s + 8 * lg;
+    bh[s] = *reinterpret_cast<const bf16x8*>(a.Bh + off);
+    bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
+  }
+  float alpha[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) alpha[j] = a.alpha[cb + 4 * lg + j];
+  f32x4 g[2 * KS];
+#pragma unroll
+  for (int i = 0; i < 2 * KS; ++i) g[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  double lp[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0}, bad[4] = {0, 0, 0, 0};
+
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * a.tiles_per_split;
+  const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t n0 = t * 32;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      f32x4 h = {0.f, 0.f, 0.f, 0.f};
+      const int64_t n = n0 + 16 * rb + lr;
+      const int64_t xoff = n * Kp + 8 * lg;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const bf16x8 xh = *reinterpret_cast<const bf16x8*>(a.X1h + xoff + 32 * s);
+        const bf16x8 xl = *reinterpret_cast<const bf16x8*>(a.X1l + xoff + 32 * s);
+        h = mma3(bh[s], bl[s], xh, xl, h);
+      }
+      // h[j] = (X B)[n][cb + 4 lg + j]
+      const bool valid = n < a.N;
+      float yv = 0.f;
+      if (valid) {
+        if (FAM == SX_LOGIT)
+          yv = static_cast<const int8_t*>(a.y)[n] ? 1.0f : -1.0f;
+        else
+          yv = static_cast<float>(static_cast<const double*>(a.y)[n]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float eta = h[j] + alpha[j];
+        float term = 0.f, r = 0.f;
+        if (valid) {
+          if (FAM == SX_LOGIT) {
+            logit_row_f32(eta, yv, &term, &r);
+          } else {
+            r = yv - eta;
+            term = r * r;
+          }
+          if (!isfinite(eta)) bad[j] += 1.0;
+        }
+        lp[j] += term;
+        sr[j] += r;
+        rs[w][4 * lg + j][16 * rb + lr] = r;
+      }
+    }
+    __syncthreads();
+    // adjoint A operand: R^T[c = lr][n = 8 lg + j] as hi/lo
+    bf16x8 rh, rl;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      __bf16 hh, ll;
+      split(rs[w][lr][8 * lg + j], &hh, &ll);
+      rh[j] = hh;
+      rl[j] = ll;
+    }
+    const int64_t x2base = ((n0 >> 3) + lg) * Kp;
+#pragma unroll
+    for (int kb = 0; kb < 2 * KS; ++kb) {
+      const int64_t off = (x2base + 16 * kb + lr) * 8;
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(a.X2h + off);
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(a.X2l + off);
+      g[kb] = mma3(rh, rl, xh, xl, g[kb]);
+    }
+    __syncthreads();
+  }
+  // g[kb][j] = G^T[cb + 4 lg + j][16 kb + lr]
+  const int64_t pbase = static_cast<int64_t>(blockIdx.x) * a.Cp;
+#pragma unroll
+  for (int kb = 0; kb < 2 * KS; ++kb)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      a.gpart[(pbase + cb + 4 * lg + j) * Kp + 16 * kb + lr] = g[kb][j];
+  // row terms: reduce over the 16 lanes that held different rows
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      lp[j] += __shfl_xor(lp[j], off, 64);
+      sr[j] += __shfl_xor(sr[j], off, 64);
+      bad[j] += __shfl_xor(bad[j], off, 64);
+    }
+  }
+  if (lr == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double* sp = a.spart + (pbase + cb + 4 * lg + j) * 3;
+      sp[0] = lp[j];
+      sp[1] = sr[j];
+      sp[2] = bad[j];
+    }
+  }
+}
+
+// Per chain: FP64 sums of the block partials, then priors and output.
+template <int FAM>
+__global__ __launch_bounds__(256) void mx_reduce(mx_args a, pass_args pa) {
+  extern __shared__ double st[];
+  const int c = blockIdx.x;
+  pa.theta += static_cast<int64_t>(c) * pa.theta_stride;
+  pa.out += static_cast<int64_t>(c) * pa.out_stride;
+  const int K = pa.K;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    double v = 0.0;
+    for (int s = 0; s < a.splits; ++s)
+      v += static_cast<double>(a.gpart[(static_cast<int64_t>(s) * a.Cp + c) * a.Kp + k]);
+    st[2 + k] = v;
+  }
+  if (threadIdx.x < 3) {
+    double v = 0.0;
+    for (int s = 0; s < a.splits; ++s) v += a.spart[(static_cast<int64_t>(s) * a.Cp + c) * 3 + threadIdx.x];
+    st[threadIdx.x == 2 ? K + 2 : threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) finalize<FAM>(pa, st);
+}
+
+template <int KS>
+hipError_t launch_ks(int family, const mx_args& a, hipStream_t s) {
+  dim3 grid(a.splits, a.Cp / 64);
+  if (family == SX_LOGIT)
+    hipLaunchKernelGGL((glm_mfma<KS, SX_LOGIT>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((glm_mfma<KS, SX_NORMAL>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_split_x(const double* Xrm, uint16_t* X1h, uint16_t* X1l, uint16_t* X2h,
+                          uint16_t* X2l, int64_t N, int64_t K, int64_t Np, int64_t Kp,
+                          hipStream_t s) {
+  const int64_t total = Np * Kp;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(split_x, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
+                     Xrm, reinterpret_cast<__bf16*>(X1h), reinterpret_cast<__bf16*>(X1l),
+                     reinterpret_cast<__bf16*>(X2h), reinterpret_cast<__bf16*>(X2l), N, K, Kp,
+                     total);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, const mx_args& a,
+                            const pass_args& pa, hipStream_t s) {
+  hipLaunchKernelGGL(prep_chains, dim3(a.Cp), dim3(256), 0, s, theta, D, C, pa.K, a.Kp,
+                     reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bh)),
+                     reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bl)),
+                     const_cast<float*>(a.alpha));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  switch (a.Kp / 32) {
+    case 1: e = launch_ks<1>(family, a, s); break;
+    case 2: e = launch_ks<2>(family, a, s); break;
+    case 3: e = launch_ks<3>(family, a, s); break;
+    case 4: e = launch_ks<4>(family, a, s); break;
+    case 5: e = launch_ks<5>(family, a, s); break;
+    case 6: e = launch_ks<6>(family, a, s); break;
+    case 7: e = launch_ks<7>(family, a, s); break;
+    case 8: e = launch_ks<8>(family, a, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  const size_t lds = static_cast<size_t>(pa.E) * sizeof(double);
+  if (family == SX_LOGIT)
+    hipLaunchKernelGGL((mx_reduce<SX_LOGIT>), dim3(C), dim3(256), lds, s, a, pa);
+  else
+    hipLaunchKernelGGL((mx_reduce<SX_NORMAL>), dim3(C), dim3(256), lds, s, a, pa);
+  return hipGetLastError();
+}
+
+}  // namespace sx

@@ -65,7 +65,43 @@ hipError_t launch_pass_f64(int family, const tile_geom& g, const pass_args& a,
 hipError_t launch_tile_x_f64(const double* Xrm, double* Xt, int64_t N,
                              int64_t K, const tile_geom& g, hipStream_t s);
 
+// Arguments of the batched BF16 hi+lo MFMA pass.  BF16 arrays are stored
+// as uint16_t on the host side.
+struct mx_args {
+  const uint16_t *X1h, *X1l, *X2h, *X2l;  // k-packed and 8-row-packed X
+  const void* y;
+  const uint16_t *Bh, *Bl;  // Cp x Kp chain coefficients
+  const float* alpha;       // Cp intercepts
+  float* gpart;             // splits x Cp x Kp
+  double* spart;            // splits x Cp x 3
+  int64_t N, n_tiles, tiles_per_split;
+  int Kp, Cp, splits;
+};
+
+hipError_t launch_split_x(const double* Xrm, uint16_t* X1h, uint16_t* X1l, uint16_t* X2h,
+                          uint16_t* X2l, int64_t N, int64_t K, int64_t Np, int64_t Kp,
+                          hipStream_t s);
+// prep (theta -> bf16 coefficients), MFMA pass, FP64 reduction + finalize
+// into pa.out (stride pa.out_stride) for C chains
+hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C,
+                            const mx_args& a, const pass_args& pa, hipStream_t s);
+
+// Device workspace of batched MFMA evaluations for up to cap chains.
+struct mx_workspace {
+  int cap = 0, Cp = 0, splits = 0;
+  uint16_t *Bh = nullptr, *Bl = nullptr;
+  float* alpha = nullptr;
+  float* gpart = nullptr;
+  double* spart = nullptr;
+};
+
 }  // namespace sx
+
+struct sx_glm;
+namespace sx {
+int mx_enqueue(sx_glm* m, const double* d_theta, int C, double* d_out, int propto, int jacobian,
+               hipStream_t stream);
+}
 
 struct sx_glm {
   int family = 0, precision = 0, device = 0;
@@ -81,6 +117,13 @@ struct sx_glm {
   double* h_theta = nullptr;  // pinned
   double* h_out = nullptr;    // pinned
   int blocks = 0, threads = 256, max_blocks = 0;
+  // BF16 hi+lo layouts for batched MFMA evaluation (precision SX_BF16X2)
+  int64_t Np = 0, Kp = 0;
+  uint16_t *d_X1h = nullptr, *d_X1l = nullptr, *d_X2h = nullptr, *d_X2l = nullptr;
+  sx::mx_workspace mx;
+  double* d_btheta = nullptr;  // batched theta / out staging for the C-ABI batch call
+  double* d_bout = nullptr;
+  int64_t bcap = 0;
   std::string tuning_log;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

