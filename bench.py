@@ -65,6 +65,41 @@ def nuts_run(model, warmup, samples, seed=1234):
                 warmup=warmup, samples=samples)
 
 
+def batched_run(model, chains, warmup, samples, grad_precision, seed=1234):
+    """Device-resident NUTS over `chains` chains; min-ESS over pooled chains."""
+    import stan_amd as S
+    from stan_amd import analyze
+    res = S.hmc_nuts_diag_e_adapt_batch(model, chains, grad_precision=grad_precision,
+                                        random_seed=seed, chain=1, num_warmup=warmup,
+                                        num_samples=samples)
+    if not (res["status"] == 0).all():
+        raise RuntimeError(f"chain status {res['status']}")
+    q = np.transpose(res["draws"][:, :, 7:], (1, 0, 2))  # iterations x chains x params
+    ess = analyze.min_bulk_ess(q)
+    secs = res["seconds"]
+    return dict(chains=chains, min_ess=ess, seconds=secs, min_ess_per_s=ess / secs,
+                grad_evals=int(res["n_grad"].sum()),
+                grad_evals_per_s=float(res["n_grad"].sum()) / secs, steps=int(res["steps"]),
+                ms_per_step=1e3 * secs / max(1, res["steps"]),
+                mean_n_leapfrog=float(res["draws"][:, :, 4].mean()),
+                stepsize_median=float(np.median(res["stepsize"])), warmup=warmup,
+                samples=samples)
+
+
+def mixed_error(model, ref_data, family, thetas):
+    """Max relative lp error and normwise gradient error of the batched BF16
+    hi+lo pass against the FP64 oracle at the given points."""
+    import oracle as O
+    ref = O.GlmData(family, ref_data["X"], ref_data["y"], prior=ref_data["prior"])
+    lp, g = model.log_prob_grad_batch(thetas)
+    e_lp = e_g = 0.0
+    for c in range(len(thetas)):
+        st, lp_r, g_r = ref.lpgrad(thetas[c], mode=1)
+        e_lp = max(e_lp, abs(lp[c] - lp_r) / abs(lp_r))
+        e_g = max(e_g, float(np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r)))
+    return e_lp, e_g
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +107,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nuts", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--chains", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -134,13 +171,50 @@ def main():
         "roofline": roofline,
         "host_api_ms_per_eval": round(ms_host, 4),
     }
+    warm, samp = 200, 300
     if not args.no_nuts:
-        log("plain NUTS run (1 chain, 200 warmup + 300 samples)")
-        result["nuts_plain_fp64"] = nuts_run(m, 200, 300)
+        log(f"plain NUTS run (1 chain, FP64, {warm} warmup + {samp} samples)")
+        result["nuts_plain_fp64"] = nuts_run(m, warm, samp)
+    m.close()
+    if not args.no_batched:
+        log(f"batched mixed-precision NUTS on the same data ({args.chains} chains)")
+        mb = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"], precision=S.BF16X2)
+        C = args.chains
+        th = W.random_thetas(mb.D, C, seed=7)
+        ms_b, _ = mb.time_log_prob_grad(th, iters=20, warmup=3, chains=C)
+        bat = batched_run(mb, C, warm, samp, S.BF16X2)
+        bat["ms_per_batched_eval"] = ms_b
+        Kp = (K + 31) // 32 * 32
+        bat["mfma_tflops_algorithmic"] = 4.0 * N * K * C / (ms_b * 1e-3) / 1e12
+        bat["mfma_tflops_issued"] = 3 * 4.0 * N * Kp * C / (ms_b * 1e-3) / 1e12
+        if not args.no_nuts:
+            bat["min_ess_per_s_vs_plain"] = (bat["min_ess_per_s"] /
+                                            result["nuts_plain_fp64"]["min_ess_per_s"])
+        log("mixed-precision error vs FP64 oracle at 3 points")
+        e_lp, e_g = mixed_error(mb, d, S.LOGIT, th[:3] * 0.5)
+        bat["max_rel_err_lp"] = e_lp
+        bat["max_rel_err_grad"] = e_g
+        result["nuts_batched_mixed"] = bat
+        mb.close()
+
+        log("normal GLM, N=2e5, K=256, 256 chains (500 warmup + 500 samples)")
+        dn = W.normal_batched()
+        mn = S.GlmModel(S.NORMAL, dn["X"], dn["y"], prior=dn["prior"], precision=S.BF16X2)
+        thn = W.random_thetas(mn.D, 256, seed=8) * 0.5
+        ms_n, _ = mn.time_log_prob_grad(thn, iters=20, warmup=3, chains=256)
+        bn = batched_run(mn, 256, 500, 500, S.BF16X2)
+        Nn, Kn = dn["X"].shape
+        bn["ms_per_batched_eval"] = ms_n
+        bn["mfma_tflops_algorithmic"] = 4.0 * Nn * Kn * 256 / (ms_n * 1e-3) / 1e12
+        bn["mfma_frac_bf16_peak"] = 3 * bn["mfma_tflops_algorithmic"] / 2500.0
+        e_lp, e_g = mixed_error(mn, dn, S.NORMAL, thn[:4])
+        bn["max_rel_err_lp"] = e_lp
+        bn["max_rel_err_grad"] = e_g
+        result["normal_glm_batched_256"] = bn
+        mn.close()
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 core)")
         result["cpu_baseline"] = cpu_baseline(d, theta)
-    m.close()
     print(json.dumps(result))
 
 
