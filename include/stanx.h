@@ -66,7 +66,9 @@ extern "C" {
 
 /* storage/arithmetic precision of the bulk X pass */
 #define SX_FP64 0   /* FP64 X, FP64 arithmetic: the plain drop-in           */
-#define SX_FP32 1   /* FP32 X, FP32 products, FP64 block reductions         */
+#define SX_FP32 1   /* SX_HIER_LOGIT: FP64 X for single evaluations, plus X
+                        in FP32 for batched evaluations (FP32 products and
+                        chunk sums, FP64 sums over chunks and groups)   */
 #define SX_BF16X2 2 /* FP64 X for single evaluations, plus X as BF16 hi+lo
                         for batched MFMA evaluations (K <= 256)        */
 
@@ -120,6 +122,9 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y,
 const char* sx_glm_tuning_log(const sx_glm* m);
 void sx_glm_destroy(sx_glm* m);
 int sx_glm_num_params(const sx_glm* m, int64_t* D);
+/* data sizes the context was created with (any pointer may be NULL);
+ * a generated model knows these from its data block                   */
+int sx_glm_dims(const sx_glm* m, int64_t* N, int64_t* K, int64_t* J);
 /* family the context was created with (SX_LOGIT, ...), or -1 */
 int sx_glm_family(const sx_glm* m);
 const char* sx_last_error(void);
@@ -134,7 +139,9 @@ int sx_glm_logp(sx_glm* m, const double* theta, int propto, int jacobian,
 /* C chains at once: theta is C x D (row-major), lp C, grad C x D.  On a
  * context created with SX_BF16X2 all chains are evaluated together by the
  * MFMA pass (X as BF16 hi+lo, FP32 accumulation, FP64 cross-block sums);
- * otherwise chain by chain in FP64.  A chain whose density cannot be
+ * SX_HIER_LOGIT evaluates all chains in one pass (FP32 with SX_FP32, else
+ * FP64, each chain's result bitwise its single evaluation); otherwise
+ * chain by chain in FP64.  A chain whose density cannot be
  * evaluated gets lp = -inf and the call returns SX_EDOMAIN.             */
 int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C,
                            int propto, int jacobian, double* lp,
@@ -145,7 +152,9 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C,
  * bracketed by HIP events on the context's stream.  ms_per_eval: mean
  * time of a full evaluation (bulk X pass + cross-block reduction).
  * ms_kernel: mean duration of the bulk X pass alone, from a second loop
- * that launches only that kernel.  C must be 1.                       */
+ * that launches only that kernel.  C > 1 times the batched evaluation of
+ * sx_glm_logp_grad_batch (SX_BF16X2 contexts and SX_HIER_LOGIT); then
+ * ms_kernel = ms_per_eval.                                              */
 int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C,
                           int iters, int warmup, double* ms_per_eval,
                           double* ms_kernel);

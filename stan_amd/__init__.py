@@ -20,6 +20,7 @@ FP64, FP32, BF16X2 = 0, 1, 2
 # every entry point declared in include/stanx.h
 EXPORTED = [
     "sx_glm_create", "sx_glm_create_ex", "sx_glm_tuning_log", "sx_glm_destroy", "sx_glm_num_params", "sx_glm_family",
+    "sx_glm_dims",
     "sx_last_error", "sx_glm_logp_grad", "sx_glm_logp", "sx_glm_logp_grad_batch",
     "sx_glm_time_logp_grad", "sx_glm_kernel_config", "sx_hmc_nuts_diag_e_adapt",
     "sx_hmc_nuts_diag_e_adapt_cb", "sx_nuts_batch_create", "sx_nuts_batch_run",
@@ -88,6 +89,7 @@ def lib():
         L.sx_glm_destroy.restype = None
         L.sx_glm_num_params.argtypes = [vp, C.POINTER(i64)]
         L.sx_glm_family.argtypes = [vp]
+        L.sx_glm_dims.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
         L.sx_last_error.restype = C.c_char_p
         L.sx_glm_logp_grad.argtypes = [vp, vp, C.c_int, C.c_int, dp, vp]
         L.sx_glm_logp.argtypes = [vp, vp, C.c_int, C.c_int, dp]
@@ -130,9 +132,11 @@ def _ptr(a):
 class GlmModel:
     """A GLM log density with its data resident on the GPU.
 
-    family: LOGIT, NORMAL (HIER_LOGIT needs ``offsets``).  X is N x K,
-    y int {0,1} (logistic) or float (normal).  Theta layouts are those of
-    include/stanx.h.
+    family: LOGIT, NORMAL, or HIER_LOGIT with ``offsets`` (J + 1 CSR row
+    offsets of the groups).  X is N x K, y int {0,1} (logistic) or float
+    (normal).  precision: FP64; BF16X2 (LOGIT/NORMAL: batched evaluations on
+    MFMA); FP32 (HIER_LOGIT: batched evaluations in FP32).  Theta layouts
+    are those of include/stanx.h.
     """
 
     def __init__(self, family, X, y, offsets=None, precision=FP64,

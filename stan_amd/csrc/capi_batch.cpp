@@ -31,6 +31,8 @@ struct sx_nuts_batch {
   double* d_out = nullptr;
   double* d_theta = nullptr;
   double* d_inits = nullptr;
+  sx::mx_workspace mx;     // this sampler's MFMA workspace
+  sx::hier_workspace hw;   // this sampler's hierarchical partial sums
   unsigned* h_done = nullptr;  // pinned
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -64,8 +66,12 @@ void window_params(unsigned num_warmup, unsigned init_buffer, unsigned term_buff
 }
 
 int enqueue_gradient(sx_nuts_batch* b) {
+  if (b->m->family == SX_HIER_LOGIT)
+    return sx::hier_enqueue(b->m, b->hw, b->d_theta, static_cast<int>(b->C), b->d_out, 1, 1,
+                            b->grad_precision == SX_FP32, b->stream);
   if (b->grad_precision == SX_BF16X2)
-    return sx::mx_enqueue(b->m, b->d_theta, static_cast<int>(b->C), b->d_out, 1, 1, b->stream);
+    return sx::mx_enqueue(b->m, b->mx, b->d_theta, static_cast<int>(b->C), b->d_out, 1, 1,
+                          b->stream);
   SX_HIP(sx::launch_pass_f64(b->m->family, b->m->geom, b->g, b->grad_blocks, b->m->threads,
                              b->stream, true));
   return SX_OK;
@@ -89,6 +95,8 @@ void release(sx_nuts_batch* b) {
   (void)hipFree(b->d_out);
   (void)hipFree(b->d_theta);
   (void)hipFree(b->d_inits);
+  sx::free_workspace(b->mx);
+  sx::free_workspace(b->hw);
   if (b->h_done) (void)hipHostFree(b->h_done);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -106,8 +114,10 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
     return SX_EINVAL;
   }
   *out = nullptr;
-  if (grad_precision != SX_FP64 && !(grad_precision == SX_BF16X2 && m->precision == SX_BF16X2)) {
-    set_error("sx_nuts_batch_create: SX_BF16X2 gradients need a context created with SX_BF16X2");
+  if (grad_precision != SX_FP64 && !(grad_precision == SX_BF16X2 && m->precision == SX_BF16X2) &&
+      !(grad_precision == SX_FP32 && m->precision == SX_FP32)) {
+    set_error("sx_nuts_batch_create: mixed-precision gradients need a context created with the "
+              "same precision");
     return SX_EINVAL;
   }
   const int max_depth = cfg->max_depth > 0 ? cfg->max_depth : 5;
@@ -222,7 +232,7 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
   a.w_init_buffer = ib;
   a.w_term_buffer = tb;
   // batched FP64 gradient: every chain's pass on grid.y
-  {
+  if (m->family != SX_HIER_LOGIT) {
     // each chain's pass uses the launch shape of the single-chain pass: a
     // chain's gradient is bitwise the plain path's, and independent of the
     // other chains in the batch
@@ -257,9 +267,14 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
     g.beta_const = g.beta_sd > 0 ? -std::log(g.beta_sd) - h : 0.0;
     g.sigma_const = g.sigma_sd > 0 ? -std::log(g.sigma_sd) - h : 0.0;
   }
-  if (grad_precision == SX_BF16X2) {
+  if (m->family == SX_HIER_LOGIT) {
+    // allocate the partial sums outside the captured region
+    const int rcp = sx::hier_prepare(m, b->hw, static_cast<int>(chains));
+    if (rcp != SX_OK) return fail(rcp, "hierarchical workspace");
+  } else if (grad_precision == SX_BF16X2) {
     // allocate the MFMA workspace outside the captured region
-    const int rcp = sx::mx_enqueue(m, b->d_theta, static_cast<int>(chains), b->d_out, 1, 1, b->stream);
+    const int rcp = sx::mx_enqueue(m, b->mx, b->d_theta, static_cast<int>(chains), b->d_out, 1,
+                                   1, b->stream);
     if (rcp != SX_OK) return fail(rcp, "MFMA workspace");
   }
   // capture a fixed run of lockstep steps in a graph
@@ -324,11 +339,20 @@ int sx_nuts_batch_results(sx_nuts_batch* b, double* draws, int64_t* rows, int* s
   if (draws) {
     SX_HIP(hipMemcpy(draws, b->a.draws, sizeof(double) * b->C * b->rows_cap * ncol,
                      hipMemcpyDeviceToHost));
-    // write_array: constrained parameters (sigma = exp(u) for the normal GLM)
-    if (b->m->family == SX_NORMAL)
-      for (int64_t c = 0; c < b->C; ++c)
-        for (int64_t r = 0; r < st[c].rows; ++r) {
-          double& u = draws[(c * b->rows_cap + r) * ncol + ncol - 1];
+    // write_array: constrained parameters (sigma = exp(u) for the normal
+    // GLM, tau_k = exp(v_k) for the hierarchical model)
+    int64_t c0 = 0, c1 = 0;
+    if (b->m->family == SX_NORMAL) {
+      c0 = ncol - 1;
+      c1 = ncol;
+    } else if (b->m->family == SX_HIER_LOGIT) {
+      c0 = 7 + 1 + b->m->K;
+      c1 = c0 + b->m->K + 1;
+    }
+    for (int64_t c = 0; c < b->C; ++c)
+      for (int64_t r = 0; r < st[c].rows; ++r)
+        for (int64_t j = c0; j < c1; ++j) {
+          double& u = draws[(c * b->rows_cap + r) * ncol + j];
           u = std::exp(u);
         }
   }

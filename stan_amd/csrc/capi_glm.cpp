@@ -134,6 +134,42 @@ int autotune(sx_glm* m, const double* d_rowmajor, int cus) {
   return SX_OK;
 }
 
+// Hierarchical model: cut every group into chunks of at most kHierChunk
+// rows (a chunk never straddles a group), upload the chunk tables, an FP32
+// copy of X for batched evaluations (SX_FP32), and the single-chain
+// partials.
+int hier_setup(sx_glm* m, const double* d_X, const double* X, const int64_t* off) {
+  std::vector<int> cg;
+  std::vector<int64_t> cr0, gc0;
+  gc0.reserve(static_cast<size_t>(m->J) + 1);
+  for (int64_t j = 0; j < m->J; ++j) {
+    gc0.push_back(static_cast<int64_t>(cg.size()));
+    for (int64_t r = off[j]; r < off[j + 1]; r += sx::kHierChunk) {
+      cg.push_back(static_cast<int>(j));
+      cr0.push_back(r);
+    }
+  }
+  gc0.push_back(static_cast<int64_t>(cg.size()));
+  cr0.push_back(m->N);
+  m->n_chunks = static_cast<int64_t>(cg.size());
+  (void)d_X;
+  SX_HIP(hipMalloc(&m->d_chunk_group, sizeof(int) * std::max<size_t>(1, cg.size())));
+  SX_HIP(hipMalloc(&m->d_chunk_row0, sizeof(int64_t) * cr0.size()));
+  SX_HIP(hipMalloc(&m->d_group_chunk0, sizeof(int64_t) * gc0.size()));
+  if (!cg.empty())
+    SX_HIP(hipMemcpy(m->d_chunk_group, cg.data(), sizeof(int) * cg.size(), hipMemcpyHostToDevice));
+  SX_HIP(hipMemcpy(m->d_chunk_row0, cr0.data(), sizeof(int64_t) * cr0.size(), hipMemcpyHostToDevice));
+  SX_HIP(hipMemcpy(m->d_group_chunk0, gc0.data(), sizeof(int64_t) * gc0.size(), hipMemcpyHostToDevice));
+  if (m->precision == SX_FP32 && m->N * m->K > 0) {
+    const size_t n = static_cast<size_t>(m->N * m->K);
+    std::vector<float> x32(n);
+    for (size_t i = 0; i < n; ++i) x32[i] = static_cast<float>(X[i]);
+    SX_HIP(hipMalloc(&m->d_X32, n * sizeof(float)));
+    SX_HIP(hipMemcpy(m->d_X32, x32.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return sx::hier_prepare(m, m->hw, 1);
+}
+
 }  // namespace
 
 extern "C" {
@@ -155,13 +191,34 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     return SX_EINVAL;
   }
   *out = nullptr;
-  if (family != SX_LOGIT && family != SX_NORMAL) {
-    set_error("sx_glm_create: family not supported by this build");
+  if (family != SX_LOGIT && family != SX_NORMAL && family != SX_HIER_LOGIT) {
+    set_error("sx_glm_create: unknown family");
     return SX_EINVAL;
   }
-  if (precision != SX_FP64 && precision != SX_BF16X2) {
+  const bool hier = family == SX_HIER_LOGIT;
+  if ((!hier && precision != SX_FP64 && precision != SX_BF16X2) ||
+      (hier && precision != SX_FP64 && precision != SX_FP32)) {
     set_error("sx_glm_create: precision not supported for this family");
     return SX_EINVAL;
+  }
+  if (hier) {
+    if (K > 32) {
+      set_error("sx_glm_create: the hierarchical model supports K <= 32");
+      return SX_EINVAL;
+    }
+    if (J < 1 || !group_offsets) {
+      set_error("sx_glm_create: the hierarchical model needs J >= 1 group offsets");
+      return SX_EINVAL;
+    }
+    if (group_offsets[0] != 0 || group_offsets[J] != N) {
+      set_error("sx_glm_create: group offsets must start at 0 and end at N");
+      return SX_EINVAL;
+    }
+    for (int64_t j = 0; j < J; ++j)
+      if (group_offsets[j + 1] < group_offsets[j]) {
+        set_error("sx_glm_create: group offsets must be non-decreasing");
+        return SX_EINVAL;
+      }
   }
   if (precision == SX_BF16X2 && K > 256) {
     set_error("sx_glm_create: BF16 hi+lo batched evaluation supports K <= 256");
@@ -171,7 +228,6 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     set_error("sx_glm_create: invalid sizes or NULL data");
     return SX_EINVAL;
   }
-  (void)group_offsets;
   sx_tuning tune{0, 0, -1, 0, -1};
   if (tuning) tune = *tuning;
   if (tune.lanes_per_row < 0 || tune.lanes_per_row > 64 ||
@@ -183,11 +239,11 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
   }
   // data checks the reference's GLM functions perform (y in {0,1},
   // finite y and X) happen once here instead of per evaluation
-  if (family == SX_LOGIT) {
+  if (family == SX_LOGIT || hier) {
     const int32_t* yi = static_cast<const int32_t*>(y);
     for (int64_t n = 0; n < N; ++n)
       if (yi[n] != 0 && yi[n] != 1) {
-        set_error("bernoulli_logit_glm: y[" + std::to_string(n) + "] is " +
+        set_error(std::string(hier ? "bernoulli_logit" : "bernoulli_logit_glm") + ": y[" + std::to_string(n) + "] is " +
                   std::to_string(yi[n]) + ", but must be in {0, 1}");
         return SX_EINVAL;
       }
@@ -221,7 +277,7 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
   m->N = N;
   m->K = K;
   m->J = J;
-  m->D = family == SX_LOGIT ? K + 1 : K + 2;
+  m->D = family == SX_LOGIT ? K + 1 : family == SX_NORMAL ? K + 2 : 2 * K + 2 + J * (K + 1);
   m->prior = prior ? *prior : sx_prior{5.0, 2.0, 2.0, 1.0};
   auto fail = [&](int code) {
     sx_glm_destroy(m);
@@ -232,7 +288,7 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     delete m;
     return SX_EDEVICE;
   }
-  if (family == SX_LOGIT) {
+  if (family == SX_LOGIT || hier) {
     std::vector<int8_t> y8(static_cast<size_t>(std::max<int64_t>(N, 1)));
     const int32_t* yi = static_cast<const int32_t*>(y);
     for (int64_t n = 0; n < N; ++n) y8[n] = static_cast<int8_t>(yi[n]);
@@ -270,6 +326,17 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
       set_error("X upload failed");
       return fail(SX_EDEVICE);
     }
+  }
+  if (hier) {
+    rc = hier_setup(m, staging, X, group_offsets);
+    if (rc == SX_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = SX_EDEVICE;
+    if (rc) {
+      if (staging) (void)hipFree(staging);
+      return fail(rc);
+    }
+    m->d_X = staging;  // the FP64 pass reads X row-major
+    *out = m;
+    return SX_OK;
   }
   const bool do_tune =
       tune.autotune == 1 || (tune.autotune == -1 && tuning == nullptr && N * K >= (1 << 22));
@@ -324,13 +391,14 @@ void sx_glm_destroy(sx_glm* m) {
   (void)hipFree(m->d_X1l);
   (void)hipFree(m->d_X2h);
   (void)hipFree(m->d_X2l);
-  (void)hipFree(m->mx.Bh);
-  (void)hipFree(m->mx.Bl);
-  (void)hipFree(m->mx.alpha);
-  (void)hipFree(m->mx.gpart);
-  (void)hipFree(m->mx.spart);
+  sx::free_workspace(m->mx);
   (void)hipFree(m->d_btheta);
   (void)hipFree(m->d_bout);
+  (void)hipFree(m->d_X32);
+  (void)hipFree(m->d_chunk_group);
+  (void)hipFree(m->d_chunk_row0);
+  (void)hipFree(m->d_group_chunk0);
+  sx::free_workspace(m->hw);
   if (m->h_theta) (void)hipHostFree(m->h_theta);
   if (m->h_out) (void)hipHostFree(m->h_out);
   if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -350,11 +418,24 @@ int sx_glm_num_params(const sx_glm* m, int64_t* D) {
 
 int sx_glm_family(const sx_glm* m) { return m ? m->family : -1; }
 
+int sx_glm_dims(const sx_glm* m, int64_t* N, int64_t* K, int64_t* J) {
+  if (!m) {
+    set_error("sx_glm_dims: NULL model");
+    return SX_EINVAL;
+  }
+  if (N) *N = m->N;
+  if (K) *K = m->K;
+  if (J) *J = m->J;
+  return SX_OK;
+}
+
 }  // extern "C"
 
 namespace {
 
 int enqueue_pass(sx_glm* m, int propto, int jacobian, bool reduce) {
+  if (m->family == SX_HIER_LOGIT)
+    return sx::hier_enqueue(m, m->hw, m->d_theta, 1, m->d_out, propto, jacobian, false, m->stream);
   sx::pass_args a;
   a.X = m->d_X;
   a.y = m->d_y;
@@ -406,9 +487,10 @@ int eval(sx_glm* m, const double* theta, int propto, int jacobian, double* lp,
   *lp = m->h_out[0];
   if (grad) std::memcpy(grad, m->h_out + 1, D * sizeof(double));
   if (m->h_out[D + 1] != 0.0) {
-    set_error(m->family == SX_LOGIT
-                  ? "bernoulli_logit_glm_lpmf: linear predictor or log density is not finite"
-                  : "normal_id_glm_lpdf: scale, linear predictor or log density is not finite");
+    set_error(m->family == SX_LOGIT ? "bernoulli_logit_glm_lpmf: linear predictor or log density is not finite"
+              : m->family == SX_NORMAL
+                  ? "normal_id_glm_lpdf: scale, linear predictor or log density is not finite"
+                  : "bernoulli_logit_lpmf: linear predictor or log density is not finite");
     return SX_EDOMAIN;
   }
   return SX_OK;
@@ -421,22 +503,31 @@ namespace sx {
 // Batched MFMA evaluation of C chains: theta (device, C x D) -> out
 // (device, C x (D + 2)).  Row splits depend on N only, so each chain's
 // result is independent of the other chains in the batch.
-int mx_prepare(sx_glm* m, int C) {
-  if (!m->d_X1h) {
-    set_error("batched BF16 evaluation needs a context created with SX_BF16X2");
-    return SX_EINVAL;
-  }
-  const int Cp = (C + 63) / 64 * 64;
-  mx_workspace& w = m->mx;
-  const int64_t n_tiles = m->Np / 32;
-  int splits = static_cast<int>(std::min<int64_t>(512, std::max<int64_t>(8, ((n_tiles + 7) / 8 + 7) / 8 * 8)));
-  if (w.cap >= Cp && w.splits == splits) return SX_OK;
+void free_workspace(mx_workspace& w) {
   (void)hipFree(w.Bh);
   (void)hipFree(w.Bl);
   (void)hipFree(w.alpha);
   (void)hipFree(w.gpart);
   (void)hipFree(w.spart);
   w = mx_workspace();
+}
+
+void free_workspace(hier_workspace& w) {
+  (void)hipFree(w.cpart);
+  (void)hipFree(w.gpart);
+  w = hier_workspace();
+}
+
+int mx_prepare(sx_glm* m, mx_workspace& w, int C) {
+  if (!m->d_X1h) {
+    set_error("batched BF16 evaluation needs a context created with SX_BF16X2");
+    return SX_EINVAL;
+  }
+  const int Cp = (C + 63) / 64 * 64;
+  const int64_t n_tiles = m->Np / 32;
+  int splits = static_cast<int>(std::min<int64_t>(512, std::max<int64_t>(8, ((n_tiles + 7) / 8 + 7) / 8 * 8)));
+  if (w.cap >= Cp && w.splits == splits) return SX_OK;
+  free_workspace(w);
   const size_t ck = static_cast<size_t>(Cp) * m->Kp;
   SX_HIP(hipMalloc(&w.Bh, ck * 2));
   SX_HIP(hipMalloc(&w.Bl, ck * 2));
@@ -449,9 +540,9 @@ int mx_prepare(sx_glm* m, int C) {
   return SX_OK;
 }
 
-int mx_enqueue(sx_glm* m, const double* d_theta, int C, double* d_out, int propto, int jacobian,
-               hipStream_t stream) {
-  int rc = mx_prepare(m, C);
+int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double* d_out,
+               int propto, int jacobian, hipStream_t stream) {
+  int rc = mx_prepare(m, w, C);
   if (rc) return rc;
   mx_args a{};
   a.X1h = m->d_X1h;
@@ -459,16 +550,16 @@ int mx_enqueue(sx_glm* m, const double* d_theta, int C, double* d_out, int propt
   a.X2h = m->d_X2h;
   a.X2l = m->d_X2l;
   a.y = m->d_y;
-  a.Bh = m->mx.Bh;
-  a.Bl = m->mx.Bl;
-  a.alpha = m->mx.alpha;
-  a.gpart = m->mx.gpart;
-  a.spart = m->mx.spart;
+  a.Bh = w.Bh;
+  a.Bl = w.Bl;
+  a.alpha = w.alpha;
+  a.gpart = w.gpart;
+  a.spart = w.spart;
   a.N = m->N;
   a.n_tiles = m->Np / 32;
   a.Kp = static_cast<int>(m->Kp);
   a.Cp = (C + 63) / 64 * 64;
-  a.splits = m->mx.splits;
+  a.splits = w.splits;
   a.tiles_per_split = (a.n_tiles + a.splits - 1) / a.splits;
   pass_args pa{};
   pa.theta = d_theta;
@@ -490,6 +581,59 @@ int mx_enqueue(sx_glm* m, const double* d_theta, int C, double* d_out, int propt
   pa.beta_const = pa.beta_sd > 0 ? -std::log(pa.beta_sd) - h : 0.0;
   pa.sigma_const = pa.sigma_sd > 0 ? -std::log(pa.sigma_sd) - h : 0.0;
   SX_HIP(launch_mfma_pass(m->family, d_theta, m->D, C, a, pa, stream));
+  return SX_OK;
+}
+
+// Hierarchical model, C chains: theta (device, C x D) -> out (device,
+// C x (D + 2)); FP64 X, or FP32 X with chains on lanes.
+int hier_prepare(sx_glm* m, hier_workspace& w, int C) {
+  if (w.cap >= C) return SX_OK;
+  free_workspace(w);
+  const size_t E = static_cast<size_t>(m->K) + 3, G = 2 * static_cast<size_t>(m->K) + 5;
+  SX_HIP(hipMalloc(&w.cpart, sizeof(double) * E * std::max<int64_t>(1, m->n_chunks) * C));
+  SX_HIP(hipMalloc(&w.gpart, sizeof(double) * G * m->J * C));
+  w.cap = C;
+  return SX_OK;
+}
+
+int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, double* d_out,
+                 int propto, int jacobian, bool fp32, hipStream_t stream) {
+  if (fp32 && !m->d_X32 && m->N * m->K > 0) {
+    set_error("FP32 evaluation needs a context created with SX_FP32");
+    return SX_EINVAL;
+  }
+  int rc = hier_prepare(m, w, C);
+  if (rc) return rc;
+  hier_args a{};
+  a.X = m->d_X;
+  a.X32 = m->d_X32;
+  a.y = static_cast<const int8_t*>(m->d_y);
+  a.theta = d_theta;
+  a.out = d_out;
+  a.cpart = w.cpart;
+  a.gpart = w.gpart;
+  a.chunk_group = m->d_chunk_group;
+  a.chunk_row0 = m->d_chunk_row0;
+  a.group_chunk0 = m->d_group_chunk0;
+  a.n_chunks = m->n_chunks;
+  a.theta_stride = m->D;
+  a.out_stride = m->D + 2;
+  a.J = static_cast<int>(m->J);
+  a.K = static_cast<int>(m->K);
+  a.D = static_cast<int>(m->D);
+  a.E = static_cast<int>(m->K) + 3;
+  a.G = 2 * static_cast<int>(m->K) + 5;
+  a.chains = C;
+  a.propto = propto;
+  a.jacobian = jacobian;
+  a.alpha_sd = m->prior.alpha_sd;
+  a.beta_sd = m->prior.beta_sd;
+  a.tau_sd = m->prior.tau_sd;
+  const double h = 0.91893853320467274178032973640562;
+  a.alpha_const = a.alpha_sd > 0 ? -std::log(a.alpha_sd) - h : 0.0;
+  a.beta_const = a.beta_sd > 0 ? -std::log(a.beta_sd) - h : 0.0;
+  a.tau_const = a.tau_sd > 0 ? -std::log(a.tau_sd) - h : 0.0;
+  SX_HIP(launch_hier(a, fp32, stream));
   return SX_OK;
 }
 
@@ -530,15 +674,18 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C, int propto
     return SX_EINVAL;
   }
   if (C == 0) return SX_OK;
-  if (m->precision == SX_BF16X2) {
+  if (m->precision == SX_BF16X2 || m->family == SX_HIER_LOGIT) {
     SX_HIP(hipSetDevice(m->device));
     int rc = ensure_batch_staging(m, C);
     if (rc) return rc;
     const int64_t D = m->D;
     SX_HIP(hipMemcpyAsync(m->d_btheta, theta, sizeof(double) * C * D, hipMemcpyHostToDevice,
                           m->stream));
-    rc = sx::mx_enqueue(m, m->d_btheta, static_cast<int>(C), m->d_bout, propto, jacobian,
-                        m->stream);
+    rc = m->family == SX_HIER_LOGIT
+             ? sx::hier_enqueue(m, m->hw, m->d_btheta, static_cast<int>(C), m->d_bout, propto,
+                                jacobian, m->precision == SX_FP32, m->stream)
+             : sx::mx_enqueue(m, m->mx, m->d_btheta, static_cast<int>(C), m->d_bout, propto,
+                              jacobian, m->stream);
     if (rc) return rc;
     std::vector<double> out(static_cast<size_t>(C * (D + 2)));
     SX_HIP(hipMemcpyAsync(out.data(), m->d_bout, sizeof(double) * out.size(),
@@ -552,7 +699,9 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C, int propto
       if (o[D + 1] != 0.0) {
         lp[c] = -INFINITY;
         worst = SX_EDOMAIN;
-        set_error("batched GLM: linear predictor or log density is not finite");
+        set_error(m->family == SX_HIER_LOGIT
+                      ? "bernoulli_logit_lpmf: linear predictor or log density is not finite"
+                      : "batched GLM: linear predictor or log density is not finite");
       }
     }
     return worst;
@@ -573,7 +722,8 @@ int sx_glm_logp_grad_batch(sx_glm* m, const double* theta, int64_t C, int propto
 
 int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C, int iters, int warmup,
                           double* ms_per_eval, double* ms_kernel) {
-  if (!m || !theta || iters <= 0 || C < 1 || (C > 1 && m->precision != SX_BF16X2)) {
+  if (!m || !theta || iters <= 0 || C < 1 ||
+      (C > 1 && m->precision != SX_BF16X2 && m->family != SX_HIER_LOGIT)) {
     set_error("sx_glm_time_logp_grad: invalid argument");
     return SX_EINVAL;
   }
@@ -588,7 +738,11 @@ int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C, int iters, 
     SX_HIP(hipMemcpy(m->d_theta, theta, D * sizeof(double), hipMemcpyHostToDevice));
   }
   auto one = [&](bool reduce) -> int {
-    if (batched) return sx::mx_enqueue(m, m->d_btheta, static_cast<int>(C), m->d_bout, 1, 1, m->stream);
+    if (batched && m->family == SX_HIER_LOGIT)
+      return sx::hier_enqueue(m, m->hw, m->d_btheta, static_cast<int>(C), m->d_bout, 1, 1,
+                              m->precision == SX_FP32, m->stream);
+    if (batched)
+      return sx::mx_enqueue(m, m->mx, m->d_btheta, static_cast<int>(C), m->d_bout, 1, 1, m->stream);
     return enqueue_pass(m, 1, 1, reduce);
   };
   for (int i = 0; i < warmup; ++i) {

@@ -1,0 +1,315 @@
+// Hierarchical logistic regression with ragged groups (segmented
+// reductions), single chain in FP64 and many chains in FP32.
+//
+// Model (theta = [alpha, beta_1..K, v_0..v_K, z_{1,0..K} .. z_{J,0..K}],
+// tau_k = exp(v_k)):
+//   eta_n = alpha + tau_0 z_{g,0} + sum_k x_nk (beta_k + tau_k z_{g,k}),  g = group(n)
+//   y_n ~ bernoulli_logit(eta_n), z ~ normal(0, 1), tau ~ half-normal(0, tau_sd),
+//   alpha ~ normal(0, alpha_sd), beta ~ normal(0, beta_sd), log-Jacobian sum v_k.
+// The gradient needs the segment sums S_{g,0} = sum_{n in g} r_n and
+// S_{g,k} = sum_{n in g} x_nk r_n (r = y - inv_logit(eta)):
+//   d/dz_{g,k} = tau_k S_{g,k} - z_{g,k},  d/dbeta_k = sum_g S_{g,k},
+//   d/dalpha = sum_g S_{g,0},  d/dv_k = tau_k sum_g z_{g,k} S_{g,k} - tau_k^2/tau_sd^2 + 1.
+// (Specification analogue: src/test/test-models/good/variational/hier_logistic.stan.)
+//
+// Rows are cut into chunks of at most `chunk` rows that never straddle a
+// group, so ragged group sizes become uniform work items and the largest
+// group does not serialise the pass.  Every chunk yields FP64 partials
+// [S_0..S_K, lp, bad]; a per-group launch sums its chunks in order (z
+// gradients), a final launch sums over groups (alpha, beta, v, priors).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../include/stanx/sx_math.hpp"
+#include "glm_finalize.hpp"
+#include "sx_internal.hpp"
+
+namespace sx {
+
+namespace {
+
+constexpr double kHalfLog2PiH = 0.91893853320467274178032973640562;
+
+// FP64: a wave per chunk, rows on lanes; chains on grid.y.  A chunk's sums
+// are formed by one wave in a fixed order, so each chain's result does not
+// depend on the launch shape or on the other chains.
+template <int KMAX>
+__global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
+  __shared__ double sw[4][KMAX + 1];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int K = a.K;
+  const int c = blockIdx.y;
+  const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
+  const int64_t W = static_cast<int64_t>(gridDim.x) * 4;
+  for (int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w; ch < a.n_chunks; ch += W) {
+    const int g = a.chunk_group[ch];
+    const int64_t r0 = a.chunk_row0[ch], r1 = a.chunk_row0[ch + 1];
+    const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+    if (lane <= K) {
+      const double tau = exp(th[1 + K + lane]);
+      sw[w][lane] = lane == 0 ? th[0] + tau * z[0] : th[lane] + tau * z[lane];
+    }
+    // sw[w] is this wave's: wait for the LDS writes, then a wave barrier
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    double acc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) acc[k] = 0.0;
+    double s0 = 0.0, s1 = 0.0, bad = 0.0;
+    for (int64_t base = r0; base < r1; base += 64) {
+      const int64_t n = base + lane;
+      const bool valid = n < r1;
+      const double* x = a.X + n * K;
+      double e = sw[w][0];
+      double xv[KMAX];
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        xv[k] = (valid && k < K) ? x[k] : 0.0;
+        e = fma(xv[k], k < K ? sw[w][k + 1] : 0.0, e);
+      }
+      double term = 0.0, r = 0.0;
+      if (valid) {
+        const int yv = a.y[n];
+        stanx::math::sx_logit_row(e, yv ? 1.0 : -1.0, &term, &r);
+        if (!isfinite(e)) bad += 1.0;
+      }
+      s0 += term;
+      s1 += r;
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) acc[k] = fma(xv[k], r, acc[k]);
+    }
+    double* out = a.cpart + (ch * a.chains + c) * a.E;
+    s1 = wave_sum(s1);
+    if (lane == 0) out[0] = s1;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (k < K) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) out[1 + k] = v;
+      }
+    }
+    s0 = wave_sum(s0);
+    bad = wave_sum(bad);
+    if (lane == 0) {
+      out[K + 1] = s0;
+      out[K + 2] = bad;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// FP32, many chains: a wave per chunk, chains on lanes; the chunk's rows
+// are staged 64 at a time in the wave's LDS and read as broadcasts.
+template <int KMAX>
+__global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
+  __shared__ float sx[4][64 * KMAX];
+  __shared__ float sy[4][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int K = a.K;
+  const int c = blockIdx.y * 64 + lane;
+  const bool live = c < a.chains;
+  const double* th = a.theta + static_cast<int64_t>(live ? c : 0) * a.theta_stride;
+  const int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (ch >= a.n_chunks) return;  // whole wave; no block barrier follows
+  const int g = a.chunk_group[ch];
+  const int64_t r0 = a.chunk_row0[ch], r1 = a.chunk_row0[ch + 1];
+  const int rows = static_cast<int>(r1 - r0);
+  const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+  float wk[KMAX];
+  float b = 0.f;
+  {
+    const double tau0 = exp(th[1 + K]);
+    b = static_cast<float>(th[0] + tau0 * z[0]);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+      wk[k] = k < K ? static_cast<float>(th[1 + k] + exp(th[2 + K + k]) * z[1 + k]) : 0.f;
+  }
+  float acc[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) acc[k] = 0.f;
+  float s1 = 0.f;
+  double s0 = 0.0, bad = 0.0;
+  for (int p0 = 0; p0 < rows; p0 += 64) {
+    const int cnt = rows - p0 < 64 ? rows - p0 : 64;
+    const float* src = a.X32 + (r0 + p0) * K;
+    // LDS accesses of one wave complete in order: the reads of the
+    // previous piece are done before these writes land
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < cnt * K; i += 64) sx[w][i] = src[i];
+    if (lane < cnt) sy[w][lane] = a.y[r0 + p0 + lane] ? 1.0f : -1.0f;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    for (int i = 0; i < cnt; ++i) {
+      const float* x = &sx[w][i * K];
+      float e = b;
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k)
+        if (k < K) e = fmaf(x[k], wk[k], e);
+      const float s = sy[w][i];
+      const float se = s * e;
+      const float t = __expf(-fabsf(se));
+      const float inv = 1.0f / (1.0f + t);
+      const float term = -(fmaxf(-se, 0.0f) + log1pf(t));
+      const float r = s * (se >= 0.0f ? t * inv : inv);
+      if (!isfinite(e)) bad += 1.0;
+      s0 += term;
+      s1 += r;
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k)
+        if (k < K) acc[k] = fmaf(x[k], r, acc[k]);
+    }
+  }
+  if (!live) return;
+  double* out = a.cpart + (ch * a.chains + c) * a.E;
+  out[0] = s1;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k)
+    if (k < K) out[1 + k] = acc[k];
+  out[K + 1] = s0;
+  out[K + 2] = bad;
+}
+
+// Per group and chain: sum the group's chunks in order; z gradients;
+// per-group contributions [S_0..S_K, z.S terms (K+1), lp, bad, sum z^2].
+__global__ __launch_bounds__(64) void hier_groups(hier_args a) {
+  const int g = blockIdx.x;
+  const int c = blockIdx.y;
+  const int K = a.K;
+  const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
+  double* out = a.out + static_cast<int64_t>(c) * a.out_stride;
+  const int64_t ch0 = a.group_chunk0[g], ch1 = a.group_chunk0[g + 1];
+  const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+  double* gs = a.gpart + (static_cast<int64_t>(c) * a.J + g) * a.G;
+  for (int e = threadIdx.x; e < K + 3; e += blockDim.x) {
+    double v = 0.0;
+    for (int64_t ch = ch0; ch < ch1; ++ch) v += a.cpart[(ch * a.chains + c) * a.E + e];
+    if (e <= K) {
+      const double tau = exp(th[1 + K + e]);
+      out[1 + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1) + e] = tau * v - z[e];
+      gs[e] = v;
+      gs[K + 1 + e] = tau * z[e] * v;
+    } else {
+      gs[K + 1 + e] = v;  // lp (e = K+1) at 2K+2, bad (e = K+2) at 2K+3
+    }
+  }
+  if (threadIdx.x == 0) {
+    double zz = 0.0;
+    for (int k = 0; k <= K; ++k) zz += z[k] * z[k];
+    gs[2 * K + 4] = zz;
+  }
+}
+
+// Per chain: sums over groups (four fixed interleaved slices, combined in
+// order), priors, Jacobian, output.
+__global__ __launch_bounds__(512) void hier_final(hier_args a) {
+  __shared__ double red[4][128];
+  __shared__ double st[128];
+  const int c = blockIdx.x;
+  const int K = a.K;
+  const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
+  double* out = a.out + static_cast<int64_t>(c) * a.out_stride;
+  const int e = threadIdx.x & 127;
+  const int sl = threadIdx.x >> 7;
+  if (e < a.G) {
+    const double* gp = a.gpart + static_cast<int64_t>(c) * a.J * a.G + e;
+    double v0 = 0.0, v1 = 0.0;
+    int g = sl;
+    for (; g + 4 < a.J; g += 8) {
+      v0 += gp[static_cast<int64_t>(g) * a.G];
+      v1 += gp[static_cast<int64_t>(g + 4) * a.G];
+    }
+    if (g < a.J) v0 += gp[static_cast<int64_t>(g) * a.G];
+    red[sl][e] = v0 + v1;
+  }
+  __syncthreads();
+  if (threadIdx.x < a.G) st[threadIdx.x] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double lp = st[2 * K + 2];
+  double bad = st[2 * K + 3];
+  // alpha
+  double ga = st[0];
+  if (a.alpha_sd > 0) {
+    const double zz = th[0] / a.alpha_sd;
+    lp += -0.5 * zz * zz + (a.propto ? 0.0 : a.alpha_const);
+    ga += -th[0] / (a.alpha_sd * a.alpha_sd);
+  }
+  out[1] = ga;
+  for (int k = 0; k < K; ++k) {
+    double gb = st[1 + k];
+    const double bk = th[1 + k];
+    if (a.beta_sd > 0) {
+      const double zz = bk / a.beta_sd;
+      lp += -0.5 * zz * zz + (a.propto ? 0.0 : a.beta_const);
+      gb += -bk / (a.beta_sd * a.beta_sd);
+    }
+    out[2 + k] = gb;
+  }
+  for (int k = 0; k <= K; ++k) {
+    const double v = th[1 + K + k];
+    const double tau = exp(v);
+    double gv = st[K + 1 + k];
+    if (a.tau_sd > 0) {
+      const double zz = tau / a.tau_sd;
+      lp += -0.5 * zz * zz + (a.propto ? 0.0 : a.tau_const);
+      gv += -zz * zz;
+    }
+    if (a.jacobian) {
+      lp += v;
+      gv += 1.0;
+    }
+    out[2 + K + k] = gv;
+  }
+  lp += -0.5 * st[2 * K + 4];
+  if (!a.propto) lp += -static_cast<double>(a.J) * (K + 1) * kHalfLog2PiH;
+  out[0] = lp;
+  if (!isfinite(lp)) bad += 1.0;
+  out[a.D + 1] = bad > 0 ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s) {
+  if (a.K > 32 || 2 * a.K + 5 > 128 || a.chains < 1 || a.J < 1) return hipErrorInvalidValue;
+  if (a.n_chunks > 0) {
+    if (!fp32) {
+      const int64_t want = (a.n_chunks + 3) / 4;
+      const dim3 grid(static_cast<unsigned>(want < 2048 ? want : 2048), static_cast<unsigned>(a.chains));
+      if (a.K <= 4)
+        hipLaunchKernelGGL((hier_pass_f64<4>), grid, dim3(256), 0, s, a);
+      else if (a.K <= 8)
+        hipLaunchKernelGGL((hier_pass_f64<8>), grid, dim3(256), 0, s, a);
+      else if (a.K <= 16)
+        hipLaunchKernelGGL((hier_pass_f64<16>), grid, dim3(256), 0, s, a);
+      else if (a.K <= 24)
+        hipLaunchKernelGGL((hier_pass_f64<24>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((hier_pass_f64<32>), grid, dim3(256), 0, s, a);
+    } else {
+      const dim3 grid(static_cast<unsigned>((a.n_chunks + 3) / 4), static_cast<unsigned>((a.chains + 63) / 64));
+      if (a.K <= 4)
+        hipLaunchKernelGGL((hier_pass_f32<4>), grid, dim3(256), 0, s, a);
+      else if (a.K <= 8)
+        hipLaunchKernelGGL((hier_pass_f32<8>), grid, dim3(256), 0, s, a);
+      else if (a.K <= 16)
+        hipLaunchKernelGGL((hier_pass_f32<16>), grid, dim3(256), 0, s, a);
+      else if (a.K <= 24)
+        hipLaunchKernelGGL((hier_pass_f32<24>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((hier_pass_f32<32>), grid, dim3(256), 0, s, a);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(hier_groups, dim3(a.J, a.chains), dim3(64), 0, s, a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(hier_final, dim3(a.chains), dim3(512), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace sx

@@ -2,8 +2,10 @@
 //
 // Every chain runs the same iterative state machine as the host sampler
 // (stanx/mcmc/base_nuts.hpp), suspended whenever it needs a gradient: one
-// launch of nuts_step advances each chain (one wave per chain) from the
-// gradient it just received to the point where it needs the next one,
+// launch of nuts_step advances each chain (one block per chain: one wave
+// for D <= 2048, four or eight waves above; vector work is strided over the
+// block's threads, scalar chain state lives in LDS and only thread 0 writes
+// it, between barriers) from the gradient it just received to the point where it needs the next one,
 // writes that point to theta[chain], and the batched gradient launches
 // evaluate all chains together.  Chains therefore move in lockstep one
 // gradient at a time but are otherwise independent: each owns its trees,
@@ -70,7 +72,9 @@ struct ctx {
   const batch_args* a;
   chain_state* s;
   double* v;  // chain vector base
-  int c, lane;
+  int c, lane;  // lane = thread index in the chain's block
+  int nt;       // threads per chain
+  double* red;  // LDS scratch for block reductions (one slot per wave)
   uint32_t chain_id;
   __device__ double* slot(int i) const { return v + static_cast<int64_t>(i) * a->Dpad; }
 };
@@ -80,16 +84,32 @@ __device__ __forceinline__ double wave_sum(double x) {
   for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
   return x;
 }
+// Sum over the chain's block (fixed order: waves, then wave 0..nw-1), the
+// result in every thread.  Called in block-uniform control flow.
+__device__ double block_sum(const struct ctx& k, double x);
 
-// ---- vector helpers (lanes stride over D) ----
+__device__ double block_sum(const ctx& k, double x) {
+  x = wave_sum(x);
+  if (k.nt == 64) return x;
+  const int w = k.lane >> 6;
+  __syncthreads();
+  if ((k.lane & 63) == 0) k.red[w] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (k.nt >> 6); ++i) t += k.red[i];
+  __syncthreads();
+  return t;
+}
+
+// ---- vector helpers (threads stride over D) ----
 __device__ void vcopy(const ctx& k, int dst, int src) {
   double* d = k.slot(dst);
   const double* s = k.slot(src);
-  for (int i = k.lane; i < k.a->D; i += 64) d[i] = s[i];
+  for (int i = k.lane; i < k.a->D; i += k.nt) d[i] = s[i];
 }
 __device__ void vzero(const ctx& k, int dst) {
   double* d = k.slot(dst);
-  for (int i = k.lane; i < k.a->D; i += 64) d[i] = 0.0;
+  for (int i = k.lane; i < k.a->D; i += k.nt) d[i] = 0.0;
 }
 // sum_i p_sharp_i * rho_i with p_sharp = minv * p
 __device__ double dot_sharp(const ctx& k, int p, int rho) {
@@ -97,8 +117,8 @@ __device__ double dot_sharp(const ctx& k, int p, int rho) {
   const double* pp = k.slot(p);
   const double* r = k.slot(rho);
   double acc = 0.0;
-  for (int i = k.lane; i < k.a->D; i += 64) acc += (m[i] * pp[i]) * r[i];
-  return wave_sum(acc);
+  for (int i = k.lane; i < k.a->D; i += k.nt) acc += (m[i] * pp[i]) * r[i];
+  return block_sum(k, acc);
 }
 // sum_i p_sharp_i * (r1_i + r2_i)
 __device__ double dot_sharp2(const ctx& k, int p, int r1, int r2) {
@@ -107,20 +127,20 @@ __device__ double dot_sharp2(const ctx& k, int p, int r1, int r2) {
   const double* a = k.slot(r1);
   const double* b = k.slot(r2);
   double acc = 0.0;
-  for (int i = k.lane; i < k.a->D; i += 64) acc += (m[i] * pp[i]) * (a[i] + b[i]);
-  return wave_sum(acc);
+  for (int i = k.lane; i < k.a->D; i += k.nt) acc += (m[i] * pp[i]) * (a[i] + b[i]);
+  return block_sum(k, acc);
 }
 __device__ double kinetic(const ctx& k, int p) {
   const double* m = k.slot(MINV);
   const double* pp = k.slot(p);
   double acc = 0.0;
-  for (int i = k.lane; i < k.a->D; i += 64) acc += pp[i] * (m[i] * pp[i]);
-  return 0.5 * wave_sum(acc);
+  for (int i = k.lane; i < k.a->D; i += k.nt) acc += pp[i] * (m[i] * pp[i]);
+  return 0.5 * block_sum(k, acc);
 }
 
 // ---- RNG: the chain's Philox stream, one variate per counter value ----
-// Called in wave-uniform control flow: every lane draws the same value,
-// then lane 0 advances the counter.
+// Called in block-uniform control flow: every thread draws the same value,
+// then thread 0 advances the counter.
 __device__ double uniform(const ctx& k) {
   const uint64_t ctr = k.s->rng;
   const double u = stanx::philox_uniform(k.a->seed, k.chain_id, ctr);
@@ -135,7 +155,7 @@ __device__ void sample_p(const ctx& k) {
   double* p = k.slot(ZP);
   const double* m = k.slot(MINV);
   const uint64_t base = k.s->rng;
-  for (int i = k.lane; i < k.a->D; i += 64)
+  for (int i = k.lane; i < k.a->D; i += k.nt)
     p[i] = stanx::philox_normal(k.a->seed, k.chain_id, base + i) / sqrt(m[i]);
   __syncthreads();
   if (k.lane == 0) k.s->rng = base + k.a->D;
@@ -181,7 +201,7 @@ __device__ void copy_point(const ctx& k, int dq, int sq, double& dV, double sV) 
 __device__ void request_gradient(const ctx& k, int next_phase) {
   double* th = k.a->theta + static_cast<int64_t>(k.c) * k.a->D;
   const double* q = k.slot(ZQ);
-  for (int i = k.lane; i < k.a->D; i += 64) th[i] = q[i];
+  for (int i = k.lane; i < k.a->D; i += k.nt) th[i] = q[i];
   if (k.lane == 0) {
     k.s->phase = next_phase;
     k.s->pending = 1;
@@ -194,7 +214,7 @@ __device__ void consume_gradient(const ctx& k) {
   const double* o = k.a->out + static_cast<int64_t>(k.c) * k.a->out_stride;
   const bool ok = o[k.a->D + 1] == 0.0 && isfinite(o[0]);
   double* g = k.slot(ZG);
-  for (int i = k.lane; i < k.a->D; i += 64) g[i] = ok ? -o[1 + i] : -g[i];
+  for (int i = k.lane; i < k.a->D; i += k.nt) g[i] = ok ? -o[1 + i] : -g[i];
   if (k.lane == 0) {
     k.s->V_z = ok ? -o[0] : kInf;
     k.s->last_lp = o[0];
@@ -209,7 +229,7 @@ __device__ void leapfrog_begin(const ctx& k, double eps) {
   const double* g = k.slot(ZG);
   const double* m = k.slot(MINV);
   const double he = 0.5 * eps;
-  for (int i = k.lane; i < k.a->D; i += 64) {
+  for (int i = k.lane; i < k.a->D; i += k.nt) {
     p[i] -= he * g[i];
     q[i] += eps * (m[i] * p[i]);
   }
@@ -218,7 +238,7 @@ __device__ void leapfrog_end(const ctx& k, double eps) {
   double* p = k.slot(ZP);
   const double* g = k.slot(ZG);
   const double he = 0.5 * eps;
-  for (int i = k.lane; i < k.a->D; i += 64) p[i] -= he * g[i];
+  for (int i = k.lane; i < k.a->D; i += k.nt) p[i] -= he * g[i];
 }
 
 __device__ dest init_dest(const dest& d, int l) {
@@ -269,7 +289,7 @@ __device__ bool leaf(const ctx& k, const dest& d) {
     double* r = k.slot(d.rho);
     double* pb = k.slot(d.pbeg);
     double* pe = k.slot(d.pend);
-    for (int i = k.lane; i < k.a->D; i += 64) {
+    for (int i = k.lane; i < k.a->D; i += k.nt) {
       r[i] += p[i];
       pb[i] = p[i];
       pe[i] = p[i];
@@ -301,7 +321,7 @@ __device__ bool merge(const ctx& k, int l, const dest& d) {
     const double* rf = k.slot(RF(l));
     double* t = k.slot(TMP);
     double* r = k.slot(d.rho);
-    for (int i = k.lane; i < k.a->D; i += 64) {
+    for (int i = k.lane; i < k.a->D; i += k.nt) {
       t[i] = ri[i] + rf[i];
       r[i] += t[i];
     }
@@ -340,7 +360,7 @@ __device__ int learn_variance(const ctx& k) {
     double* m = k.slot(WM);
     double* m2 = k.slot(WM2);
     const double* q = k.slot(ZQ);
-    for (int i = k.lane; i < a->D; i += 64) {
+    for (int i = k.lane; i < a->D; i += k.nt) {
       const double delta = q[i] - m[i];
       m[i] += delta / n;
       m2[i] += (q[i] - m[i]) * delta;
@@ -370,7 +390,7 @@ __device__ int learn_variance(const ctx& k) {
   double* m = k.slot(WM);
   double* m2 = k.slot(WM2);
   int finite = 1;
-  for (int i = k.lane; i < a->D; i += 64) {
+  for (int i = k.lane; i < a->D; i += k.nt) {
     double var = minv[i];
     if (n > 1) var = m2[i] / (n - 1.0);
     var = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0)) * 1.0;
@@ -379,7 +399,7 @@ __device__ int learn_variance(const ctx& k) {
     m[i] = 0.0;
     m2[i] = 0.0;
   }
-  const int all_finite = __all(finite);
+  const int all_finite = __syncthreads_and(finite);
   __syncthreads();
   if (k.lane == 0) {
     s->w_next = next;
@@ -405,7 +425,7 @@ __device__ void write_draw(const ctx& k) {
     row[6] = s->T_samp + s->V_samp;
   }
   const double* q = k.slot(SQ);
-  for (int i = k.lane; i < a->D; i += 64) row[7 + i] = q[i];
+  for (int i = k.lane; i < a->D; i += k.nt) row[7 + i] = q[i];
   __syncthreads();
   if (k.lane == 0) s->rows++;
 }
@@ -438,17 +458,19 @@ __device__ void begin_stepsize_heuristic(const ctx& k, int ret_phase) {
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void nuts_step(batch_args a) {
+__global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   __shared__ chain_state s;
+  __shared__ double red[8];
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
+  const int nt = blockDim.x;
   {
     const int* src = reinterpret_cast<const int*>(a.states + c);
     int* dst = reinterpret_cast<int*>(&s);
-    for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += 64) dst[i] = src[i];
+    for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += nt) dst[i] = src[i];
   }
   __syncthreads();
-  ctx k{&a, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane,
+  ctx k{&a, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane, nt, red,
         a.chain0 + static_cast<uint32_t>(c)};
   if (s.phase == PH_DONE) return;
   if (s.pending) {
@@ -467,10 +489,10 @@ __global__ __launch_bounds__(64) void nuts_step(batch_args a) {
         double* q = k.slot(ZQ);
         if (a.inits) {
           const double* src = a.inits + static_cast<int64_t>(c) * a.D;
-          for (int i = lane; i < a.D; i += 64) q[i] = src[i];
+          for (int i = lane; i < a.D; i += k.nt) q[i] = src[i];
         } else {
           const uint64_t base = s.rng;
-          for (int i = lane; i < a.D; i += 64)
+          for (int i = lane; i < a.D; i += k.nt)
             q[i] = a.init_radius == 0.0
                        ? 0.0
                        : -a.init_radius + stanx::philox_uniform(a.seed, k.chain_id, base + i) *
@@ -485,8 +507,8 @@ __global__ __launch_bounds__(64) void nuts_step(batch_args a) {
       case PH_INIT_CHECK: {
         double gs = 0.0;
         const double* g = k.slot(ZG);
-        for (int i = lane; i < a.D; i += 64) gs += g[i];
-        gs = wave_sum(gs);
+        for (int i = lane; i < a.D; i += k.nt) gs += g[i];
+        gs = block_sum(k, gs);
         const bool ok = s.last_ok && isfinite(s.last_lp) && isfinite(gs);
         __syncthreads();
         if (ok) {
@@ -570,7 +592,10 @@ __global__ __launch_bounds__(64) void nuts_step(batch_args a) {
           finish(k, 0);
           break;
         }
-        if (s.iter == a.num_warmup && s.adapt_flag) {
+        // every wave reads the condition before lane 0 changes adapt_flag
+        const bool end_warmup = s.iter == a.num_warmup && s.adapt_flag;
+        __syncthreads();
+        if (end_warmup) {
           // disengage_adaptation: complete_adaptation sets eps = exp(x_bar)
           if (lane == 0) {
             s.adapt_flag = 0;
@@ -701,7 +726,7 @@ __global__ __launch_bounds__(64) void nuts_step(batch_args a) {
           const double* rb = k.slot(RHOB);
           const double* rf = k.slot(RHOF);
           double* r = k.slot(RHO);
-          for (int i = lane; i < a.D; i += 64) r[i] = rb[i] + rf[i];
+          for (int i = lane; i < a.D; i += k.nt) r[i] = rb[i] + rf[i];
         }
         __syncthreads();
         bool persist = dot_sharp(k, FP, RHO) > 0 && dot_sharp(k, BP, RHO) > 0;
@@ -757,12 +782,14 @@ __global__ __launch_bounds__(64) void nuts_step(batch_args a) {
   {
     const int* src = reinterpret_cast<const int*>(&s);
     int* dst = reinterpret_cast<int*>(a.states + c);
-    for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += 64) dst[i] = src[i];
+    for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += nt) dst[i] = src[i];
   }
 }
 
 hipError_t launch_nuts_step(const batch_args& a, hipStream_t s) {
-  hipLaunchKernelGGL(nuts_step, dim3(a.C), dim3(64), 0, s, a);
+  // one wave per chain for small D, up to 8 waves for large D
+  const int nt = a.D <= 2048 ? 64 : (a.D <= 8192 ? 256 : 512);
+  hipLaunchKernelGGL(nuts_step, dim3(a.C), dim3(nt), 0, s, a);
   return hipGetLastError();
 }
 

@@ -86,6 +86,26 @@ hipError_t launch_split_x(const double* Xrm, uint16_t* X1h, uint16_t* X1l, uint1
 hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C,
                             const mx_args& a, const pass_args& pa, hipStream_t s);
 
+// Hierarchical logistic model: group-aligned row chunks of at most
+// kHierChunk rows; partials per chunk and per group.
+constexpr int kHierChunk = 256;
+struct hier_args {
+  const double* X;          // N x K row-major (FP64 pass)
+  const float* X32;         // N x K row-major (FP32 batched pass)
+  const int8_t* y;
+  const double* theta;      // chains x theta_stride
+  double* out;              // chains x out_stride: lp, grad[D], status
+  double* cpart;            // n_chunks x chains x E
+  double* gpart;            // chains x J x G
+  const int* chunk_group;   // n_chunks
+  const int64_t* chunk_row0;    // n_chunks + 1
+  const int64_t* group_chunk0;  // J + 1
+  int64_t n_chunks, theta_stride, out_stride;
+  int J, K, D, E, G, chains, propto, jacobian;
+  double alpha_sd, beta_sd, tau_sd, alpha_const, beta_const, tau_const;
+};
+hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s);
+
 // Device workspace of batched MFMA evaluations for up to cap chains.
 struct mx_workspace {
   int cap = 0, Cp = 0, splits = 0;
@@ -95,12 +115,28 @@ struct mx_workspace {
   double* spart = nullptr;
 };
 
+// Partial sums of hierarchical evaluations for up to cap chains.
+struct hier_workspace {
+  int64_t cap = 0;
+  double* cpart = nullptr;  // n_chunks x chains x (K + 3)
+  double* gpart = nullptr;  // chains x J x (2K + 5)
+};
+
+// Each user of batched evaluations (the context's own batch calls, every
+// batched sampler) owns its workspace: a captured graph keeps the
+// addresses it was captured with, and samplers may run concurrently.
+void free_workspace(mx_workspace& w);
+void free_workspace(hier_workspace& w);
+
 }  // namespace sx
 
 struct sx_glm;
 namespace sx {
-int mx_enqueue(sx_glm* m, const double* d_theta, int C, double* d_out, int propto, int jacobian,
-               hipStream_t stream);
+int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double* d_out,
+               int propto, int jacobian, hipStream_t stream);
+int hier_prepare(sx_glm* m, hier_workspace& w, int C);
+int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, double* d_out,
+                 int propto, int jacobian, bool fp32, hipStream_t stream);
 }
 
 struct sx_glm {
@@ -124,6 +160,12 @@ struct sx_glm {
   double* d_btheta = nullptr;  // batched theta / out staging for the C-ABI batch call
   double* d_bout = nullptr;
   int64_t bcap = 0;
+  // hierarchical model
+  float* d_X32 = nullptr;
+  int* d_chunk_group = nullptr;
+  int64_t *d_chunk_row0 = nullptr, *d_group_chunk0 = nullptr;
+  int64_t n_chunks = 0;
+  sx::hier_workspace hw;
   std::string tuning_log;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

@@ -1,0 +1,226 @@
+"""Hierarchical logistic model (ragged groups, non-centred varying
+intercepts and slopes): segmented GPU passes against the oracle.
+
+Bars:
+  FP64 (single chain and batched): log density and gradient within 1e-12
+  of the compensated oracle, relative (normwise for the gradient); a chain's
+  batched result is bitwise its single evaluation.
+  FP32 batched (X in FP32, FP32 products and chunk sums of at most 256 rows,
+  FP64 sums over chunks and groups), random test points theta ~ N(0, 0.5^2):
+  relative error of the log density <= LP_TOL_FP32 and normwise relative
+  error of the gradient <= GRAD_TOL_FP32; the measured values are printed.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import stan_amd as S
+from stan_amd import analyze
+from stan_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+LP_TOL_FP32 = 1e-5
+GRAD_TOL_FP32 = 1e-4
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def _data(J, K, mean_size, seed, empty=()):
+    d = W.hier_logistic(J=J, K=K, seed=seed, mean_log_size=np.log(mean_size), sd_log=1.0)
+    if empty:
+        # drop the rows of some groups: empty groups keep their z
+        sizes = np.diff(d["offsets"])
+        keep = np.ones(d["X"].shape[0], bool)
+        for j in empty:
+            keep[d["offsets"][j]:d["offsets"][j + 1]] = False
+            sizes[j] = 0
+        d["X"], d["y"] = d["X"][keep], d["y"][keep]
+        d["offsets"] = np.concatenate([[0], np.cumsum(sizes)])
+    return d
+
+
+def _models(d, precision=S.FP64):
+    m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                   precision=precision)
+    ref = O.GlmData(O.HIER, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"])
+    return m, ref
+
+
+@pytest.mark.parametrize("J,K,size,empty", [(1, 1, 10, ()), (3, 0, 40, ()), (5, 3, 60, (1,)),
+                                            (40, 4, 300, (0, 39)), (17, 8, 900, ()),
+                                            (50, 20, 400, (7,)), (9, 24, 250, ()),
+                                            (12, 32, 500, (3, 4))])
+def test_hier_fp64_parity(J, K, size, empty):
+    d = _data(J, K, size, seed=J * 100 + K, empty=empty)
+    _say(f"hier fp64 J={J} K={K} N={d['X'].shape[0]} empty={empty}")
+    m, ref = _models(d)
+    assert m.D == 2 * K + 2 + J * (K + 1)
+    for th in W.random_thetas(m.D, 3, seed=K + J) * 0.5:
+        for propto, jac in [(True, True), (False, False), (False, True)]:
+            lp, g = m.log_prob_grad(th, propto=propto, jacobian=jac)
+            st, lp_r, g_r = ref.lpgrad(th, mode=1, propto=propto, jacobian=jac)
+            assert st == 0
+            assert abs(lp - lp_r) <= TOL * max(1.0, abs(lp_r)), (lp, lp_r)
+            err = np.linalg.norm(g - g_r)
+            assert err <= TOL * max(1.0, np.linalg.norm(g_r)), (err, np.linalg.norm(g_r))
+    m.close()
+
+
+def test_hier_fp64_batch_bitwise_single():
+    d = _data(30, 6, 200, seed=11)
+    m, ref = _models(d)
+    th = W.random_thetas(m.D, 9, seed=3) * 0.5
+    lp, g = m.log_prob_grad_batch(th)
+    for c in range(9):
+        lp1, g1 = m.log_prob_grad(th[c])
+        assert lp[c] == lp1
+        np.testing.assert_array_equal(g[c], g1)
+    m.close()
+
+
+@pytest.mark.parametrize("J,K,size,C", [(5, 3, 60, 1), (40, 4, 300, 7), (50, 20, 400, 64),
+                                        (12, 32, 500, 130), (3, 0, 40, 65)])
+def test_hier_fp32_batch_against_oracle(J, K, size, C):
+    d = _data(J, K, size, seed=J + K + C)
+    m, ref = _models(d, precision=S.FP32)
+    th = W.random_thetas(m.D, C, seed=C + K) * 0.5
+    lp, g = m.log_prob_grad_batch(th)
+    worst_lp = worst_g = 0.0
+    for c in range(C):
+        st, lp_r, g_r = ref.lpgrad(th[c], mode=1)
+        assert st == 0
+        worst_lp = max(worst_lp, abs(lp[c] - lp_r) / abs(lp_r))
+        worst_g = max(worst_g, np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r))
+    _say(f"hier fp32 J={J} K={K} C={C}: max rel lp err {worst_lp:.2e}, "
+         f"max normwise grad err {worst_g:.2e}")
+    assert worst_lp < LP_TOL_FP32 and worst_g < GRAD_TOL_FP32
+    # single evaluations of the same context stay FP64
+    lp64, g64 = m.log_prob_grad(th[0])
+    st, lp_r, g_r = ref.lpgrad(th[0], mode=1)
+    assert abs(lp64 - lp_r) <= TOL * abs(lp_r)
+    m.close()
+
+
+def test_hier_fp32_chain_independent_of_batch():
+    d = _data(25, 5, 150, seed=5)
+    m, _ = _models(d, precision=S.FP32)
+    th = W.random_thetas(m.D, 100, seed=6) * 0.5
+    lp_all, g_all = m.log_prob_grad_batch(th)
+    perm = np.random.Generator(np.random.PCG64(1)).permutation(100)[:37]
+    lp_p, g_p = m.log_prob_grad_batch(th[perm])
+    np.testing.assert_array_equal(lp_p, lp_all[perm])
+    np.testing.assert_array_equal(g_p, g_all[perm])
+    m.close()
+
+
+@pytest.mark.parametrize("precision", [S.FP64, S.FP32])
+def test_hier_domain_error_per_chain(precision):
+    d = _data(6, 3, 50, seed=9)
+    m, _ = _models(d, precision=precision)
+    th = W.random_thetas(m.D, 4, seed=2) * 0.5
+    th[2, 1 + 3] = 800.0  # tau_0 = exp(800) overflows: eta is not finite
+    lp, g = m.log_prob_grad_batch(th)
+    assert np.isneginf(lp[2]) and np.all(np.isfinite(lp[[0, 1, 3]]))
+    with pytest.raises(S.DomainError):
+        m.log_prob_grad(th[2])
+    m.close()
+
+
+def test_hier_create_validation():
+    d = _data(4, 2, 20, seed=1)
+    X, y, off = d["X"], d["y"], d["offsets"]
+    bad = off.copy()
+    bad[-1] += 1
+    with pytest.raises(S.StanxError):
+        S.GlmModel(S.HIER_LOGIT, X, y, offsets=bad)
+    bad = off.copy()
+    bad[1], bad[2] = bad[2], bad[1]
+    with pytest.raises(S.StanxError):
+        S.GlmModel(S.HIER_LOGIT, X, y, offsets=bad)
+    with pytest.raises(S.StanxError):
+        S.GlmModel(S.HIER_LOGIT, X, y)
+    with pytest.raises(S.StanxError):
+        S.GlmModel(S.HIER_LOGIT, X, y, offsets=off, precision=S.BF16X2)
+    wide = np.zeros((X.shape[0], 33))
+    with pytest.raises(S.StanxError):
+        S.GlmModel(S.HIER_LOGIT, wide, y, offsets=off)
+
+
+def _oracle_posterior(d, D, chains, warmup, samples, seed):
+    ref = O.GlmData(O.HIER, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"])
+    f = O.model_fn("orc_glm_model_cb")
+    ctx = ref.ctx(mode=0)
+    out = []
+    for chain in range(1, chains + 1):
+        rc, dr, _, _, _ = O.hmc_nuts_diag_e_adapt(f, ctx, D, seed=seed, chain=chain,
+                                                  num_warmup=warmup, num_samples=samples)
+        assert rc == 0
+        out.append(dr[:, 7:])
+    return np.stack(out, 1)  # iterations x chains x params (unconstrained)
+
+
+@pytest.mark.parametrize("precision", [S.FP64, S.FP32])
+def test_hier_batch_posterior_matches_oracle_sampler(precision):
+    d = _data(6, 2, 80, seed=21)
+    m, _ = _models(d, precision=precision)
+    K = 2
+    res = S.hmc_nuts_diag_e_adapt_batch(m, 8, grad_precision=precision, random_seed=17,
+                                        chain=1, num_warmup=500, num_samples=1000)
+    assert np.all(res["status"] == 0), res["status"]
+    dev = np.transpose(res["draws"][:, :, 7:], (1, 0, 2)).copy()
+    # draws carry tau = exp(v); compare on the unconstrained scale
+    dev[:, :, 1 + K:2 + 2 * K] = np.log(dev[:, :, 1 + K:2 + 2 * K])
+    cpu = _oracle_posterior(d, m.D, 4, 500, 1000, seed=4711)
+    bad = []
+    for p in range(m.D):
+        se = np.hypot(analyze.mcse_mean(dev[:, :, p]), analyze.mcse_mean(cpu[:, :, p]))
+        if abs(dev[:, :, p].mean() - cpu[:, :, p].mean()) >= 4 * se:
+            bad.append(p)
+    _say(f"hier batch precision={precision}: min bulk ESS {analyze.min_bulk_ess(dev):.0f}, "
+         f"steps {res['steps']}, params outside 4 MCSE: {bad}")
+    assert not bad
+    m.close()
+
+
+def test_hier_single_chain_service():
+    d = _data(5, 2, 60, seed=33)
+    m, _ = _models(d)
+    res = S.hmc_nuts_diag_e_adapt(m, random_seed=5, chain=1, num_warmup=200, num_samples=200)
+    assert res["rc"] == 0
+    dr = res["draws"]
+    assert dr.shape == (200, 7 + m.D)
+    # write_array: tau columns are exp(v) > 0
+    assert np.all(dr[:, 7 + 1 + 2:7 + 2 + 2 * 2] > 0)
+    m.close()
+
+
+@pytest.mark.parametrize("J,K", [(250, 8), (1000, 8)])
+def test_hier_batch_large_d_follows_host_service(J, K):
+    """D = 2268 and 9018: the device sampler runs 4 and 8 waves per chain
+    (block reductions), the host service one thread; same stream, same
+    phases, so the draws agree until last-bit differences of the dot
+    products grow (see the batched sampler tests)."""
+    d = _data(J, K, 40, seed=J + K)
+    m, _ = _models(d)
+    _say(f"hier large-D batch vs host J={J} K={K} D={m.D} N={d['X'].shape[0]}")
+    cfg = dict(random_seed=99, chain=1, num_warmup=40, num_samples=20, save_warmup=1,
+               max_depth=5)
+    res = S.hmc_nuts_diag_e_adapt_batch(m, 2, **cfg)
+    assert np.all(res["status"] == 0), res["status"]
+    for c in range(2):
+        host = S.hmc_nuts_diag_e_adapt(m, **{**cfg, "chain": 1 + c})
+        assert host["rc"] == 0
+        dev, hd = res["draws"][c], host["draws"]
+        assert dev.shape == hd.shape
+        np.testing.assert_allclose(dev[:10], hd[:10], rtol=1e-5, atol=1e-8)
+        discrete = np.all(dev[:, 3:6] == hd[:, 3:6], axis=1)
+        _say(f"chain {c}: discrete columns equal in {discrete.mean():.3f} of rows")
+        assert discrete[:20].all()
+    m.close()
