@@ -87,10 +87,11 @@ def batched_run(model, chains, warmup, samples, grad_precision, seed=1234):
 
 
 def mixed_error(model, ref_data, family, thetas):
-    """Max relative lp error and normwise gradient error of the batched BF16
-    hi+lo pass against the FP64 oracle at the given points."""
+    """Max relative lp error and normwise gradient error of the batched
+    mixed-precision pass against the FP64 oracle at the given points."""
     import oracle as O
-    ref = O.GlmData(family, ref_data["X"], ref_data["y"], prior=ref_data["prior"])
+    ref = O.GlmData(family, ref_data["X"], ref_data["y"], offsets=ref_data.get("offsets"),
+                    prior=ref_data["prior"])
     lp, g = model.log_prob_grad_batch(thetas)
     e_lp = e_g = 0.0
     for c in range(len(thetas)):
@@ -98,6 +99,83 @@ def mixed_error(model, ref_data, family, thetas):
         e_lp = max(e_lp, abs(lp[c] - lp_r) / abs(lp_r))
         e_g = max(e_g, float(np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r)))
     return e_lp, e_g
+
+
+def hier_section(warm, samp, chains=64):
+    """Hierarchical logistic model, J=1000 ragged groups, K=20: batched FP32
+    evaluation and device NUTS over 64 chains, against the single-chain FP64
+    service on the same context."""
+    import stan_amd as S
+    from stan_amd import workloads as W
+    dh = W.hier_logistic()
+    N, K = dh["X"].shape
+    J = len(dh["offsets"]) - 1
+    log(f"hierarchical logistic J={J} N={N} K={K}, {chains} chains FP32")
+    mh = S.GlmModel(S.HIER_LOGIT, dh["X"], dh["y"], offsets=dh["offsets"], prior=dh["prior"],
+                    precision=S.FP32)
+    th = W.random_thetas(mh.D, chains, seed=9) * 0.5
+    ms_b, _ = mh.time_log_prob_grad(th, iters=20, warmup=3, chains=chains)
+    ms_1, _ = mh.time_log_prob_grad(th[0], iters=20, warmup=3)
+    out = batched_run(mh, chains, warm, samp, S.FP32)
+    # X (FP32) + y (int8) + chunk tables; theta/out of every chain
+    alg_bytes = 4.0 * N * K + N + 16.0 * (N / 256 + J) + 8.0 * chains * (2 * mh.D + 2)
+    out.update(N=N, K=K, J=J, D=mh.D, ms_per_batched_eval=ms_b,
+               batched_eval_gbs=alg_bytes / (ms_b * 1e-3) / 1e9,
+               batched_eval_tflops_fp32=(4.0 * N * K + 4.0 * N) * chains / (ms_b * 1e-3) / 1e12,
+               ms_single_fp64_eval=ms_1)
+    e_lp, e_g = mixed_error(mh, dh, S.HIER_LOGIT, th[:3])
+    out["max_rel_err_lp"] = e_lp
+    out["max_rel_err_grad"] = e_g
+    log("hierarchical: single-chain FP64 service for comparison")
+    out["plain_fp64_single_chain"] = nuts_run(mh, warm, samp)
+    out["min_ess_per_s_vs_plain"] = (out["min_ess_per_s"] /
+                                     out["plain_fp64_single_chain"]["min_ess_per_s"])
+    mh.close()
+    return out
+
+
+def c1_section(with_cpu=True):
+    """The reference's own CPU-runnable case: logistic N=1e4, K=50, X~N(0,1),
+    4 chains x (1000 warmup + 1000 draws), max_depth 10, seed 1234.  The
+    device sampler runs the 4 chains together with FP64 gradients; the
+    oracle sampler runs them on 4 host threads."""
+    import stan_amd as S
+    from stan_amd import analyze
+    from stan_amd import workloads as W
+    d = W.logistic_small()
+    m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"])
+    out = dict(gpu_batched_fp64=batched_run(m, 4, 1000, 1000, S.FP64))
+    m.close()
+    if with_cpu:
+        import threading
+        import oracle as O
+        ref = O.GlmData(O.LOGIT, d["X"], d["y"], prior=d["prior"])
+        f = O.model_fn("orc_glm_model_cb")
+        ctx = ref.ctx(mode=0)
+        res = [None] * 4
+
+        def run(c):
+            res[c] = O.hmc_nuts_diag_e_adapt(f, ctx, 51, seed=1234, chain=c + 1,
+                                             num_warmup=1000, num_samples=1000)
+        log("C1 on the host: oracle sampler, 4 chains on 4 threads")
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=run, args=(c,)) for c in range(4)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        el = time.perf_counter() - t0
+        if any(r[0] != 0 for r in res):
+            raise RuntimeError("oracle sampler failed")
+        q = np.stack([r[1][:, 7:] for r in res], 1)
+        ess = analyze.min_bulk_ess(q)
+        ng = sum(r[4] for r in res)
+        out["cpu_oracle_4_threads"] = dict(seconds=el, min_ess=ess, min_ess_per_s=ess / el,
+                                           grad_evals=ng, grad_evals_per_s=ng / el, cores=4,
+                                           kind="port")
+        out["gpu_vs_cpu_min_ess_per_s"] = (out["gpu_batched_fp64"]["min_ess_per_s"] /
+                                           out["cpu_oracle_4_threads"]["min_ess_per_s"])
+    return out
 
 
 def main():
@@ -109,6 +187,8 @@ def main():
     ap.add_argument("--no-nuts", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--chains", type=int, default=256)
+    ap.add_argument("--no-hier", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -212,6 +292,11 @@ def main():
         bn["max_rel_err_grad"] = e_g
         result["normal_glm_batched_256"] = bn
         mn.close()
+    if not args.no_hier:
+        result["hier_logistic_64"] = hier_section(300, 300)
+    if not args.no_c1:
+        log("reference CPU case N=1e4 K=50, 4 chains x (1000 + 1000)")
+        result["c1_logistic_4_chains"] = c1_section(with_cpu=not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 core)")
         result["cpu_baseline"] = cpu_baseline(d, theta)
