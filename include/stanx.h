@@ -26,6 +26,10 @@
  *                          as a table instead of through writer callbacks.
  *   sx_hmc_nuts_diag_e_adapt_cb
  *                          the same service for a model given as a C callback.
+ *   sx_hmc_nuts_diag_e, sx_hmc_nuts_diag_e_cb
+ *                          stan::services::sample::hmc_nuts_diag_e
+ *                          (src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91):
+ *                          the same transition without adaptation.
  *
  * Conventions
  *   - Host buffers are caller-owned.  X (row-major N x K doubles) and y are
@@ -191,7 +195,8 @@ typedef struct sx_nuts_config {
 
 /* Outputs of one run.  draws: rows x (7 + D) doubles with columns
  *   lp__, accept_stat__, stepsize__, treedepth__, n_leapfrog__,
- *   divergent__, energy__, then the D unconstrained parameters
+ *   divergent__, energy__, then the D constrained parameters of
+ *   model.write_array (sigma = exp(u), tau = exp(v); the others as is)
  * (src/stan/mcmc/base_nuts.hpp:206-220, src/stan/mcmc/sample.hpp:42-45).
  * Capacity must be (save_warmup ? num_warmup : 0) + num_samples rows
  * (before thinning).                                                   */
@@ -222,6 +227,20 @@ int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D,
                                 const double* init,
                                 const double* init_inv_metric,
                                 sx_nuts_result* out);
+
+/* hmc_nuts_diag_e (src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91)
+ * through run_sampler (src/stan/services/util/run_sampler.hpp:39-85): the
+ * step size cfg->stepsize (with jitter) and the metric init_inv_metric (or
+ * unit) stay fixed; warm-up iterations are transitions without
+ * adaptation and no step-size heuristic runs.  delta, gamma, kappa, t0
+ * and the window parameters of cfg are ignored; out->stepsize and
+ * out->inv_metric return the fixed values.                              */
+int sx_hmc_nuts_diag_e(sx_glm* m, const sx_nuts_config* cfg,
+                       const double* init, const double* init_inv_metric,
+                       sx_nuts_result* out);
+int sx_hmc_nuts_diag_e_cb(sx_lpgrad_fn f, void* ctx, int D,
+                          const sx_nuts_config* cfg, const double* init,
+                          const double* init_inv_metric, sx_nuts_result* out);
 
 /* ---------------- batched device-resident sampler ---------------- */
 /* `chains` chains of adaptive diag_e NUTS on the device, chain ids

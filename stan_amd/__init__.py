@@ -23,7 +23,8 @@ EXPORTED = [
     "sx_glm_dims",
     "sx_last_error", "sx_glm_logp_grad", "sx_glm_logp", "sx_glm_logp_grad_batch",
     "sx_glm_time_logp_grad", "sx_glm_kernel_config", "sx_hmc_nuts_diag_e_adapt",
-    "sx_hmc_nuts_diag_e_adapt_cb", "sx_nuts_batch_create", "sx_nuts_batch_run",
+    "sx_hmc_nuts_diag_e_adapt_cb", "sx_hmc_nuts_diag_e", "sx_hmc_nuts_diag_e_cb",
+    "sx_nuts_batch_create", "sx_nuts_batch_run",
     "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
 ]
 
@@ -100,6 +101,10 @@ def lib():
                                                C.POINTER(NutsResult)]
         L.sx_hmc_nuts_diag_e_adapt_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                                   vp, vp, C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e.argtypes = [vp, C.POINTER(NutsConfig), vp, vp,
+                                         C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
+                                            vp, vp, C.POINTER(NutsResult)]
         L.sx_nuts_batch_create.argtypes = [C.POINTER(vp), vp, C.POINTER(NutsConfig), i64,
                                            C.c_int, vp, vp]
         L.sx_nuts_batch_run.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64)]
@@ -273,6 +278,25 @@ def hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
     return _run_service(
         lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, C.byref(cfg),
                                                           _ptr(i), _ptr(m), C.byref(r)),
+        D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e(model, init=None, inv_metric=None, **kw):
+    """diag_e NUTS without adaptation on a GlmModel (fixed step size and
+    metric); same return dict as hmc_nuts_diag_e_adapt."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e(model.handle, C.byref(cfg), _ptr(i), _ptr(m),
+                                                 C.byref(r)),
+        model.D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
+    """hmc_nuts_diag_e for a model given as a C callback."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_cb(fn_ptr, ctx, D, C.byref(cfg), _ptr(i),
+                                                    _ptr(m), C.byref(r)),
         D, cfg, init, inv_metric)
 
 

@@ -1,6 +1,6 @@
-// C-ABI of the sampler service: runs the C++ adaptive diag_e NUTS
-// (stanx/services.hpp) on a GPU-backed GLM model or on a model given as
-// a C callback, and returns the draws as a table.
+// C-ABI of the sampler services: runs the C++ diag_e NUTS, adaptive or
+// not (stanx/services.hpp), on a GPU-backed GLM model or on a model given
+// as a C callback, and returns the draws as a table.
 #include <cstring>
 #include <exception>
 #include <string>
@@ -20,9 +20,10 @@ namespace {
 
 template <class Model>
 int run_service(const Model& model, const sx_nuts_config* cfg, const double* init,
-                const double* init_inv_metric, sx_nuts_result* out) {
+                const double* init_inv_metric, sx_nuts_result* out, bool adapt = true) {
   if (!cfg || !out || !out->draws) {
-    sx::set_error("sx_hmc_nuts_diag_e_adapt: NULL argument");
+    sx::set_error(adapt ? "sx_hmc_nuts_diag_e_adapt: NULL argument"
+                        : "sx_hmc_nuts_diag_e: NULL argument");
     return SX_CODE_CONFIG;
   }
   const size_t D = model.num_params_r();
@@ -38,14 +39,25 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
   long long n_grad = 0;
   int rc = SX_CODE_SOFTWARE;
   try {
-    rc = stanx::services::sample::hmc_nuts_diag_e_adapt(
-        model, init ? &init_v : nullptr, init_inv_metric ? &minv_v : nullptr,
-        cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
-        cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
-        cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma,
-        cfg->kappa, cfg->t0, cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt,
-        logger, init_writer, sample_writer, diag_writer, &eps, &minv_out, &n_grad, &warm,
-        &samp);
+    if (adapt) {
+      rc = stanx::services::sample::hmc_nuts_diag_e_adapt(
+          model, init ? &init_v : nullptr, init_inv_metric ? &minv_v : nullptr,
+          cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
+          cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
+          cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma,
+          cfg->kappa, cfg->t0, cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt,
+          logger, init_writer, sample_writer, diag_writer, &eps, &minv_out, &n_grad, &warm,
+          &samp);
+    } else {
+      rc = stanx::services::sample::hmc_nuts_diag_e(
+          model, init ? &init_v : nullptr, init_inv_metric ? &minv_v : nullptr,
+          cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
+          cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
+          cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, interrupt, logger,
+          init_writer, sample_writer, diag_writer, &n_grad, &warm, &samp);
+      eps = cfg->stepsize;
+      minv_out = init_inv_metric ? minv_v : std::vector<double>(D, 1.0);
+    }
   } catch (const std::exception& e) {
     sx::set_error(e.what());
     return SX_CODE_SOFTWARE;
@@ -96,6 +108,32 @@ int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_
   }
   stanx::model::callback_model model(f, ctx, D);
   return run_service(model, cfg, init, init_inv_metric, out);
+}
+
+int sx_hmc_nuts_diag_e(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                       const double* init_inv_metric, sx_nuts_result* out) {
+  if (!m) {
+    sx::set_error("sx_hmc_nuts_diag_e: NULL model");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, false);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_diag_e_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config* cfg,
+                          const double* init, const double* init_inv_metric,
+                          sx_nuts_result* out) {
+  if (!f || D < 0) {
+    sx::set_error("sx_hmc_nuts_diag_e_cb: invalid model");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out, false);
 }
 
 }  // extern "C"

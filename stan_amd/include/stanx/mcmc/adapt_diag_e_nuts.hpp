@@ -1,5 +1,7 @@
-// Adaptive diag_e NUTS: a transition followed by dual averaging of the step
-// size and windowed variance estimation of the inverse metric, mirroring
+// diag_e NUTS, plain and adaptive.  The adaptive sampler follows a
+// transition with dual averaging of the step size and windowed variance
+// estimation of the inverse metric, mirroring
+//   diag_e_nuts           src/stan/mcmc/hmc/nuts/diag_e_nuts.hpp:15-22
 //   adapt_diag_e_nuts     src/stan/mcmc/hmc/nuts/adapt_diag_e_nuts.hpp:15-49
 //   stepsize_var_adapter  src/stan/mcmc/stepsize_var_adapter.hpp:12-36
 //   base_adapter          src/stan/mcmc/base_adapter.hpp:7-19
@@ -16,11 +18,33 @@
 namespace stanx {
 namespace mcmc {
 
+// NUTS with a diagonal metric and no adaptation (hmc_nuts_diag_e)
 template <class Model>
-class adapt_diag_e_nuts : public base_nuts<Model> {
+class diag_e_nuts : public base_nuts<Model> {
+ public:
+  diag_e_nuts(const Model& model, rng_t& rng) : base_nuts<Model>(model, rng) {}
+
+  void get_sampler_param_names(std::vector<std::string>& names) const {
+    names.push_back("stepsize__");
+    names.push_back("treedepth__");
+    names.push_back("n_leapfrog__");
+    names.push_back("divergent__");
+    names.push_back("energy__");
+  }
+  void get_sampler_params(std::vector<double>& values) const {
+    values.push_back(this->epsilon_);
+    values.push_back(this->depth_);
+    values.push_back(this->n_leapfrog_);
+    values.push_back(this->divergent_);
+    values.push_back(this->energy_);
+  }
+};
+
+template <class Model>
+class adapt_diag_e_nuts : public diag_e_nuts<Model> {
  public:
   adapt_diag_e_nuts(const Model& model, rng_t& rng)
-      : base_nuts<Model>(model, rng),
+      : diag_e_nuts<Model>(model, rng),
         var_adaptation_(static_cast<int>(model.num_params_r())) {}
 
   void engage_adaptation() { adapt_flag_ = true; }
@@ -51,21 +75,6 @@ class adapt_diag_e_nuts : public base_nuts<Model> {
       }
     }
     return s;
-  }
-
-  void get_sampler_param_names(std::vector<std::string>& names) const {
-    names.push_back("stepsize__");
-    names.push_back("treedepth__");
-    names.push_back("n_leapfrog__");
-    names.push_back("divergent__");
-    names.push_back("energy__");
-  }
-  void get_sampler_params(std::vector<double>& values) const {
-    values.push_back(this->epsilon_);
-    values.push_back(this->depth_);
-    values.push_back(this->n_leapfrog_);
-    values.push_back(this->divergent_);
-    values.push_back(this->energy_);
   }
 
  protected:

@@ -6,8 +6,11 @@
 //   util::mcmc_writer      src/stan/services/util/mcmc_writer.hpp:22-260
 //   util::run_adaptive_sampler
 //                          src/stan/services/util/run_adaptive_sampler.hpp:46-103
+//   util::run_sampler      src/stan/services/util/run_sampler.hpp:39-85
 //   sample::hmc_nuts_diag_e_adapt
 //                          src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:57-112
+//   sample::hmc_nuts_diag_e
+//                          src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91
 #ifndef STANX_SERVICES_HPP
 #define STANX_SERVICES_HPP
 
@@ -40,12 +43,14 @@ inline rng_t create_rng(unsigned int seed, unsigned int chain) {
 
 // Random (or user-supplied) unconstrained inits, checked for a finite log
 // density and gradient; up to 100 tries.  Inits are uniform(-R, R) per
-// coordinate (io/random_var_context.hpp:42-66).
+// coordinate (io/random_var_context.hpp:42-66).  n_evals (optional) counts
+// the gradient evaluations made.
 template <class Model>
 std::vector<double> initialize(const Model& model, const std::vector<double>* init,
                                rng_t& rng, double init_radius, bool print_timing,
                                callbacks::logger& logger,
-                               callbacks::writer& init_writer) {
+                               callbacks::writer& init_writer,
+                               long long* n_evals = nullptr) {
   const size_t D = model.num_params_r();
   std::vector<double> unconstrained(D);
   const bool zero_init = init_radius == 0.0;
@@ -62,6 +67,7 @@ std::vector<double> initialize(const Model& model, const std::vector<double>* in
     std::vector<double> gradient;
     double lp = 0;
     auto start = std::chrono::steady_clock::now();
+    if (n_evals) ++*n_evals;
     try {
       lp = stanx::model::log_prob_grad<true, true>(model, unconstrained,
                                                    params_i, gradient);
@@ -218,9 +224,90 @@ void run_adaptive_sampler(Sampler& sampler, const Model& model,
   if (samp_s) *samp_s = sm;
 }
 
+// Warm-up and sampling without adaptation; unlike run_adaptive_sampler no
+// step-size heuristic runs first.
+template <class Sampler, class Model>
+void run_sampler(Sampler& sampler, const Model& model, std::vector<double>& cont_vector,
+                 int num_warmup, int num_samples, int num_thin, int refresh,
+                 bool save_warmup, rng_t& rng, callbacks::interrupt& interrupt,
+                 callbacks::logger& logger, callbacks::writer& sample_writer,
+                 callbacks::writer& diagnostic_writer, double* warm_s = nullptr,
+                 double* samp_s = nullptr, size_t chain_id = 1, size_t num_chains = 1) {
+  mcmc_writer<Sampler, Model> writer(sample_writer, diagnostic_writer, logger);
+  mcmc::sample s(cont_vector, 0, 0);
+  sampler.z().q = cont_vector;
+  writer.write_sample_names(sampler, model);
+  auto t0 = std::chrono::steady_clock::now();
+  generate_transitions(sampler, num_warmup, 0, num_warmup + num_samples, num_thin,
+                       refresh, save_warmup, true, writer, s, model, rng, interrupt,
+                       logger, chain_id, num_chains);
+  auto t1 = std::chrono::steady_clock::now();
+  writer.write_adapt_finish(sampler);
+  auto t2 = std::chrono::steady_clock::now();
+  generate_transitions(sampler, num_samples, num_warmup, num_warmup + num_samples,
+                       num_thin, refresh, true, false, writer, s, model, rng, interrupt,
+                       logger, chain_id, num_chains);
+  auto t3 = std::chrono::steady_clock::now();
+  const double w = std::chrono::duration<double>(t1 - t0).count();
+  const double sm = std::chrono::duration<double>(t3 - t2).count();
+  writer.write_timing(w, sm);
+  if (warm_s) *warm_s = w;
+  if (samp_s) *samp_s = sm;
+}
+
 }  // namespace util
 
 namespace sample {
+
+// hmc_nuts_diag_e: the same transition, no adaptation.
+template <class Model>
+int hmc_nuts_diag_e(const Model& model, const std::vector<double>* init,
+                    const std::vector<double>* init_inv_metric, unsigned int random_seed,
+                    unsigned int chain, double init_radius, int num_warmup,
+                    int num_samples, int num_thin, bool save_warmup, int refresh,
+                    double stepsize, double stepsize_jitter, int max_depth,
+                    callbacks::interrupt& interrupt, callbacks::logger& logger,
+                    callbacks::writer& init_writer, callbacks::writer& sample_writer,
+                    callbacks::writer& diagnostic_writer, long long* n_grad_out = nullptr,
+                    double* warm_s = nullptr, double* samp_s = nullptr) {
+  rng_t rng = util::create_rng(random_seed, chain);
+  std::vector<double> cont_vector;
+  std::vector<double> inv_metric;
+  long long init_evals = 0;
+  try {
+    cont_vector = util::initialize(model, init, rng, init_radius, true, logger,
+                                   init_writer, &init_evals);
+    if (init_inv_metric) {
+      inv_metric = *init_inv_metric;
+      if (inv_metric.size() != model.num_params_r())
+        throw std::domain_error("inverse metric has the wrong size");
+    } else {
+      inv_metric.assign(model.num_params_r(), 1.0);
+    }
+    for (double v : inv_metric)
+      if (!(v > 0) || !std::isfinite(v))
+        throw std::domain_error("Inverse metric must be positive and finite");
+  } catch (const std::exception& e) {
+    logger.error(e.what());
+    return error_codes::CONFIG;
+  }
+  mcmc::diag_e_nuts<Model> sampler(model, rng);
+  sampler.set_metric(inv_metric);
+  sampler.set_nominal_stepsize(stepsize);
+  sampler.set_stepsize_jitter(stepsize_jitter);
+  sampler.set_max_depth(max_depth);
+  int rc = error_codes::OK;
+  try {
+    util::run_sampler(sampler, model, cont_vector, num_warmup, num_samples, num_thin,
+                      refresh, save_warmup, rng, interrupt, logger, sample_writer,
+                      diagnostic_writer, warm_s, samp_s);
+  } catch (const std::exception& e) {
+    logger.error(e.what());
+    rc = error_codes::SOFTWARE;
+  }
+  if (n_grad_out) *n_grad_out = init_evals + sampler.hamiltonian().n_grad();
+  return rc;
+}
 
 // hmc_nuts_diag_e_adapt with explicit init / metric vectors (the
 // var_context forms of the reference reduce to these after reading).
@@ -244,9 +331,10 @@ int hmc_nuts_diag_e_adapt(const Model& model, const std::vector<double>* init,
   rng_t rng = util::create_rng(random_seed, chain);
   std::vector<double> cont_vector;
   std::vector<double> inv_metric;
+  long long init_evals = 0;
   try {
     cont_vector = util::initialize(model, init, rng, init_radius, true, logger,
-                                   init_writer);
+                                   init_writer, &init_evals);
     if (init_inv_metric) {
       inv_metric = *init_inv_metric;
       if (inv_metric.size() != model.num_params_r())
@@ -284,7 +372,7 @@ int hmc_nuts_diag_e_adapt(const Model& model, const std::vector<double>* init,
   }
   if (stepsize_out) *stepsize_out = sampler->get_nominal_stepsize();
   if (inv_metric_out) *inv_metric_out = sampler->z().inv_e_metric_;
-  if (n_grad_out) *n_grad_out = sampler->hamiltonian().n_grad();
+  if (n_grad_out) *n_grad_out = init_evals + sampler->hamiltonian().n_grad();
   return rc;
 }
 

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import stan_amd as S
+from stan_amd import analyze
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -97,4 +98,35 @@ def test_service_config_error_codes():
     bad_metric = np.array([1.0, -1.0])
     out = S.hmc_nuts_diag_e_adapt_cb(f, None, 2, inv_metric=bad_metric, num_warmup=10,
                                      num_samples=10)
+    assert out["rc"] == 78
+
+
+def test_non_adaptive_service_fixed_stepsize_and_moments():
+    """hmc_nuts_diag_e: the same transition with the step size and metric
+    held fixed (no heuristic, no adaptation); gradient accounting of
+    base_nuts.hpp:85 (one per transition) plus the initialisation."""
+    f = O.model_fn("orc_gauss_model_cb")
+    draws = []
+    for chain in range(1, 5):
+        out = S.hmc_nuts_diag_e_cb(f, None, 3, random_seed=77, chain=chain, num_warmup=100,
+                                   num_samples=1000, save_warmup=1, stepsize=0.9)
+        assert out["rc"] == 0
+        d = out["draws"]
+        assert d.shape == (1100, 10)
+        assert np.all(d[:, 2] == 0.9)
+        assert out["stepsize"] == 0.9
+        np.testing.assert_array_equal(out["inv_metric"], np.ones(3))
+        assert out["n_grad"] == int(d[:, 4].sum()) + d.shape[0] + 1
+        draws.append(d[100:, 7:])
+    x = np.stack(draws, axis=1)
+    for p in range(3):
+        assert abs(x[:, :, p].mean()) < 4 * analyze.mcse_mean(x[:, :, p])
+        assert abs(x[:, :, p].var() - 1.0) < 0.15
+    # jitter: step sizes uniform in eps (1 +- j)
+    out = S.hmc_nuts_diag_e_cb(f, None, 3, random_seed=5, num_warmup=0, num_samples=200,
+                               stepsize=0.5, stepsize_jitter=0.4)
+    eps = out["draws"][:, 2]
+    assert eps.min() >= 0.3 and eps.max() <= 0.7 and eps.std() > 0.05
+    # a bad metric is a configuration error, as in the adaptive service
+    out = S.hmc_nuts_diag_e_cb(f, None, 2, inv_metric=np.array([1.0, 0.0]), num_samples=5)
     assert out["rc"] == 78
