@@ -58,6 +58,7 @@ extern "C" {
 #define SX_EINVAL 2
 #define SX_EDEVICE 3
 #define SX_ENOMEM 4
+#define SX_ETIMEOUT 5
 
 /* families: theta layouts (unconstrained)
  *   SX_LOGIT      : [alpha, beta_1..beta_K]                      D = K+1
@@ -271,6 +272,80 @@ int sx_nuts_batch_results(sx_nuts_batch* b, double* draws, int64_t* rows,
                           int* status, double* stepsize, double* inv_metric,
                           int64_t* n_grad, double* seconds, int64_t* steps);
 void sx_nuts_batch_destroy(sx_nuts_batch* b);
+
+/* ---------------- diagnostics (host only) ---------------- */
+/* x: m chains of n draws, column-major (chain c at x + c * n).
+ * kind 0: stan::analyze::ess (src/stan/analyze/mcmc/ess.hpp:28-103);
+ * kind 1: the bulk ESS of split_rank_normalized_ess
+ * (src/stan/analyze/mcmc/split_rank_normalized_ess.hpp:28-55), NaN when
+ * the split chains do not vary.  Used by the serving loop's stopping rule. */
+int sx_analyze_ess(const double* x, int64_t n, int64_t m, int kind, double* out);
+
+/* ---------------- serving ---------------- */
+/* A request loop over posterior jobs on one GPU (no counterpart in the
+ * reference, whose services run one request at a time).  Jobs are admitted
+ * in arrival order as soon as their HBM footprint fits the budget (first
+ * fit), at most max_active at once; each resident job keeps its chain state
+ * on the device and its graph launches overlap with the other jobs' on its
+ * own stream.  When a job's chains finish, its min bulk ESS is computed; if
+ * it is below target_min_ess the sampling phase is continued (up to
+ * max_samples iterations per chain), else the job completes and its device
+ * memory is freed.  A job's draws do not depend on the other jobs.      */
+typedef struct sx_server sx_server; /* opaque */
+
+#define SX_JOB_QUEUED 0
+#define SX_JOB_RUNNING 1
+#define SX_JOB_DONE 2
+#define SX_JOB_FAILED 3
+
+typedef struct sx_job_spec {
+  int family;                 /* SX_LOGIT, SX_NORMAL, SX_HIER_LOGIT            */
+  const double* X;            /* N x K row-major (copied at submit)            */
+  const void* y;              /* as sx_glm_create (copied)                     */
+  int64_t N, K;
+  const int64_t* group_offsets; /* J + 1 (SX_HIER_LOGIT), else NULL (copied) */
+  int64_t J;
+  sx_prior prior;
+  int64_t chains;
+  int grad_precision;         /* SX_BF16X2 (used when K <= 256), SX_FP32
+                                 (SX_HIER_LOGIT) or SX_FP64                    */
+  sx_nuts_config cfg;         /* chain ids cfg.chain, cfg.chain + 1, ...;
+                                 cfg.num_samples: first sampling phase       */
+  double target_min_ess;      /* <= 0: run cfg.num_samples only              */
+  int64_t max_samples;        /* cap on sampling iterations when extending   */
+} sx_job_spec;
+
+typedef struct sx_job_info {
+  int state;                  /* SX_JOB_*                                      */
+  int rc;                     /* status of a failed job                        */
+  int64_t footprint_bytes;    /* HBM reserved while resident                   */
+  double t_submit, t_admit, t_done; /* seconds since the server started      */
+  int64_t n_grad;             /* gradient evaluations over all chains          */
+  int64_t steps;              /* lockstep steps                                */
+  int64_t D, chains, rows_cap;
+  int64_t sampling_iterations;
+  int extensions;
+  double min_ess;             /* min bulk ESS over parameters at completion    */
+} sx_job_info;
+
+int sx_server_create(sx_server** out, int device, int64_t hbm_budget_bytes,
+                     int max_active);
+/* estimated device bytes of a job (what admission reserves)            */
+int sx_server_footprint(const sx_job_spec* spec, int64_t* bytes);
+/* non-blocking; SX_ENOMEM if the footprint exceeds the whole budget     */
+int sx_server_submit(sx_server* srv, const sx_job_spec* spec, int64_t* job_id);
+int sx_server_info(sx_server* srv, int64_t job_id, sx_job_info* info);
+/* waits for completion (timeout_s < 0: no limit); SX_ETIMEOUT on timeout,
+ * the job's status if it failed                                          */
+int sx_server_wait(sx_server* srv, int64_t job_id, double timeout_s);
+/* draws: chains x rows_cap x (7 + D) as sx_nuts_batch_results; rows and
+ * status: per chain                                                     */
+int sx_server_results(sx_server* srv, int64_t job_id, double* draws,
+                      int64_t* rows, int* status);
+/* forget a finished job and its host copy of the draws                  */
+int sx_server_release(sx_server* srv, int64_t job_id);
+/* drops queued jobs, waits for resident ones, stops the scheduler       */
+void sx_server_destroy(sx_server* srv);
 
 #ifdef __cplusplus
 }

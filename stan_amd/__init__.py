@@ -24,7 +24,9 @@ EXPORTED = [
     "sx_last_error", "sx_glm_logp_grad", "sx_glm_logp", "sx_glm_logp_grad_batch",
     "sx_glm_time_logp_grad", "sx_glm_kernel_config", "sx_hmc_nuts_diag_e_adapt",
     "sx_hmc_nuts_diag_e_adapt_cb", "sx_hmc_nuts_diag_e", "sx_hmc_nuts_diag_e_cb",
-    "sx_nuts_batch_create", "sx_nuts_batch_run",
+    "sx_nuts_batch_create", "sx_server_create", "sx_server_footprint", "sx_server_submit",
+    "sx_server_info", "sx_server_wait", "sx_server_results", "sx_server_release",
+    "sx_server_destroy", "sx_analyze_ess", "sx_nuts_batch_run",
     "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
 ]
 
@@ -60,6 +62,23 @@ class NutsConfig(C.Structure):
                 ("max_depth", C.c_int32), ("delta", C.c_double), ("gamma", C.c_double),
                 ("kappa", C.c_double), ("t0", C.c_double), ("init_buffer", C.c_uint32),
                 ("term_buffer", C.c_uint32), ("window", C.c_uint32)]
+
+
+class JobSpec(C.Structure):
+    _fields_ = [("family", C.c_int), ("X", C.c_void_p), ("y", C.c_void_p), ("N", C.c_int64),
+                ("K", C.c_int64), ("group_offsets", C.c_void_p), ("J", C.c_int64),
+                ("prior", Prior), ("chains", C.c_int64), ("grad_precision", C.c_int),
+                ("cfg", NutsConfig), ("target_min_ess", C.c_double),
+                ("max_samples", C.c_int64)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("rc", C.c_int), ("footprint_bytes", C.c_int64),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("n_grad", C.c_int64), ("steps", C.c_int64), ("D", C.c_int64),
+                ("chains", C.c_int64), ("rows_cap", C.c_int64),
+                ("sampling_iterations", C.c_int64), ("extensions", C.c_int),
+                ("min_ess", C.c_double)]
 
 
 class NutsResult(C.Structure):
@@ -105,6 +124,16 @@ def lib():
                                          C.POINTER(NutsResult)]
         L.sx_hmc_nuts_diag_e_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                             vp, vp, C.POINTER(NutsResult)]
+        L.sx_analyze_ess.argtypes = [vp, i64, i64, C.c_int, dp]
+        L.sx_server_create.argtypes = [C.POINTER(vp), C.c_int, i64, C.c_int]
+        L.sx_server_footprint.argtypes = [C.POINTER(JobSpec), C.POINTER(i64)]
+        L.sx_server_submit.argtypes = [vp, C.POINTER(JobSpec), C.POINTER(i64)]
+        L.sx_server_info.argtypes = [vp, i64, C.POINTER(JobInfo)]
+        L.sx_server_wait.argtypes = [vp, i64, C.c_double]
+        L.sx_server_results.argtypes = [vp, i64, vp, vp, vp]
+        L.sx_server_release.argtypes = [vp, i64]
+        L.sx_server_destroy.argtypes = [vp]
+        L.sx_server_destroy.restype = None
         L.sx_nuts_batch_create.argtypes = [C.POINTER(vp), vp, C.POINTER(NutsConfig), i64,
                                            C.c_int, vp, vp]
         L.sx_nuts_batch_run.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64)]
@@ -360,3 +389,92 @@ def hmc_nuts_diag_e_adapt_batch(model, chains, grad_precision=FP64, inits=None,
         return b.results()
     finally:
         b.close()
+
+
+SX_JOB_QUEUED, SX_JOB_RUNNING, SX_JOB_DONE, SX_JOB_FAILED = 0, 1, 2, 3
+SX_ETIMEOUT = 5
+
+
+def job_spec(family, X, y, chains, offsets=None, prior=(5.0, 2.0, 2.0, 1.0),
+             grad_precision=BF16X2, target_min_ess=0.0, max_samples=0, **kw):
+    """A serving job: data, chains, sampler configuration (nuts_config
+    keywords) and the min-ESS target.  Returns (spec, arrays kept alive)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    N, K = X.shape
+    y = np.ascontiguousarray(y, dtype=np.float64 if family == NORMAL else np.int32)
+    off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.int64)
+    J = 0 if off is None else len(off) - 1
+    spec = JobSpec(family, _ptr(X).value if X.size else _ptr(np.zeros(1)).value,
+                   _ptr(y).value, N, K, None if off is None else _ptr(off).value, J,
+                   Prior(*prior), chains, grad_precision, nuts_config(**kw),
+                   float(target_min_ess), int(max_samples))
+    return spec, (X, y, off)
+
+
+def job_footprint(spec):
+    b = C.c_int64()
+    _check(lib().sx_server_footprint(C.byref(spec), C.byref(b)))
+    return b.value
+
+
+class Server:
+    """Request loop over posterior jobs on one GPU (admission by HBM
+    footprint, device-resident chains, overlapping work, min-ESS target)."""
+
+    def __init__(self, hbm_budget_bytes, max_active=16, device=0):
+        self._h = C.c_void_p()
+        _check(lib().sx_server_create(C.byref(self._h), device, int(hbm_budget_bytes),
+                                      int(max_active)))
+        self._jobs = {}
+
+    def submit(self, family, X, y, chains, **kw):
+        spec, keep = job_spec(family, X, y, chains, **kw)
+        jid = C.c_int64()
+        _check(lib().sx_server_submit(self._h, C.byref(spec), C.byref(jid)))
+        self._jobs[jid.value] = spec.K + (2 if family == NORMAL else 1)
+        return jid.value
+
+    def info(self, jid):
+        inf = JobInfo()
+        _check(lib().sx_server_info(self._h, jid, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in JobInfo._fields_}
+
+    def wait(self, jid, timeout=-1.0):
+        rc = lib().sx_server_wait(self._h, jid, float(timeout))
+        if rc == SX_ETIMEOUT:
+            return False
+        _check(rc)
+        return True
+
+    def results(self, jid):
+        inf = self.info(jid)
+        Cn, cap, D = inf["chains"], inf["rows_cap"], inf["D"]
+        draws = np.zeros((Cn, max(cap, 1), 7 + D))
+        rows = np.zeros(Cn, dtype=np.int64)
+        status = np.zeros(Cn, dtype=np.int32)
+        _check(lib().sx_server_results(self._h, jid, _ptr(draws), _ptr(rows), _ptr(status)))
+        return dict(draws=draws[:, :cap], rows=rows, status=status, info=inf)
+
+    def release(self, jid):
+        _check(lib().sx_server_release(self._h, jid))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sx_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def native_ess(chains, kind="bulk"):
+    """ESS of (iterations x chains) draws by the library's C++ routines
+    (kind "ess" or "bulk"), as the serving loop computes it."""
+    x = np.ascontiguousarray(np.asarray(chains, dtype=np.float64).T)
+    out = C.c_double()
+    _check(lib().sx_analyze_ess(_ptr(x), x.shape[1], x.shape[0], 0 if kind == "ess" else 1,
+                                C.byref(out)))
+    return out.value

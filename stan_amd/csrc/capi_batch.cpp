@@ -39,6 +39,11 @@ struct sx_nuts_batch {
   int64_t steps = 0;
   double seconds = 0.0;
   bool finished = false;
+  // non-blocking driving (serving): graph launches in flight, each followed
+  // by a copy of the done counter and an event
+  static constexpr int kRing = 2;
+  hipEvent_t ev[kRing] = {nullptr, nullptr};
+  int head = 0, inflight = 0;
 };
 
 namespace {
@@ -95,6 +100,8 @@ void release(sx_nuts_batch* b) {
   (void)hipFree(b->d_out);
   (void)hipFree(b->d_theta);
   (void)hipFree(b->d_inits);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
   sx::free_workspace(b->mx);
   sx::free_workspace(b->hw);
   if (b->h_done) (void)hipHostFree(b->h_done);
@@ -104,11 +111,12 @@ void release(sx_nuts_batch* b) {
 
 }  // namespace
 
-extern "C" {
+namespace sx {
+namespace nb {
 
-int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg,
-                         int64_t chains, int grad_precision, const double* inits,
-                         const double* inv_metric) {
+int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int64_t chains,
+                 int grad_precision, const double* inits, const double* inv_metric,
+                 int64_t max_samples) {
   if (!out || !m || !cfg || chains <= 0) {
     set_error("sx_nuts_batch_create: invalid argument");
     return SX_EINVAL;
@@ -141,7 +149,8 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
   b->grad_precision = grad_precision;
   const int thin = cfg->num_thin > 0 ? cfg->num_thin : 1;
   const int64_t warm_rows = cfg->save_warmup ? (cfg->num_warmup + thin - 1) / thin : 0;
-  b->rows_cap = warm_rows + (cfg->num_samples + thin - 1) / thin;
+  const int64_t cap_samples = std::max<int64_t>(std::max(0, cfg->num_samples), max_samples);
+  b->rows_cap = warm_rows + (cap_samples + thin - 1) / thin;
   auto fail = [&](int code, const char* what) {
     set_error(std::string("sx_nuts_batch_create: ") + what);
     release(b);
@@ -149,6 +158,9 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(SX_EDEVICE, "stream");
+  for (auto& e : b->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(SX_EDEVICE, "event");
   const int ncol = static_cast<int>(7 + D);
   const size_t nvec = static_cast<size_t>(chains) * sx::nb::kNumSlots * b->Dpad;
   if (hipMalloc(&b->a.states, sizeof(sx::nb::chain_state) * chains) != hipSuccess ||
@@ -197,6 +209,7 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
       s.mu = std::log(10 * cfg->stepsize);
       s.w_size = bw;
       s.w_next = ib + bw - 1;
+      s.iter_end = std::max(0, cfg->num_warmup) + std::max(0, cfg->num_samples);
     }
     if (hipMemcpy(b->a.states, st.data(), sizeof(sx::nb::chain_state) * chains,
                   hipMemcpyHostToDevice) != hipSuccess)
@@ -291,6 +304,86 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
     return fail(SX_EDEVICE, "graph instantiate");
   *out = b;
   return SX_OK;
+}
+
+// One graph launch (steps_per_graph lockstep steps) without waiting, then
+// the done counter copy and an event; at most kRing launches in flight.
+int launch(sx_nuts_batch* b) {
+  // nothing left to do once every chain has finished (a run is resumed by
+  // extend); a full ring waits for poll
+  if (b->finished || b->inflight >= sx_nuts_batch::kRing) return SX_OK;
+  SX_HIP(hipGraphLaunch(b->exec, b->stream));
+  SX_HIP(hipMemcpyAsync(b->h_done, b->a.done_count, sizeof(unsigned), hipMemcpyDeviceToHost,
+                        b->stream));
+  const int slot = (b->head + b->inflight) % sx_nuts_batch::kRing;
+  SX_HIP(hipEventRecord(b->ev[slot], b->stream));
+  ++b->inflight;
+  b->steps += b->steps_per_graph;
+  return SX_OK;
+}
+
+// Retire completed launches; *finished once every chain is done.
+int poll(sx_nuts_batch* b, bool* finished) {
+  while (b->inflight > 0) {
+    const hipError_t e = hipEventQuery(b->ev[b->head]);
+    if (e == hipErrorNotReady) break;
+    SX_HIP(e);
+    b->head = (b->head + 1) % sx_nuts_batch::kRing;
+    --b->inflight;
+    if (*b->h_done >= static_cast<unsigned>(b->C)) b->finished = true;
+  }
+  *finished = b->finished && b->inflight == 0;
+  return SX_OK;
+}
+
+int inflight(const sx_nuts_batch* b) { return b->inflight; }
+
+// Raise the sampling iterations of every chain that finished normally by
+// `extra` and resume it where it stopped: the continued run is the run
+// that would have had the larger num_samples from the start.
+int extend(sx_nuts_batch* b, int extra) {
+  SX_HIP(hipStreamSynchronize(b->stream));
+  b->inflight = 0;
+  std::vector<chain_state> st(static_cast<size_t>(b->C));
+  SX_HIP(hipMemcpy(st.data(), b->a.states, sizeof(chain_state) * b->C, hipMemcpyDeviceToHost));
+  unsigned still_done = 0;
+  for (auto& s : st) {
+    if (s.phase == PH_DONE && s.status == 0) {
+      s.iter_end += extra;
+      s.phase = PH_TRANS_BEGIN;
+      s.pending = 0;
+    } else if (s.phase == PH_DONE) {
+      ++still_done;
+    }
+  }
+  SX_HIP(hipMemcpy(b->a.states, st.data(), sizeof(chain_state) * b->C, hipMemcpyHostToDevice));
+  SX_HIP(hipMemcpy(b->a.done_count, &still_done, sizeof(unsigned), hipMemcpyHostToDevice));
+  b->finished = still_done >= static_cast<unsigned>(b->C);
+  return SX_OK;
+}
+
+int64_t sampling_rows(const sx_nuts_batch* b, int64_t* warm_rows) {
+  const int thin = b->a.num_thin;
+  *warm_rows = b->a.save_warmup ? (b->a.num_warmup + thin - 1) / thin : 0;
+  return b->rows_cap;
+}
+
+hipStream_t stream(const sx_nuts_batch* b) { return b->stream; }
+
+}  // namespace nb
+}  // namespace sx
+
+extern "C" {
+
+int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg,
+                         int64_t chains, int grad_precision, const double* inits,
+                         const double* inv_metric) {
+  if (!cfg) {
+    set_error("sx_nuts_batch_create: invalid argument");
+    return SX_EINVAL;
+  }
+  return sx::nb::create_batch(out, m, cfg, chains, grad_precision, inits, inv_metric,
+                              cfg->num_samples);
 }
 
 void sx_nuts_batch_destroy(sx_nuts_batch* b) { release(b); }

@@ -588,7 +588,7 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         break;
       }
       case PH_TRANS_BEGIN: {
-        if (s.iter == a.num_warmup + a.num_samples) {
+        if (s.iter >= s.iter_end) {
           finish(k, 0);
           break;
         }

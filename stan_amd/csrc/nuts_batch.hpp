@@ -33,7 +33,8 @@ struct chain_state {
   int depth, tree_depth, n_leapfrog, divergent, fwd;
   int ss_ret, ss_iter, ss_dir, last_ok;
   int stage[kMaxDepth + 1];
-  unsigned w_counter, w_next, w_size, pad0;
+  unsigned w_counter, w_next, w_size;
+  int iter_end;  // warm-up + sampling iterations; raised to extend a run
   uint64_t rng;
   int64_t n_grad;
   double nom_eps, eps, H0, lsw, lsw_sub, smp, accept;
@@ -61,6 +62,23 @@ struct batch_args {
 
 hipError_t launch_nuts_step(const batch_args& a, hipStream_t s);
 
+}  // namespace nb
+}  // namespace sx
+
+struct sx_glm;
+struct sx_nuts_batch;
+struct sx_nuts_config;
+
+namespace sx {
+namespace nb {
+// create with room for up to max_samples sampling iterations per chain
+int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int64_t chains,
+                 int grad_precision, const double* inits, const double* inv_metric,
+                 int64_t max_samples);
+int launch(sx_nuts_batch* b);                  // non-blocking graph launch
+int poll(sx_nuts_batch* b, bool* finished);    // retire finished launches
+int inflight(const sx_nuts_batch* b);
+int extend(sx_nuts_batch* b, int extra);       // more sampling iterations
 }  // namespace nb
 }  // namespace sx
 #endif

@@ -98,3 +98,19 @@ def test_min_bulk_ess_matches_per_parameter(bern):
     d = np.stack([bern["lp"], bern["theta"]], axis=2)
     assert abs(A.min_bulk_ess(d) - 1407.5124) < 1e-4
     assert np.isnan(A.min_bulk_ess(x[:, :, [9]]))
+
+
+def test_native_ess_matches_goldens(bern):
+    """The C++ port used by the serving loop (stanx/analyze.hpp)."""
+    import stan_amd as S
+    assert abs(S.native_ess(bern["lp"], "ess") - 1335.41366787) < 1e-8
+    assert abs(S.native_ess(bern["theta"], "ess") - 1377.5030282) < 1e-8
+    assert abs(S.native_ess(bern["lp"]) - 1512.7684) < 1e-4
+    assert abs(S.native_ess(bern["theta"]) - 1407.5124) < 1e-4
+    assert np.isnan(S.native_ess(bern["divergent"]))
+    rng = np.random.Generator(np.random.PCG64(9))
+    x = np.zeros((301, 3))
+    for t in range(1, 301):
+        x[t] = 0.6 * x[t - 1] + rng.standard_normal(3)
+    x[10:14, 2] = x[9, 2]
+    assert abs(S.native_ess(x) - A.split_rank_normalized_ess(x)[0]) < 1e-9 * S.native_ess(x)
