@@ -65,13 +65,24 @@ def nuts_run(model, warmup, samples, seed=1234):
                 warmup=warmup, samples=samples)
 
 
-def batched_run(model, chains, warmup, samples, grad_precision, seed=1234):
-    """Device-resident NUTS over `chains` chains; min-ESS over pooled chains."""
+def batched_run(model, chains, warmup, samples, grad_precision, seed=1234, **kw):
+    """Device-resident NUTS over `chains` chains; min-ESS over pooled chains.
+    Runs in bounded pieces so that progress is reported while it runs."""
     import stan_amd as S
     from stan_amd import analyze
-    res = S.hmc_nuts_diag_e_adapt_batch(model, chains, grad_precision=grad_precision,
-                                        random_seed=seed, chain=1, num_warmup=warmup,
-                                        num_samples=samples)
+    b = S.NutsBatch(model, chains, grad_precision=grad_precision, random_seed=seed, chain=1,
+                    num_warmup=warmup, num_samples=samples, **kw)
+    t0 = time.perf_counter()
+    last = t0
+    while True:
+        steps, done = b.run(max_steps=2048)
+        if done >= chains or steps == 0:
+            break
+        if time.perf_counter() - last > 10:
+            last = time.perf_counter()
+            log(f"  ... {done}/{chains} chains done, {last - t0:.0f} s")
+    res = b.results()
+    b.close()
     if not (res["status"] == 0).all():
         raise RuntimeError(f"chain status {res['status']}")
     q = np.transpose(res["draws"][:, :, 7:], (1, 0, 2))  # iterations x chains x params
@@ -82,8 +93,86 @@ def batched_run(model, chains, warmup, samples, grad_precision, seed=1234):
                 grad_evals_per_s=float(res["n_grad"].sum()) / secs, steps=int(res["steps"]),
                 ms_per_step=1e3 * secs / max(1, res["steps"]),
                 mean_n_leapfrog=float(res["draws"][:, :, 4].mean()),
+                mean_treedepth=float(res["draws"][:, :, 3].mean()),
                 stepsize_median=float(np.median(res["stepsize"])), warmup=warmup,
                 samples=samples)
+
+
+def serving_stream(n_jobs, rate=20.0, seed=7):
+    """Poisson arrivals (rate jobs/s); per job: logistic or normal GLM with
+    equal probability, N log-uniform in [1e3, 2e6], K log-uniform in
+    [8, 512], chains uniform in {4, 8, 16, 32, 64}."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    logit = rng.random(n_jobs) < 0.5
+    N = np.exp(rng.uniform(np.log(1e3), np.log(2e6), n_jobs)).astype(np.int64)
+    K = np.exp(rng.uniform(np.log(8), np.log(512), n_jobs)).astype(np.int64)
+    C = rng.choice([4, 8, 16, 32, 64], n_jobs)
+    return [dict(t=float(t[i]), logit=bool(logit[i]), N=int(N[i]), K=int(K[i]), C=int(C[i]),
+                 seed=1000 + i) for i in range(n_jobs)]
+
+
+def serving_data(job):
+    """X ~ N(0, 1/K); logistic: beta ~ N(0,1), alpha = 0.5; normal: sigma = 1."""
+    rng = np.random.Generator(np.random.PCG64(job["seed"]))
+    N, K = job["N"], job["K"]
+    X = rng.standard_normal((N, K), dtype=np.float32).astype(np.float64) / np.sqrt(K)
+    beta = rng.standard_normal(K)
+    eta = X @ beta + 0.5
+    if job["logit"]:
+        y = (rng.random(N) < 1.0 / (1.0 + np.exp(-eta))).astype(np.int32)
+    else:
+        y = eta + rng.standard_normal(N)
+    return X, y
+
+
+def serving_section(n_jobs, target_ess=1000.0, warmup=200):
+    """C5: a bounded prefix of the arrival stream through the serving loop."""
+    import stan_amd as S
+    from stan_amd import workloads as W
+    jobs = serving_stream(n_jobs)
+    log(f"serving: generating {n_jobs} jobs")
+    data = [serving_data(j) for j in jobs]
+    srv = S.Server(200 * (1 << 30), max_active=16)
+    ids = []
+    t0 = time.perf_counter()
+    for j, (X, y) in zip(jobs, data):
+        delay = j["t"] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        fam = S.LOGIT if j["logit"] else S.NORMAL
+        prior = W.PRIOR_LOGISTIC if j["logit"] else W.PRIOR_NORMAL
+        first = int(np.ceil(1.5 * target_ess / j["C"])) + 20
+        ids.append(srv.submit(fam, X, y, j["C"], prior=prior, grad_precision=S.BF16X2,
+                              target_min_ess=target_ess, max_samples=max(400, 8 * first),
+                              num_warmup=warmup, num_samples=first, random_seed=j["seed"],
+                              chain=1))
+    del data
+    last = time.perf_counter()
+    pending = list(ids)
+    while pending:
+        pending = [i for i in pending if not srv.wait(i, 0.5)]
+        if time.perf_counter() - last > 10:
+            last = time.perf_counter()
+            log(f"  ... serving: {len(ids) - len(pending)}/{len(ids)} jobs done")
+    infos = [srv.info(i) for i in ids]
+    srv.close()
+    lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    queue = np.array([i["t_admit"] - i["t_submit"] for i in infos])
+    t_first = min(i["t_submit"] for i in infos)
+    makespan = max(i["t_done"] for i in infos) - t_first
+    n_grad = sum(i["n_grad"] for i in infos)
+    ess = [i["min_ess"] for i in infos]
+    return dict(jobs=n_jobs, arrival_rate_per_s=20.0, seed=7, target_min_ess=target_ess,
+                warmup=warmup, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), max_s=float(lat.max()),
+                p50_queue_s=float(np.percentile(queue, 50)), makespan_s=makespan,
+                grad_evals_per_s=n_grad / makespan,
+                min_ess_per_s=float(np.nansum(ess)) / makespan,
+                jobs_reaching_target=int(sum(e >= target_ess for e in ess)),
+                min_ess_min=float(np.nanmin(ess)),
+                extensions=int(sum(i["extensions"] for i in infos)),
+                mfma_jobs=int(sum(1 for j in jobs if j["K"] <= 256)))
 
 
 def mixed_error(model, ref_data, family, thetas):
@@ -188,6 +277,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--no-hier", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=30)
     ap.add_argument("--no-c1", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -292,6 +383,9 @@ def main():
         bn["max_rel_err_grad"] = e_g
         result["normal_glm_batched_256"] = bn
         mn.close()
+    if not args.no_serve:
+        log(f"serving: first {args.serve_jobs} jobs of the arrival stream")
+        result["serving"] = serving_section(args.serve_jobs)
     if not args.no_hier:
         result["hier_logistic_64"] = hier_section(300, 300)
     if not args.no_c1:

@@ -229,6 +229,27 @@ int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D,
                                 const double* init_inv_metric,
                                 sx_nuts_result* out);
 
+/* The adaptive service writing its sample output as a Stan CSV file, as
+ * the reference's services do through a stream_writer with comment prefix
+ * "# " (src/stan/callbacks/stream_writer.hpp:14-93): a configuration block
+ * in CmdStan's layout, the header (lp__, accept_stat__, ..., parameter
+ * names), warm-up rows when saved, "Adaptation terminated", the step size
+ * and inverse metric lines, the draws, and the timing block
+ * (src/stan/services/util/mcmc_writer.hpp:91-238).  sig_figs > 0 sets the
+ * significant digits of the values (the stream default otherwise).  out
+ * receives the same rows as the table entry points.                      */
+int sx_hmc_nuts_diag_e_adapt_csv(sx_glm* m, const sx_nuts_config* cfg,
+                                 const double* init,
+                                 const double* init_inv_metric,
+                                 const char* csv_path, int sig_figs,
+                                 sx_nuts_result* out);
+int sx_hmc_nuts_diag_e_adapt_cb_csv(sx_lpgrad_fn f, void* ctx, int D,
+                                    const sx_nuts_config* cfg,
+                                    const double* init,
+                                    const double* init_inv_metric,
+                                    const char* csv_path, int sig_figs,
+                                    sx_nuts_result* out);
+
 /* hmc_nuts_diag_e (src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91)
  * through run_sampler (src/stan/services/util/run_sampler.hpp:39-85): the
  * step size cfg->stepsize (with jitter) and the metric init_inv_metric (or

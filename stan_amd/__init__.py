@@ -26,7 +26,8 @@ EXPORTED = [
     "sx_hmc_nuts_diag_e_adapt_cb", "sx_hmc_nuts_diag_e", "sx_hmc_nuts_diag_e_cb",
     "sx_nuts_batch_create", "sx_server_create", "sx_server_footprint", "sx_server_submit",
     "sx_server_info", "sx_server_wait", "sx_server_results", "sx_server_release",
-    "sx_server_destroy", "sx_analyze_ess", "sx_nuts_batch_run",
+    "sx_server_destroy", "sx_analyze_ess", "sx_hmc_nuts_diag_e_adapt_csv",
+    "sx_hmc_nuts_diag_e_adapt_cb_csv", "sx_nuts_batch_run",
     "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
 ]
 
@@ -124,6 +125,11 @@ def lib():
                                          C.POINTER(NutsResult)]
         L.sx_hmc_nuts_diag_e_cb.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                             vp, vp, C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_adapt_csv.argtypes = [vp, C.POINTER(NutsConfig), vp, vp,
+                                                   C.c_char_p, C.c_int, C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_adapt_cb_csv.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
+                                                      vp, vp, C.c_char_p, C.c_int,
+                                                      C.POINTER(NutsResult)]
         L.sx_analyze_ess.argtypes = [vp, i64, i64, C.c_int, dp]
         L.sx_server_create.argtypes = [C.POINTER(vp), C.c_int, i64, C.c_int]
         L.sx_server_footprint.argtypes = [C.POINTER(JobSpec), C.POINTER(i64)]
@@ -307,6 +313,29 @@ def hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
     return _run_service(
         lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_cb(fn_ptr, ctx, D, C.byref(cfg),
                                                           _ptr(i), _ptr(m), C.byref(r)),
+        D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e_adapt_csv(model, path, sig_figs=-1, init=None, inv_metric=None, **kw):
+    """hmc_nuts_diag_e_adapt writing a Stan CSV file at `path` (readable by
+    stan_amd.stan_csv.read_stan_csv and CmdStan tools); returns the same dict."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_csv(model.handle, C.byref(cfg), _ptr(i),
+                                                           _ptr(m), str(path).encode(),
+                                                           int(sig_figs), C.byref(r)),
+        model.D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e_adapt_cb_csv(fn_ptr, ctx, D, path, sig_figs=-1, init=None,
+                                 inv_metric=None, **kw):
+    """The CSV-writing service for a model given as a C callback."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_cb_csv(fn_ptr, ctx, D, C.byref(cfg),
+                                                              _ptr(i), _ptr(m),
+                                                              str(path).encode(),
+                                                              int(sig_figs), C.byref(r)),
         D, cfg, init, inv_metric)
 
 

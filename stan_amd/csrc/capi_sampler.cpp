@@ -3,6 +3,8 @@
 // as a C callback, and returns the draws as a table.
 #include <cstring>
 #include <exception>
+#include <fstream>
+#include <iomanip>
 #include <string>
 #include <vector>
 
@@ -18,9 +20,51 @@ void set_error(const std::string& msg);
 
 namespace {
 
+// forwards every call to two writers
+class tee_writer : public stanx::callbacks::writer {
+ public:
+  tee_writer(stanx::callbacks::writer& a, stanx::callbacks::writer& b) : a_(a), b_(b) {}
+  void operator()(const std::vector<std::string>& n) override { a_(n); b_(n); }
+  void operator()(const std::vector<double>& v) override { a_(v); b_(v); }
+  void operator()() override { a_(); b_(); }
+  void operator()(const std::string& m) override { a_(m); b_(m); }
+
+ private:
+  stanx::callbacks::writer& a_;
+  stanx::callbacks::writer& b_;
+};
+
+// the configuration block at the top of a Stan CSV file, in CmdStan's
+// layout (key = value lines, nesting by indentation)
+void write_csv_config(std::ostream& o, const std::string& model_name, const sx_nuts_config* c,
+                      bool adapt, int sig_figs) {
+  o << "# stan_version_major = 2\n# stan_version_minor = 35\n# stan_version_patch = 0\n";
+  o << "# model = " << model_name << "\n";
+  o << "# method = sample (Default)\n#   sample\n";
+  o << "#     num_samples = " << c->num_samples << "\n";
+  o << "#     num_warmup = " << c->num_warmup << "\n";
+  o << "#     save_warmup = " << (c->save_warmup ? "true" : "false") << "\n";
+  o << "#     thin = " << (c->num_thin > 0 ? c->num_thin : 1) << "\n";
+  o << "#     adapt\n#       engaged = " << (adapt ? "true" : "false") << "\n";
+  o << "#       gamma = " << c->gamma << "\n#       delta = " << c->delta << "\n";
+  o << "#       kappa = " << c->kappa << "\n#       t0 = " << c->t0 << "\n";
+  o << "#       init_buffer = " << c->init_buffer << "\n#       term_buffer = "
+    << c->term_buffer << "\n#       window = " << c->window << "\n";
+  o << "#     algorithm = hmc (Default)\n#       hmc\n#         engine = nuts (Default)\n";
+  o << "#           nuts\n#             max_depth = " << c->max_depth << "\n";
+  o << "#         metric = diag_e (Default)\n";
+  o << "#         stepsize = " << c->stepsize << "\n";
+  o << "#         stepsize_jitter = " << c->stepsize_jitter << "\n";
+  o << "#     num_chains = 1 (Default)\n";
+  o << "# id = " << c->chain << "\n# init = " << c->init_radius << "\n";
+  o << "# random\n#   seed = " << c->random_seed << "\n";
+  o << "# output\n#   sig_figs = " << sig_figs << "\n";
+}
+
 template <class Model>
 int run_service(const Model& model, const sx_nuts_config* cfg, const double* init,
-                const double* init_inv_metric, sx_nuts_result* out, bool adapt = true) {
+                const double* init_inv_metric, sx_nuts_result* out, bool adapt = true,
+                const char* csv_path = nullptr, int sig_figs = -1) {
   if (!cfg || !out || !out->draws) {
     sx::set_error(adapt ? "sx_hmc_nuts_diag_e_adapt: NULL argument"
                         : "sx_hmc_nuts_diag_e: NULL argument");
@@ -34,6 +78,20 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
   stanx::callbacks::counting_logger logger;
   stanx::callbacks::writer init_writer, diag_writer;
   stanx::callbacks::table_writer sample_writer;
+  std::ofstream csv;
+  if (csv_path) {
+    csv.open(csv_path);
+    if (!csv) {
+      sx::set_error(std::string("cannot open ") + csv_path);
+      return SX_CODE_CONFIG;
+    }
+    if (sig_figs > 0) csv << std::setprecision(sig_figs);
+    write_csv_config(csv, model.model_name(), cfg, adapt, sig_figs);
+  }
+  stanx::callbacks::stream_writer csv_writer(csv, "# ");
+  stanx::callbacks::writer null_writer;
+  tee_writer out_writer(sample_writer, csv_path ? static_cast<stanx::callbacks::writer&>(csv_writer)
+                                                : null_writer);
   double eps = 0.0, warm = 0.0, samp = 0.0;
   std::vector<double> minv_out;
   long long n_grad = 0;
@@ -46,7 +104,7 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
           cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
           cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma,
           cfg->kappa, cfg->t0, cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt,
-          logger, init_writer, sample_writer, diag_writer, &eps, &minv_out, &n_grad, &warm,
+          logger, init_writer, out_writer, diag_writer, &eps, &minv_out, &n_grad, &warm,
           &samp);
     } else {
       rc = stanx::services::sample::hmc_nuts_diag_e(
@@ -54,7 +112,7 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
           cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
           cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
           cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, interrupt, logger,
-          init_writer, sample_writer, diag_writer, &n_grad, &warm, &samp);
+          init_writer, out_writer, diag_writer, &n_grad, &warm, &samp);
       eps = cfg->stepsize;
       minv_out = init_inv_metric ? minv_v : std::vector<double>(D, 1.0);
     }
@@ -108,6 +166,33 @@ int sx_hmc_nuts_diag_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_
   }
   stanx::model::callback_model model(f, ctx, D);
   return run_service(model, cfg, init, init_inv_metric, out);
+}
+
+int sx_hmc_nuts_diag_e_adapt_csv(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                                 const double* init_inv_metric, const char* csv_path,
+                                 int sig_figs, sx_nuts_result* out) {
+  if (!m || !csv_path) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt_csv: NULL model or path");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, true, csv_path, sig_figs);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_diag_e_adapt_cb_csv(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config* cfg,
+                                    const double* init, const double* init_inv_metric,
+                                    const char* csv_path, int sig_figs, sx_nuts_result* out) {
+  if (!f || D < 0 || !csv_path) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt_cb_csv: invalid model or path");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out, true, csv_path, sig_figs);
 }
 
 int sx_hmc_nuts_diag_e(sx_glm* m, const sx_nuts_config* cfg, const double* init,

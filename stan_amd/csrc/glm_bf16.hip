@@ -97,7 +97,16 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lg = lane >> 4;
   const int Kp = a.Kp;
-  const int cb = blockIdx.y * 64 + 16 * w;  // first chain of this wave
+  // 1-D grid, XCD-aware: blocks are dealt to the 8 XCDs round-robin, so the
+  // nCB chain-blocks of one row split are given ids 8 apart: they run on
+  // the same XCD at about the same time and read the split's X tiles
+  // through one L2.
+  const int nCB = a.Cp / 64;
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7, widx = bid >> 3;
+  const int spl = (widx / nCB) * 8 + xcd;
+  const int cblk = widx % nCB;
+  const int cb = cblk * 64 + 16 * w;  // first chain of this wave
   // forward A operands: B^T rows (chains cb + lr), k = 32 s + 8 lg + j
   bf16x8 bh[KS], bl[KS];
 #pragma unroll
@@ -114,7 +123,7 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
   for (int i = 0; i < 2 * KS; ++i) g[i] = f32x4{0.f, 0.f, 0.f, 0.f};
   double lp[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0}, bad[4] = {0, 0, 0, 0};
 
-  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * a.tiles_per_split;
+  const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t n0 = t * 32;
@@ -156,7 +165,8 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
         rs[w][4 * lg + j][16 * rb + lr] = r;
       }
     }
-    __syncthreads();
+    // rs[w] belongs to this wave: LDS accesses of a wave complete in order
+    __builtin_amdgcn_wave_barrier();
     // adjoint A operand: R^T[c = lr][n = 8 lg + j] as hi/lo
     bf16x8 rh, rl;
 #pragma unroll
@@ -174,10 +184,10 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
       const bf16x8 xl = *reinterpret_cast<const bf16x8*>(a.X2l + off);
       g[kb] = mma3(rh, rl, xh, xl, g[kb]);
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
   }
   // g[kb][j] = G^T[cb + 4 lg + j][16 kb + lr]
-  const int64_t pbase = static_cast<int64_t>(blockIdx.x) * a.Cp;
+  const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll
   for (int kb = 0; kb < 2 * KS; ++kb)
 #pragma unroll
@@ -204,32 +214,59 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
   }
 }
 
-// Per chain: FP64 sums of the block partials, then priors and output.
+// Per chain: FP64 sums of the split partials (four fixed interleaved slices
+// of splits, eight loads in flight per thread, combined in order), then
+// priors and output.  Entries: gradient partials k < Kp, then lp, sum r, bad.
 template <int FAM>
-__global__ __launch_bounds__(256) void mx_reduce(mx_args a, pass_args pa) {
+__global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   extern __shared__ double st[];
+  __shared__ double red[4][256];
   const int c = blockIdx.x;
   pa.theta += static_cast<int64_t>(c) * pa.theta_stride;
   pa.out += static_cast<int64_t>(c) * pa.out_stride;
   const int K = pa.K;
-  for (int k = threadIdx.x; k < K; k += blockDim.x) {
-    double v = 0.0;
-    for (int s = 0; s < a.splits; ++s)
-      v += static_cast<double>(a.gpart[(static_cast<int64_t>(s) * a.Cp + c) * a.Kp + k]);
-    st[2 + k] = v;
+  const int sl = threadIdx.x >> 8;
+  const int nE = K + 3;
+  for (int e0 = 0; e0 < nE; e0 += 256) {
+    const int e = e0 + (threadIdx.x & 255);
+    if (e < nE) {
+      // entry e: gradient k = e (e < K), or spart component e - K
+      const bool g = e < K;
+      const float* gp = a.gpart + static_cast<int64_t>(c) * a.Kp + e;
+      const double* sp = a.spart + static_cast<int64_t>(c) * 3 + (g ? 0 : e - K);
+      const int64_t gstride = static_cast<int64_t>(a.Cp) * a.Kp, sstride = 3 * static_cast<int64_t>(a.Cp);
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int sidx = sl;
+      for (; sidx + 28 < a.splits; sidx += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t si = sidx + 4 * u;
+          v[u] += g ? static_cast<double>(gp[si * gstride]) : sp[si * sstride];
+        }
+      }
+      for (int u = 0; sidx < a.splits; sidx += 4, ++u) {
+        const int64_t si = sidx;
+        v[u] += g ? static_cast<double>(gp[si * gstride]) : sp[si * sstride];
+      }
+      red[sl][threadIdx.x & 255] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    }
+    __syncthreads();
+    if (threadIdx.x < 256 && e0 + static_cast<int>(threadIdx.x) < nE) {
+      const int ee = e0 + threadIdx.x;
+      const double t = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+      // finalize layout: st[0] = lp, st[1] = sum r, st[2 + k] = grad k, st[K + 2] = bad
+      st[ee < K ? 2 + ee : (ee == K ? 0 : (ee == K + 1 ? 1 : K + 2))] = t;
+    }
+    __syncthreads();
   }
-  if (threadIdx.x < 3) {
-    double v = 0.0;
-    for (int s = 0; s < a.splits; ++s) v += a.spart[(static_cast<int64_t>(s) * a.Cp + c) * 3 + threadIdx.x];
-    st[threadIdx.x == 2 ? K + 2 : threadIdx.x] = v;
-  }
-  __syncthreads();
   if (threadIdx.x < 64) finalize<FAM>(pa, st);
 }
 
 template <int KS>
 hipError_t launch_ks(int family, const mx_args& a, hipStream_t s) {
-  dim3 grid(a.splits, a.Cp / 64);
+  // splits is a multiple of 8 (the XCD-aware block mapping relies on it)
+  if (a.splits % 8 != 0) return hipErrorInvalidValue;
+  dim3 grid(a.splits * (a.Cp / 64));
   if (family == SX_LOGIT)
     hipLaunchKernelGGL((glm_mfma<KS, SX_LOGIT>), grid, dim3(256), 0, s, a);
   else
@@ -273,9 +310,9 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
   if (e != hipSuccess) return e;
   const size_t lds = static_cast<size_t>(pa.E) * sizeof(double);
   if (family == SX_LOGIT)
-    hipLaunchKernelGGL((mx_reduce<SX_LOGIT>), dim3(C), dim3(256), lds, s, a, pa);
+    hipLaunchKernelGGL((mx_reduce<SX_LOGIT>), dim3(C), dim3(1024), lds, s, a, pa);
   else
-    hipLaunchKernelGGL((mx_reduce<SX_NORMAL>), dim3(C), dim3(256), lds, s, a, pa);
+    hipLaunchKernelGGL((mx_reduce<SX_NORMAL>), dim3(C), dim3(1024), lds, s, a, pa);
   return hipGetLastError();
 }
 

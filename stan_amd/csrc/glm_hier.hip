@@ -101,12 +101,22 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
   }
 }
 
+// Bernoulli-logit row term and residual in FP32 for sign s = 2y - 1; s = 0
+// (a padding row) gives term = r = 0.
+__device__ __forceinline__ void logit_row_f32s(float e, float s, float* term, float* r) {
+  const float se = s * e;
+  const float t = __expf(-fabsf(se));
+  const float inv = 1.0f / (1.0f + t);
+  *term = s == 0.f ? 0.f : -(fmaxf(-se, 0.0f) + log1pf(t));
+  *r = s * (se >= 0.0f ? t * inv : inv);
+}
+
 // FP32, many chains: a wave per chunk, chains on lanes; the chunk's rows
 // are staged 64 at a time in the wave's LDS and read as broadcasts.
 template <int KMAX>
 __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
-  __shared__ float sx[4][64 * KMAX];
-  __shared__ float sy[4][64];
+  __shared__ float sx[4][65 * KMAX];
+  __shared__ float sy[4][65];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int K = a.K;
@@ -143,24 +153,37 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
     if (lane < cnt) sy[w][lane] = a.y[r0 + p0 + lane] ? 1.0f : -1.0f;
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    for (int i = 0; i < cnt; ++i) {
-      const float* x = &sx[w][i * K];
-      float e = b;
+    // two rows per iteration (independent dependency chains); an odd last
+    // row pairs with a zero row of sign 0, which adds nothing
+    if (cnt & 1) {
+      for (int k = lane; k < K; k += 64) sx[w][cnt * K + k] = 0.f;
+      if (lane == 0) sy[w][cnt] = 0.f;
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = 0; i < cnt; i += 2) {
+      const float* x0 = &sx[w][i * K];
+      const float* x1 = x0 + K;
+      float e0 = b, e1 = b;
 #pragma unroll
       for (int k = 0; k < KMAX; ++k)
-        if (k < K) e = fmaf(x[k], wk[k], e);
-      const float s = sy[w][i];
-      const float se = s * e;
-      const float t = __expf(-fabsf(se));
-      const float inv = 1.0f / (1.0f + t);
-      const float term = -(fmaxf(-se, 0.0f) + log1pf(t));
-      const float r = s * (se >= 0.0f ? t * inv : inv);
-      if (!isfinite(e)) bad += 1.0;
-      s0 += term;
-      s1 += r;
+        if (k < K) {
+          e0 = fmaf(x0[k], wk[k], e0);
+          e1 = fmaf(x1[k], wk[k], e1);
+        }
+      const float sg0 = sy[w][i], sg1 = sy[w][i + 1];
+      float term0, r0, term1, r1;
+      logit_row_f32s(e0, sg0, &term0, &r0);
+      logit_row_f32s(e1, sg1, &term1, &r1);
+      if (!isfinite(e0)) bad += 1.0;
+      if (sg1 != 0.f && !isfinite(e1)) bad += 1.0;
+      s0 += term0;
+      s0 += term1;
+      s1 += r0;
+      s1 += r1;
 #pragma unroll
       for (int k = 0; k < KMAX; ++k)
-        if (k < K) acc[k] = fmaf(x[k], r, acc[k]);
+        if (k < K) acc[k] = fmaf(x1[k], r1, fmaf(x0[k], r0, acc[k]));
     }
   }
   if (!live) return;
@@ -203,31 +226,40 @@ __global__ __launch_bounds__(64) void hier_groups(hier_args a) {
   }
 }
 
-// Per chain: sums over groups (four fixed interleaved slices, combined in
-// order), priors, Jacobian, output.
-__global__ __launch_bounds__(512) void hier_final(hier_args a) {
-  __shared__ double red[4][128];
+// Per chain: sums over groups (sixteen fixed interleaved slices of groups,
+// each summed with four accumulators, then combined in order), priors,
+// Jacobian, output.  Every load of a slice is independent, so the sums are
+// bandwidth- rather than latency-bound.
+__global__ __launch_bounds__(1024) void hier_final(hier_args a) {
+  __shared__ double red[16][64];
   __shared__ double st[128];
   const int c = blockIdx.x;
   const int K = a.K;
   const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
   double* out = a.out + static_cast<int64_t>(c) * a.out_stride;
-  const int e = threadIdx.x & 127;
-  const int sl = threadIdx.x >> 7;
-  if (e < a.G) {
-    const double* gp = a.gpart + static_cast<int64_t>(c) * a.J * a.G + e;
-    double v0 = 0.0, v1 = 0.0;
-    int g = sl;
-    for (; g + 4 < a.J; g += 8) {
-      v0 += gp[static_cast<int64_t>(g) * a.G];
-      v1 += gp[static_cast<int64_t>(g + 4) * a.G];
+  const int sl = threadIdx.x >> 6;
+  for (int e0 = 0; e0 < a.G; e0 += 64) {
+    const int e = e0 + (threadIdx.x & 63);
+    if (e < a.G) {
+      const double* gp = a.gpart + static_cast<int64_t>(c) * a.J * a.G + e;
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      int g = sl;
+      for (; g + 48 < a.J; g += 64) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] += gp[static_cast<int64_t>(g + 16 * u) * a.G];
+      }
+      for (int u = 0; g < a.J; g += 16, ++u) v[u] += gp[static_cast<int64_t>(g) * a.G];
+      red[sl][threadIdx.x & 63] = (v[0] + v[1]) + (v[2] + v[3]);
     }
-    if (g < a.J) v0 += gp[static_cast<int64_t>(g) * a.G];
-    red[sl][e] = v0 + v1;
+    __syncthreads();
+    if (threadIdx.x < 64 && e0 + static_cast<int>(threadIdx.x) < a.G) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) t += red[i][threadIdx.x];
+      st[e0 + threadIdx.x] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x < a.G) st[threadIdx.x] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-  __syncthreads();
   if (threadIdx.x != 0) return;
   double lp = st[2 * K + 2];
   double bad = st[2 * K + 3];
@@ -308,7 +340,7 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s) {
   hipLaunchKernelGGL(hier_groups, dim3(a.J, a.chains), dim3(64), 0, s, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(hier_final, dim3(a.chains), dim3(512), 0, s, a);
+  hipLaunchKernelGGL(hier_final, dim3(a.chains), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
 

@@ -86,9 +86,8 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 // Sum over the chain's block (fixed order: waves, then wave 0..nw-1), the
 // result in every thread.  Called in block-uniform control flow.
-__device__ double block_sum(const struct ctx& k, double x);
 
-__device__ double block_sum(const ctx& k, double x) {
+__device__ __forceinline__ double block_sum(const ctx& k, double x) {
   x = wave_sum(x);
   if (k.nt == 64) return x;
   const int w = k.lane >> 6;
@@ -102,39 +101,77 @@ __device__ double block_sum(const ctx& k, double x) {
 }
 
 // ---- vector helpers (threads stride over D) ----
-__device__ void vcopy(const ctx& k, int dst, int src) {
+// Elementwise pass over [0, D): each thread takes its indices kU at a time,
+// gathers the inputs of all kU (ld) before it writes any result (st), so
+// kU loads are in flight together instead of one load per round trip.
+constexpr int kU = 4;
+template <class T, class Ld, class St>
+__device__ __forceinline__ void vpass(const ctx& k, Ld ld, St st) {
+  const int D = k.a->D, nt = k.nt;
+  for (int i0 = k.lane; i0 < D; i0 += kU * nt) {
+    T v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * nt;
+      if (i < D) v[u] = ld(i);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * nt;
+      if (i < D) st(i, v[u]);
+    }
+  }
+}
+// Thread-local sum of term(i) over the thread's indices, in index order
+// (the loads of kU indices issued together).
+template <class Ld>
+__device__ __forceinline__ double vsum(const ctx& k, Ld term) {
+  const int D = k.a->D, nt = k.nt;
+  double acc = 0.0;
+  for (int i0 = k.lane; i0 < D; i0 += kU * nt) {
+    double t[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * nt;
+      t[u] = i < D ? term(i) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (i0 + u * nt < D) acc += t[u];
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void vcopy(const ctx& k, int dst, int src) {
   double* d = k.slot(dst);
   const double* s = k.slot(src);
-  for (int i = k.lane; i < k.a->D; i += k.nt) d[i] = s[i];
+  vpass<double>(k, [&](int i) { return s[i]; }, [&](int i, double v) { d[i] = v; });
 }
-__device__ void vzero(const ctx& k, int dst) {
+__device__ __forceinline__ void vzero(const ctx& k, int dst) {
   double* d = k.slot(dst);
   for (int i = k.lane; i < k.a->D; i += k.nt) d[i] = 0.0;
 }
 // sum_i p_sharp_i * rho_i with p_sharp = minv * p
-__device__ double dot_sharp(const ctx& k, int p, int rho) {
+__device__ __forceinline__ double dot_sharp(const ctx& k, int p, int rho) {
   const double* m = k.slot(MINV);
   const double* pp = k.slot(p);
   const double* r = k.slot(rho);
-  double acc = 0.0;
-  for (int i = k.lane; i < k.a->D; i += k.nt) acc += (m[i] * pp[i]) * r[i];
+  const double acc = vsum(k, [&](int i) { return (m[i] * pp[i]) * r[i]; });
   return block_sum(k, acc);
 }
 // sum_i p_sharp_i * (r1_i + r2_i)
-__device__ double dot_sharp2(const ctx& k, int p, int r1, int r2) {
+__device__ __forceinline__ double dot_sharp2(const ctx& k, int p, int r1, int r2) {
   const double* m = k.slot(MINV);
   const double* pp = k.slot(p);
   const double* a = k.slot(r1);
   const double* b = k.slot(r2);
-  double acc = 0.0;
-  for (int i = k.lane; i < k.a->D; i += k.nt) acc += (m[i] * pp[i]) * (a[i] + b[i]);
+  const double acc = vsum(k, [&](int i) { return (m[i] * pp[i]) * (a[i] + b[i]); });
   return block_sum(k, acc);
 }
-__device__ double kinetic(const ctx& k, int p) {
+__device__ __forceinline__ double kinetic(const ctx& k, int p) {
   const double* m = k.slot(MINV);
   const double* pp = k.slot(p);
-  double acc = 0.0;
-  for (int i = k.lane; i < k.a->D; i += k.nt) acc += pp[i] * (m[i] * pp[i]);
+  const double acc = vsum(k, [&](int i) { return pp[i] * (m[i] * pp[i]); });
   return 0.5 * block_sum(k, acc);
 }
 
@@ -201,7 +238,7 @@ __device__ void copy_point(const ctx& k, int dq, int sq, double& dV, double sV) 
 __device__ void request_gradient(const ctx& k, int next_phase) {
   double* th = k.a->theta + static_cast<int64_t>(k.c) * k.a->D;
   const double* q = k.slot(ZQ);
-  for (int i = k.lane; i < k.a->D; i += k.nt) th[i] = q[i];
+  vpass<double>(k, [&](int i) { return q[i]; }, [&](int i, double v) { th[i] = v; });
   if (k.lane == 0) {
     k.s->phase = next_phase;
     k.s->pending = 1;
@@ -214,7 +251,7 @@ __device__ void consume_gradient(const ctx& k) {
   const double* o = k.a->out + static_cast<int64_t>(k.c) * k.a->out_stride;
   const bool ok = o[k.a->D + 1] == 0.0 && isfinite(o[0]);
   double* g = k.slot(ZG);
-  for (int i = k.lane; i < k.a->D; i += k.nt) g[i] = ok ? -o[1 + i] : -g[i];
+  vpass<double>(k, [&](int i) { return ok ? o[1 + i] : g[i]; }, [&](int i, double v) { g[i] = -v; });
   if (k.lane == 0) {
     k.s->V_z = ok ? -o[0] : kInf;
     k.s->last_lp = o[0];
@@ -229,16 +266,21 @@ __device__ void leapfrog_begin(const ctx& k, double eps) {
   const double* g = k.slot(ZG);
   const double* m = k.slot(MINV);
   const double he = 0.5 * eps;
-  for (int i = k.lane; i < k.a->D; i += k.nt) {
-    p[i] -= he * g[i];
-    q[i] += eps * (m[i] * p[i]);
-  }
+  struct pgqm { double p, g, q, m; };
+  vpass<pgqm>(k, [&](int i) { return pgqm{p[i], g[i], q[i], m[i]}; },
+              [&](int i, const pgqm& v) {
+                const double pn = v.p - he * v.g;
+                p[i] = pn;
+                q[i] = v.q + eps * (v.m * pn);
+              });
 }
 __device__ void leapfrog_end(const ctx& k, double eps) {
   double* p = k.slot(ZP);
   const double* g = k.slot(ZG);
   const double he = 0.5 * eps;
-  for (int i = k.lane; i < k.a->D; i += k.nt) p[i] -= he * g[i];
+  struct pg { double p, g; };
+  vpass<pg>(k, [&](int i) { return pg{p[i], g[i]}; },
+            [&](int i, const pg& v) { p[i] = v.p - he * v.g; });
 }
 
 __device__ dest init_dest(const dest& d, int l) {
@@ -289,11 +331,13 @@ __device__ bool leaf(const ctx& k, const dest& d) {
     double* r = k.slot(d.rho);
     double* pb = k.slot(d.pbeg);
     double* pe = k.slot(d.pend);
-    for (int i = k.lane; i < k.a->D; i += k.nt) {
-      r[i] += p[i];
-      pb[i] = p[i];
-      pe[i] = p[i];
-    }
+    struct rp { double r, p; };
+    vpass<rp>(k, [&](int i) { return rp{r[i], p[i]}; },
+              [&](int i, const rp& v) {
+                r[i] = v.r + v.p;
+                pb[i] = v.p;
+                pe[i] = v.p;
+              });
   }
   __syncthreads();
   return !s->divergent;
@@ -321,10 +365,13 @@ __device__ bool merge(const ctx& k, int l, const dest& d) {
     const double* rf = k.slot(RF(l));
     double* t = k.slot(TMP);
     double* r = k.slot(d.rho);
-    for (int i = k.lane; i < k.a->D; i += k.nt) {
-      t[i] = ri[i] + rf[i];
-      r[i] += t[i];
-    }
+    struct abr { double a, b, r; };
+    vpass<abr>(k, [&](int i) { return abr{ri[i], rf[i], r[i]}; },
+               [&](int i, const abr& v) {
+                 const double tv = v.a + v.b;
+                 t[i] = tv;
+                 r[i] = v.r + tv;
+               });
   }
   bool persist = dot_sharp(k, d.pend, TMP) > 0 && dot_sharp(k, d.pbeg, TMP) > 0;
   persist &= dot_sharp2(k, PFBG(l), RI(l), PFBG(l)) > 0 &&
@@ -360,11 +407,14 @@ __device__ int learn_variance(const ctx& k) {
     double* m = k.slot(WM);
     double* m2 = k.slot(WM2);
     const double* q = k.slot(ZQ);
-    for (int i = k.lane; i < a->D; i += k.nt) {
-      const double delta = q[i] - m[i];
-      m[i] += delta / n;
-      m2[i] += (q[i] - m[i]) * delta;
-    }
+    struct qmm { double q, m, m2; };
+    vpass<qmm>(k, [&](int i) { return qmm{q[i], m[i], m2[i]}; },
+               [&](int i, const qmm& v) {
+                 const double delta = v.q - v.m;
+                 const double mn = v.m + delta / n;
+                 m[i] = mn;
+                 m2[i] = v.m2 + (v.q - mn) * delta;
+               });
     __syncthreads();
     if (k.lane == 0) s->w_n = n;
   }
@@ -390,15 +440,17 @@ __device__ int learn_variance(const ctx& k) {
   double* m = k.slot(WM);
   double* m2 = k.slot(WM2);
   int finite = 1;
-  for (int i = k.lane; i < a->D; i += k.nt) {
-    double var = minv[i];
-    if (n > 1) var = m2[i] / (n - 1.0);
-    var = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0)) * 1.0;
-    if (!isfinite(var)) finite = 0;
-    minv[i] = var;
-    m[i] = 0.0;
-    m2[i] = 0.0;
-  }
+  struct vm { double minv, m2; };
+  vpass<vm>(k, [&](int i) { return vm{minv[i], m2[i]}; },
+            [&](int i, const vm& v) {
+              double var = v.minv;
+              if (n > 1) var = v.m2 / (n - 1.0);
+              var = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0)) * 1.0;
+              if (!isfinite(var)) finite = 0;
+              minv[i] = var;
+              m[i] = 0.0;
+              m2[i] = 0.0;
+            });
   const int all_finite = __syncthreads_and(finite);
   __syncthreads();
   if (k.lane == 0) {
@@ -425,7 +477,7 @@ __device__ void write_draw(const ctx& k) {
     row[6] = s->T_samp + s->V_samp;
   }
   const double* q = k.slot(SQ);
-  for (int i = k.lane; i < a->D; i += k.nt) row[7 + i] = q[i];
+  vpass<double>(k, [&](int i) { return q[i]; }, [&](int i, double v) { row[7 + i] = v; });
   __syncthreads();
   if (k.lane == 0) s->rows++;
 }
@@ -460,6 +512,7 @@ __device__ void begin_stepsize_heuristic(const ctx& k, int ret_phase) {
 
 __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   __shared__ chain_state s;
+  __shared__ batch_args sa;  // helpers read the arguments through ctx::a
   __shared__ double red[8];
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
@@ -469,8 +522,9 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
     int* dst = reinterpret_cast<int*>(&s);
     for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += nt) dst[i] = src[i];
   }
+  if (lane == 0) sa = a;
   __syncthreads();
-  ctx k{&a, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane, nt, red,
+  ctx k{&sa, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane, nt, red,
         a.chain0 + static_cast<uint32_t>(c)};
   if (s.phase == PH_DONE) return;
   if (s.pending) {
@@ -505,9 +559,8 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         break;
       }
       case PH_INIT_CHECK: {
-        double gs = 0.0;
         const double* g = k.slot(ZG);
-        for (int i = lane; i < a.D; i += k.nt) gs += g[i];
+        double gs = vsum(k, [&](int i) { return g[i]; });
         gs = block_sum(k, gs);
         const bool ok = s.last_ok && isfinite(s.last_lp) && isfinite(gs);
         __syncthreads();
@@ -726,7 +779,7 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
           const double* rb = k.slot(RHOB);
           const double* rf = k.slot(RHOF);
           double* r = k.slot(RHO);
-          for (int i = lane; i < a.D; i += k.nt) r[i] = rb[i] + rf[i];
+          vpass<double>(k, [&](int i) { return rb[i] + rf[i]; }, [&](int i, double v) { r[i] = v; });
         }
         __syncthreads();
         bool persist = dot_sharp(k, FP, RHO) > 0 && dot_sharp(k, BP, RHO) > 0;

@@ -1,9 +1,10 @@
 // Output and control callbacks of the sampler services, mirroring the
 // reference's interfaces (src/stan/callbacks/writer.hpp:16-61,
-// logger.hpp:24-97, interrupt.hpp:14-28).
+// logger.hpp:24-97, interrupt.hpp:14-28, stream_writer.hpp:14-93).
 #ifndef STANX_CALLBACKS_HPP
 #define STANX_CALLBACKS_HPP
 
+#include <ostream>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -39,6 +40,31 @@ class interrupt {
  public:
   virtual ~interrupt() {}
   virtual void operator()() {}
+};
+
+// Writes names and values as comma-separated lines and messages as lines
+// behind a comment prefix, with the stream's own number formatting: the
+// sample file of the reference's services (a Stan CSV file with "# ").
+class stream_writer : public writer {
+ public:
+  explicit stream_writer(std::ostream& output, const std::string& comment_prefix = "")
+      : output_(output), comment_prefix_(comment_prefix) {}
+  void operator()(const std::vector<std::string>& names) override { write_row(names); }
+  void operator()(const std::vector<double>& state) override { write_row(state); }
+  void operator()() override { output_ << comment_prefix_ << std::endl; }
+  void operator()(const std::string& message) override {
+    output_ << comment_prefix_ << message << std::endl;
+  }
+
+ private:
+  template <class T>
+  void write_row(const std::vector<T>& v) {
+    if (v.empty()) return;
+    for (size_t i = 0; i + 1 < v.size(); ++i) output_ << v[i] << ",";
+    output_ << v.back() << std::endl;
+  }
+  std::ostream& output_;
+  std::string comment_prefix_;
 };
 
 // Collects every numeric row and every message (tests and the C-ABI use it).

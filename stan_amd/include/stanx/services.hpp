@@ -147,11 +147,22 @@ class mcmc_writer {
     for (size_t i = 0; i < im.size(); ++i) m << (i ? ", " : "") << im[i];
     sample_writer_(m.str());
   }
+  // blank line, three aligned lines, blank line (mcmc_writer.hpp:203-238)
+  void write_timing(double warm, double samp, callbacks::writer& w) {
+    const std::string title(" Elapsed Time: ");
+    w();
+    std::stringstream s1, s2, s3;
+    s1 << title << warm << " seconds (Warm-up)";
+    w(s1.str());
+    s2 << std::string(title.size(), ' ') << samp << " seconds (Sampling)";
+    w(s2.str());
+    s3 << std::string(title.size(), ' ') << warm + samp << " seconds (Total)";
+    w(s3.str());
+    w();
+  }
   void write_timing(double warm, double samp) {
-    std::stringstream ss;
-    ss << "Elapsed Time: " << warm << " seconds (Warm-up) " << samp
-       << " seconds (Sampling)";
-    sample_writer_(ss.str());
+    write_timing(warm, samp, sample_writer_);
+    write_timing(warm, samp, diagnostic_writer_);
   }
 
  private:
