@@ -89,6 +89,8 @@ typedef struct sx_prior {
   double beta_sd;  /* beta  ~ normal(0, beta_sd)                     */
   double sigma_sd; /* sigma ~ half-normal(0, sigma_sd)  (SX_NORMAL)  */
   double tau_sd;   /* tau   ~ half-normal(0, tau_sd)    (SX_HIER_LOGIT) */
+  double sigma_mu; /* location of the sigma prior: sigma ~ normal(sigma_mu,
+                      sigma_sd) on sigma > 0 (0: the half-normal above)  */
 } sx_prior;
 
 /* y: const int32_t* with values in {0,1} for SX_LOGIT / SX_HIER_LOGIT,
@@ -123,6 +125,19 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y,
                      int64_t N, int64_t K, const int64_t* group_offsets,
                      int64_t J, int precision, const sx_prior* prior,
                      int device, const sx_tuning* tuning);
+/* A context from a Stan JSON data file, read as a generated model reads its
+ * var_context (src/stan/io/json/json_data.hpp:42-73): X (N x K matrix), y
+ * (or Y); N and K when present must agree; the hierarchical model takes a
+ * 1-based group index per row (group or ids) and J (else the largest
+ * index), and its rows are grouped by a stable sort.                      */
+int sx_glm_create_json(sx_glm** out, const char* path, int family,
+                       int precision, const sx_prior* prior, int device);
+/* One variable of a Stan JSON data file: values column-major (first index
+ * fastest, the var_context layout), up to cap of them; dims up to
+ * max_dims; host only.                                                    */
+int sx_json_var(const char* path, const char* name, double* vals,
+                int64_t cap, int64_t* dims, int max_dims, int* ndims,
+                int64_t* size);
 /* Text log of the autotuner's measurements and the chosen configuration. */
 const char* sx_glm_tuning_log(const sx_glm* m);
 void sx_glm_destroy(sx_glm* m);

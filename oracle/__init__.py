@@ -71,7 +71,8 @@ def _ptr(a):
 
 class Prior(C.Structure):
     _fields_ = [("alpha_sd", C.c_double), ("beta_sd", C.c_double),
-                ("sigma_sd", C.c_double), ("tau_sd", C.c_double)]
+                ("sigma_sd", C.c_double), ("tau_sd", C.c_double),
+                ("sigma_mu", C.c_double)]
 
 
 class GlmCtx(C.Structure):

@@ -201,12 +201,13 @@ static int normal_lpgrad(int mode, const double* X, const double* y,
   normal_prior(alpha, pr->alpha_sd, propto, &acc, &grad[0]);
   for (int64_t k = 0; k < K; ++k)
     normal_prior(beta[k], pr->beta_sd, propto, &acc, &grad[1 + k]);
-  /* sigma ~ normal(0, sigma_sd) on sigma = exp(u) */
+  /* sigma ~ normal(sigma_mu, sigma_sd) on sigma = exp(u):
+     d/du = -(sigma - mu) sigma / sd^2 */
   if (pr->sigma_sd > 0) {
-    double zz = sigma / pr->sigma_sd;
+    double zz = (sigma - pr->sigma_mu) / pr->sigma_sd;
     dd_add(&acc, -0.5 * zz * zz);
     if (!propto) dd_add(&acc, -log(pr->sigma_sd) - HALF_LOG_2PI);
-    grad[K + 1] += -zz * zz;
+    grad[K + 1] += -zz * sigma / pr->sigma_sd;
   }
   if (jacobian) {
     dd_add(&acc, u);

@@ -68,6 +68,7 @@ typedef struct orc_prior {
   double beta_sd;  /* beta  ~ normal(0, beta_sd) */
   double sigma_sd; /* sigma ~ half-normal(0, sigma_sd)   (NORMAL) */
   double tau_sd;   /* tau   ~ half-normal(0, tau_sd)     (HIER)   */
+  double sigma_mu; /* sigma ~ normal(sigma_mu, sigma_sd) on sigma > 0 */
 } orc_prior;
 
 /* mode 0: Stan-equivalent two-pass double evaluation (eta = X beta + alpha,

@@ -27,7 +27,7 @@ EXPORTED = [
     "sx_nuts_batch_create", "sx_server_create", "sx_server_footprint", "sx_server_submit",
     "sx_server_info", "sx_server_wait", "sx_server_results", "sx_server_release",
     "sx_server_destroy", "sx_analyze_ess", "sx_hmc_nuts_diag_e_adapt_csv",
-    "sx_hmc_nuts_diag_e_adapt_cb_csv", "sx_nuts_batch_run",
+    "sx_hmc_nuts_diag_e_adapt_cb_csv", "sx_glm_create_json", "sx_json_var", "sx_nuts_batch_run",
     "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
 ]
 
@@ -46,7 +46,8 @@ class DomainError(StanxError, ValueError):
 
 class Prior(C.Structure):
     _fields_ = [("alpha_sd", C.c_double), ("beta_sd", C.c_double),
-                ("sigma_sd", C.c_double), ("tau_sd", C.c_double)]
+                ("sigma_sd", C.c_double), ("tau_sd", C.c_double),
+                ("sigma_mu", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -130,6 +131,10 @@ def lib():
         L.sx_hmc_nuts_diag_e_adapt_cb_csv.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                                       vp, vp, C.c_char_p, C.c_int,
                                                       C.POINTER(NutsResult)]
+        L.sx_glm_create_json.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int, C.c_int,
+                                         C.POINTER(Prior), C.c_int]
+        L.sx_json_var.argtypes = [C.c_char_p, C.c_char_p, vp, i64, C.POINTER(i64), C.c_int,
+                                  C.POINTER(C.c_int), C.POINTER(i64)]
         L.sx_analyze_ess.argtypes = [vp, i64, i64, C.c_int, dp]
         L.sx_server_create.argtypes = [C.POINTER(vp), C.c_int, i64, C.c_int]
         L.sx_server_footprint.argtypes = [C.POINTER(JobSpec), C.POINTER(i64)]
@@ -507,3 +512,33 @@ def native_ess(chains, kind="bulk"):
     _check(lib().sx_analyze_ess(_ptr(x), x.shape[1], x.shape[0], 0 if kind == "ess" else 1,
                                 C.byref(out)))
     return out.value
+
+
+def json_var(path, name):
+    """A variable of a Stan JSON data file as read by the library (numpy
+    array of the file's shape)."""
+    size, nd = C.c_int64(), C.c_int()
+    dims = (C.c_int64 * 8)()
+    _check(lib().sx_json_var(str(path).encode(), name.encode(), None, 0, dims, 8, C.byref(nd),
+                             C.byref(size)))
+    vals = np.zeros(size.value)
+    _check(lib().sx_json_var(str(path).encode(), name.encode(), _ptr(vals), size.value, dims, 8,
+                             C.byref(nd), C.byref(size)))
+    shape = tuple(dims[i] for i in range(nd.value))
+    return vals.reshape(shape, order="F") if shape else vals[0]
+
+
+def glm_from_json(path, family, precision=FP64, prior=(5.0, 2.0, 2.0, 1.0), device=0):
+    """A GlmModel whose data come from a Stan JSON data file (X, y or Y;
+    a group index for HIER_LOGIT)."""
+    m = GlmModel.__new__(GlmModel)
+    m._h = C.c_void_p()
+    m.prior = Prior(*prior)
+    _check(lib().sx_glm_create_json(C.byref(m._h), str(path).encode(), family, precision,
+                                    C.byref(m.prior), device))
+    D = C.c_int64()
+    _check(lib().sx_glm_num_params(m._h, C.byref(D)))
+    N, K, J = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(lib().sx_glm_dims(m._h, C.byref(N), C.byref(K), C.byref(J)))
+    m.family, m.N, m.K, m.J, m.D = family, N.value, K.value, J.value, D.value
+    return m

@@ -275,6 +275,7 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
     g.alpha_sd = m->prior.alpha_sd;
     g.beta_sd = m->prior.beta_sd;
     g.sigma_sd = m->prior.sigma_sd;
+    g.sigma_mu = m->prior.sigma_mu;
     const double h = 0.91893853320467274178032973640562;
     g.alpha_const = g.alpha_sd > 0 ? -std::log(g.alpha_sd) - h : 0.0;
     g.beta_const = g.beta_sd > 0 ? -std::log(g.beta_sd) - h : 0.0;

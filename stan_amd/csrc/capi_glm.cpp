@@ -455,6 +455,7 @@ int enqueue_pass(sx_glm* m, int propto, int jacobian, bool reduce) {
   a.alpha_sd = m->prior.alpha_sd;
   a.beta_sd = m->prior.beta_sd;
   a.sigma_sd = m->prior.sigma_sd;
+  a.sigma_mu = m->prior.sigma_mu;
   const double half_log_2pi = 0.91893853320467274178032973640562;
   a.alpha_const = a.alpha_sd > 0 ? -std::log(a.alpha_sd) - half_log_2pi : 0.0;
   a.beta_const = a.beta_sd > 0 ? -std::log(a.beta_sd) - half_log_2pi : 0.0;
@@ -576,6 +577,7 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   pa.alpha_sd = m->prior.alpha_sd;
   pa.beta_sd = m->prior.beta_sd;
   pa.sigma_sd = m->prior.sigma_sd;
+  pa.sigma_mu = m->prior.sigma_mu;
   const double h = 0.91893853320467274178032973640562;
   pa.alpha_const = pa.alpha_sd > 0 ? -std::log(pa.alpha_sd) - h : 0.0;
   pa.beta_const = pa.beta_sd > 0 ? -std::log(pa.beta_sd) - h : 0.0;

@@ -63,10 +63,11 @@ __device__ void finalize(const pass_args& a, const double* st) {
     if (!a.propto) lp += -static_cast<double>(a.N) * kHalfLog2Pi;
     double gu = -static_cast<double>(a.N) + SS * scale;
     if (a.sigma_sd > 0) {
-      const double z = sigma / a.sigma_sd;
+      // sigma ~ normal(sigma_mu, sigma_sd): d/du = -(sigma - mu) sigma / sd^2
+      const double z = (sigma - a.sigma_mu) / a.sigma_sd;
       lp += -0.5 * z * z;
       if (!a.propto) lp += a.sigma_const;
-      gu += -z * z;
+      gu += -z * (sigma / a.sigma_sd);
     }
     if (a.jacobian) {
       lp += u;

@@ -53,7 +53,7 @@ struct pass_args {
   int chains = 1;              // batched evaluation: chains on grid.y
   int64_t theta_stride = 0;    // doubles between chains' theta
   int64_t out_stride = 0;      // doubles between chains' outputs (>= D+2)
-  double alpha_sd, beta_sd, sigma_sd;
+  double alpha_sd, beta_sd, sigma_sd, sigma_mu;
   // normalising constants -log(sd) - log(2 pi)/2 of the normal priors
   double alpha_const, beta_const, sigma_const;
 };
