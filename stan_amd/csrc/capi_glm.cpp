@@ -519,12 +519,19 @@ void free_workspace(hier_workspace& w) {
   w = hier_workspace();
 }
 
+// chains padded to the chain tile of the MFMA kernel: 128 for K <= 128
+// (32x32 tiles, four waves of 32 chains), else 64
+int mx_chain_pad(const sx_glm* m, int C) {
+  const int tile = m->Kp <= 128 ? 128 : 64;
+  return (C + tile - 1) / tile * tile;
+}
+
 int mx_prepare(sx_glm* m, mx_workspace& w, int C) {
   if (!m->d_X1h) {
     set_error("batched BF16 evaluation needs a context created with SX_BF16X2");
     return SX_EINVAL;
   }
-  const int Cp = (C + 63) / 64 * 64;
+  const int Cp = mx_chain_pad(m, C);
   const int64_t n_tiles = m->Np / 32;
   int splits = static_cast<int>(std::min<int64_t>(512, std::max<int64_t>(8, ((n_tiles + 7) / 8 + 7) / 8 * 8)));
   if (w.cap >= Cp && w.splits == splits) return SX_OK;
@@ -559,7 +566,7 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   a.N = m->N;
   a.n_tiles = m->Np / 32;
   a.Kp = static_cast<int>(m->Kp);
-  a.Cp = (C + 63) / 64 * 64;
+  a.Cp = mx_chain_pad(m, C);
   a.splits = w.splits;
   a.tiles_per_split = (a.n_tiles + a.splits - 1) / a.splits;
   pass_args pa{};

@@ -103,7 +103,7 @@ int64_t footprint_of(const sx_job_spec& s) {
     bytes += 2048 * (K + 3) * 8;                                  // context partials
     if (prec == SX_BF16X2) {
       const int64_t Np = ceil_div(N, 32) * 32, Kp = std::max<int64_t>(32, ceil_div(K, 32) * 32);
-      const int64_t Cp = ceil_div(C, 64) * 64;
+      const int64_t Cp = ceil_div(C, 128) * 128;
       bytes += 4 * Np * Kp * 2;                                   // X1/X2 hi/lo
       bytes += Cp * Kp * 4 + Cp * 4 + Cp * Kp * 4 * 512 + 3 * Cp * 512 * 8;  // workspace
     } else {

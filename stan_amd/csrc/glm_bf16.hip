@@ -214,6 +214,208 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
   }
 }
 
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x16 mma3_32(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x16 c) {
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+// K <= 128 (KQ = Kp / 32 <= 4): workgroup = 4 waves x 32 chains = 128
+// chains.  Each 32-row tile of X (both layouts, hi and lo) is staged once
+// per workgroup in LDS (double-buffered, the next tile's global loads in
+// flight during the current tile's MFMAs) and read by all four waves, so X
+// crosses L2 once per 128 chains.  v_mfma_f32_32x32x16_bf16 operands (lane
+// l, j < 8): A[l % 32][8 (l / 32) + j], B[8 (l / 32) + j][l % 32]; result
+// d[j] = D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
+//   forward  H (32 rows x 32 chains) = X1 tile (A, from LDS) . B (registers)
+//            lane l holds chain l % 32 and rows 8 (j/4) + 4 (l/32) + j%4
+//   adjoint  G^T (32 chains x 32 k) += R^T . X: the contraction runs over
+//            rows in the order the forward left them in the lane (step t
+//            uses the lane's own r[8t .. 8t+7]), and the X operand takes the
+//            same rows from the 8-row-packed tile: rows 16t + 4 (l/32) + 0..3
+//            of groups 2t and 2t+1 (two 8-byte LDS reads).
+template <int KQ, int FAM>
+__global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
+  constexpr int Kp = 32 * KQ;
+  constexpr int S16 = 2 * KQ;          // forward k-steps of 16
+  constexpr int X1S = Kp + 8;          // padded LDS row stride of the X1 tile
+  constexpr int X1E = 32 * X1S;        // elements of one X1 tile array
+  constexpr int X2E = 32 * Kp;         // elements of one X2 tile array
+  constexpr int CH = 16 * Kp;          // 16-byte chunks per tile (4 arrays)
+  constexpr int PER = CH / 256;        // chunks per thread
+  __shared__ __attribute__((aligned(16))) __bf16 x1h[2][X1E], x1l[2][X1E];
+  __shared__ __attribute__((aligned(16))) __bf16 x2h[2][X2E], x2l[2][X2E];
+  __shared__ float ys[2][32];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 31, h = lane >> 5;
+  // XCD-aware 1-D grid (see glm_mfma)
+  const int nCB = a.Cp / 128;
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7, widx = bid >> 3;
+  const int spl = (widx / nCB) * 8 + xcd;
+  const int cblk = widx % nCB;
+  const int cbase = cblk * 128 + 32 * w;  // first chain of this wave
+  const int chain = cbase + li;
+  bf16x8 bh[S16], bl[S16];
+#pragma unroll
+  for (int s = 0; s < S16; ++s) {
+    const int64_t off = static_cast<int64_t>(chain) * Kp + 16 * s + 8 * h;
+    bh[s] = *reinterpret_cast<const bf16x8*>(a.Bh + off);
+    bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
+  }
+  const float alpha = a.alpha[chain];
+  f32x16 g[KQ];
+#pragma unroll
+  for (int i = 0; i < KQ; ++i)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) g[i][j] = 0.f;
+  double lp = 0.0, sr = 0.0, bad = 0.0;
+
+  const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
+  const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
+  const __bf16* src[4] = {reinterpret_cast<const __bf16*>(a.X1h), reinterpret_cast<const __bf16*>(a.X1l),
+                          reinterpret_cast<const __bf16*>(a.X2h), reinterpret_cast<const __bf16*>(a.X2l)};
+  uint4 stage[PER];
+  float ystage = 0.f;
+  auto gload = [&](int64_t t) {
+    const int64_t base = t * 32 * Kp;  // both layouts: a tile is 32 Kp contiguous elements
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int c = tid + 256 * i;
+      const int arr = c / (4 * Kp), o = (c % (4 * Kp)) * 8;
+      stage[i] = *reinterpret_cast<const uint4*>(src[arr] + base + o);
+    }
+    if (tid < 32) {
+      const int64_t n = t * 32 + tid;
+      float v = 0.f;
+      if (n < a.N) {
+        if (FAM == SX_LOGIT)
+          v = static_cast<const int8_t*>(a.y)[n] ? 1.0f : -1.0f;
+        else
+          v = static_cast<float>(static_cast<const double*>(a.y)[n]);
+      }
+      ystage = v;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int c = tid + 256 * i;
+      const int arr = c / (4 * Kp), o = (c % (4 * Kp)) * 8;
+      if (arr < 2) {
+        const int row = o / Kp, col = o % Kp;
+        __bf16* d = (arr == 0 ? x1h[buf] : x1l[buf]) + row * X1S + col;
+        *reinterpret_cast<uint4*>(d) = stage[i];
+      } else {
+        __bf16* d = (arr == 2 ? x2h[buf] : x2l[buf]) + o;
+        *reinterpret_cast<uint4*>(d) = stage[i];
+      }
+    }
+    if (tid < 32) ys[buf][tid] = ystage;
+  };
+  if (t0 < t1) {
+    gload(t0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int64_t t = t0; t < t1; ++t) {
+    const int buf = static_cast<int>((t - t0) & 1);
+    if (t + 1 < t1) gload(t + 1);
+    // forward
+    f32x16 hacc;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) hacc[j] = 0.f;
+#pragma unroll
+    for (int s = 0; s < S16; ++s) {
+      const int off = li * X1S + 16 * s + 8 * h;
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1h[buf][off]);
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1l[buf][off]);
+      hacc = mma3_32(xh, xl, bh[s], bl[s], hacc);
+    }
+    // link: row of element j is 8 (j/4) + 4 h + j%4
+    float r[16];
+    const int64_t n0 = t * 32;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int row = 8 * (j >> 2) + 4 * h + (j & 3);
+      const bool valid = n0 + row < a.N;
+      const float yv = ys[buf][row];
+      const float eta = hacc[j] + alpha;
+      float term = 0.f, rr = 0.f;
+      if (valid) {
+        if (FAM == SX_LOGIT) {
+          logit_row_f32(eta, yv, &term, &rr);
+        } else {
+          rr = yv - eta;
+          term = rr * rr;
+        }
+        if (!isfinite(eta)) bad += 1.0;
+      }
+      lp += term;
+      sr += rr;
+      r[j] = rr;
+    }
+    // adjoint
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      bf16x8 rh, rl;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        __bf16 hh, ll;
+        split(r[8 * st + j], &hh, &ll);
+        rh[j] = hh;
+        rl[j] = ll;
+      }
+#pragma unroll
+      for (int kb = 0; kb < KQ; ++kb) {
+        const int k = 32 * kb + li;
+        const int o0 = ((2 * st) * Kp + k) * 8 + 4 * h;
+        const int o1 = ((2 * st + 1) * Kp + k) * 8 + 4 * h;
+        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&x2h[buf][o0]);
+        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2h[buf][o1]);
+        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2l[buf][o0]);
+        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2l[buf][o1]);
+        bf16x8 xh, xl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          xh[j] = ah0[j];
+          xh[4 + j] = ah1[j];
+          xl[j] = al0[j];
+          xl[4 + j] = al1[j];
+        }
+        g[kb] = mma3_32(rh, rl, xh, xl, g[kb]);
+      }
+    }
+    if (t + 1 < t1) lstore(buf ^ 1);
+    __syncthreads();
+  }
+  // g[kb][j] = G^T[chain cbase + 8 (j/4) + 4 h + j%4][k = 32 kb + li]
+  const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
+#pragma unroll
+  for (int kb = 0; kb < KQ; ++kb)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int ci = cbase + 8 * (j >> 2) + 4 * h + (j & 3);
+      a.gpart[(pbase + ci) * Kp + 32 * kb + li] = g[kb][j];
+    }
+  // row terms of this lane's chain: combine the two halves of the rows
+  lp += __shfl_xor(lp, 32, 64);
+  sr += __shfl_xor(sr, 32, 64);
+  bad += __shfl_xor(bad, 32, 64);
+  if (h == 0) {
+    double* sp = a.spart + (pbase + chain) * 3;
+    sp[0] = lp;
+    sp[1] = sr;
+    sp[2] = bad;
+  }
+}
+
 // Per chain: FP64 sums of the split partials (four fixed interleaved slices
 // of splits, eight loads in flight per thread, combined in order), then
 // priors and output.  Entries: gradient partials k < Kp, then lp, sum r, bad.
@@ -262,6 +464,17 @@ __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   if (threadIdx.x < 64) finalize<FAM>(pa, st);
 }
 
+template <int KQ>
+hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
+  if (a.splits % 8 != 0 || a.Cp % 128 != 0) return hipErrorInvalidValue;
+  dim3 grid(a.splits * (a.Cp / 128));
+  if (family == SX_LOGIT)
+    hipLaunchKernelGGL((glm_mfma32<KQ, SX_LOGIT>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((glm_mfma32<KQ, SX_NORMAL>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int KS>
 hipError_t launch_ks(int family, const mx_args& a, hipStream_t s) {
   // splits is a multiple of 8 (the XCD-aware block mapping relies on it)
@@ -296,7 +509,15 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
                      const_cast<float*>(a.alpha));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  switch (a.Kp / 32) {
+  if (a.Kp <= 128 && a.Cp % 128 == 0) {
+    switch (a.Kp / 32) {
+      case 1: e = launch_kq32<1>(family, a, s); break;
+      case 2: e = launch_kq32<2>(family, a, s); break;
+      case 3: e = launch_kq32<3>(family, a, s); break;
+      case 4: e = launch_kq32<4>(family, a, s); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else switch (a.Kp / 32) {
     case 1: e = launch_ks<1>(family, a, s); break;
     case 2: e = launch_ks<2>(family, a, s); break;
     case 3: e = launch_ks<3>(family, a, s); break;
