@@ -48,11 +48,11 @@ def cpu_baseline(d, theta, budget_s=12.0):
                        f"(oracle two-pass FP64, one thread), {el:.1f} s")
 
 
-def nuts_run(model, warmup, samples, seed=1234):
+def nuts_run(model, warmup, samples, seed=1234, **kw):
     import stan_amd as S
     from stan_amd import analyze
     out = S.hmc_nuts_diag_e_adapt(model, random_seed=seed, chain=1, num_warmup=warmup,
-                                  num_samples=samples)
+                                  num_samples=samples, **kw)
     if out["rc"] != 0:
         raise RuntimeError(out["error"])
     q = out["draws"][:, 7:]
@@ -190,7 +190,7 @@ def mixed_error(model, ref_data, family, thetas):
     return e_lp, e_g
 
 
-def hier_section(warm, samp, chains=64):
+def hier_section(warm, samp, chains=64, max_depth=8):
     """Hierarchical logistic model, J=1000 ragged groups, K=20: batched FP32
     evaluation and device NUTS over 64 chains, against the single-chain FP64
     service on the same context."""
@@ -205,10 +205,10 @@ def hier_section(warm, samp, chains=64):
     th = W.random_thetas(mh.D, chains, seed=9) * 0.5
     ms_b, _ = mh.time_log_prob_grad(th, iters=20, warmup=3, chains=chains)
     ms_1, _ = mh.time_log_prob_grad(th[0], iters=20, warmup=3)
-    out = batched_run(mh, chains, warm, samp, S.FP32)
+    out = batched_run(mh, chains, warm, samp, S.FP32, max_depth=max_depth)
     # X (FP32) + y (int8) + chunk tables; theta/out of every chain
     alg_bytes = 4.0 * N * K + N + 16.0 * (N / 256 + J) + 8.0 * chains * (2 * mh.D + 2)
-    out.update(N=N, K=K, J=J, D=mh.D, ms_per_batched_eval=ms_b,
+    out.update(N=N, K=K, J=J, D=mh.D, max_depth=max_depth, ms_per_batched_eval=ms_b,
                batched_eval_gbs=alg_bytes / (ms_b * 1e-3) / 1e9,
                batched_eval_tflops_fp32=(4.0 * N * K + 4.0 * N) * chains / (ms_b * 1e-3) / 1e12,
                ms_single_fp64_eval=ms_1)
@@ -216,7 +216,7 @@ def hier_section(warm, samp, chains=64):
     out["max_rel_err_lp"] = e_lp
     out["max_rel_err_grad"] = e_g
     log("hierarchical: single-chain FP64 service for comparison")
-    out["plain_fp64_single_chain"] = nuts_run(mh, warm, samp)
+    out["plain_fp64_single_chain"] = nuts_run(mh, warm, samp, max_depth=max_depth)
     out["min_ess_per_s_vs_plain"] = (out["min_ess_per_s"] /
                                      out["plain_fp64_single_chain"]["min_ess_per_s"])
     mh.close()
@@ -387,7 +387,7 @@ def main():
         log(f"serving: first {args.serve_jobs} jobs of the arrival stream")
         result["serving"] = serving_section(args.serve_jobs)
     if not args.no_hier:
-        result["hier_logistic_64"] = hier_section(300, 300)
+        result["hier_logistic_64"] = hier_section(100, 100)
     if not args.no_c1:
         log("reference CPU case N=1e4 K=50, 4 chains x (1000 + 1000)")
         result["c1_logistic_4_chains"] = c1_section(with_cpu=not args.no_cpu)

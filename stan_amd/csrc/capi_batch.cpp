@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -291,8 +292,14 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
                                    1, b->stream);
     if (rcp != SX_OK) return fail(rcp, "MFMA workspace");
   }
-  // capture a fixed run of lockstep steps in a graph
+  // capture a fixed run of lockstep steps in a graph (or, with
+  // STANX_BATCH_NO_GRAPH set, launch the steps directly: for kernel tracers)
   if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(SX_EDEVICE, "sync");
+  const char* no_graph = std::getenv("STANX_BATCH_NO_GRAPH");
+  if (no_graph && no_graph[0] && no_graph[0] != '0') {
+    *out = b;
+    return SX_OK;
+  }
   if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(SX_EDEVICE, "begin capture");
   int rc = SX_OK;
@@ -309,11 +316,26 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
 
 // One graph launch (steps_per_graph lockstep steps) without waiting, then
 // the done counter copy and an event; at most kRing launches in flight.
+int run_steps(sx_nuts_batch* b) {
+  if (b->exec) {
+    SX_HIP(hipGraphLaunch(b->exec, b->stream));
+    return SX_OK;
+  }
+  for (int i = 0; i < b->steps_per_graph; ++i) {
+    const int rc = enqueue_step(b);
+    if (rc) return rc;
+  }
+  return SX_OK;
+}
+
 int launch(sx_nuts_batch* b) {
   // nothing left to do once every chain has finished (a run is resumed by
   // extend); a full ring waits for poll
   if (b->finished || b->inflight >= sx_nuts_batch::kRing) return SX_OK;
-  SX_HIP(hipGraphLaunch(b->exec, b->stream));
+  {
+    const int rc = run_steps(b);
+    if (rc) return rc;
+  }
   SX_HIP(hipMemcpyAsync(b->h_done, b->a.done_count, sizeof(unsigned), hipMemcpyDeviceToHost,
                         b->stream));
   const int slot = (b->head + b->inflight) % sx_nuts_batch::kRing;
@@ -402,7 +424,10 @@ int sx_nuts_batch_run(sx_nuts_batch* b, int64_t max_steps, int64_t* steps_run,
   int64_t run = 0;
   unsigned done = 0;
   while (run < max_steps && !b->finished) {
-    SX_HIP(hipGraphLaunch(b->exec, b->stream));
+    {
+      const int rc = sx::nb::run_steps(b);
+      if (rc) return rc;
+    }
     run += b->steps_per_graph;
     SX_HIP(hipMemcpyAsync(b->h_done, b->a.done_count, sizeof(unsigned), hipMemcpyDeviceToHost,
                           b->stream));

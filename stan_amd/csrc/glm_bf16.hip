@@ -50,6 +50,20 @@ __device__ __forceinline__ f32x4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl
   return c;
 }
 
+// FP32 Bernoulli-logit row on the hardware transcendentals (v_exp_f32,
+// v_log_f32, v_rcp_f32): term = -(max(-s eta, 0) + log(1 + t)), t =
+// exp(-|s eta|), r = s inv_logit(-s eta).  Absolute error of term ~1e-7,
+// relative error of r a few ulp: below the BF16 hi+lo representation
+// error of the products that feed eta.
+__device__ __forceinline__ void logit_row_fast(float eta, float s, float* term, float* r) {
+  const float se = s * eta;
+  const float t = __expf(-fabsf(se));
+  const float u = 1.0f + t;
+  const float inv = __builtin_amdgcn_rcpf(u);
+  *term = -(fmaxf(-se, 0.0f) + __logf(u));
+  *r = s * (se >= 0.0f ? t * inv : inv);
+}
+
 // FP32 Bernoulli-logit row: term = -log1p(exp(-s eta)), r = s inv_logit(-s eta)
 __device__ __forceinline__ void logit_row_f32(float eta, float s, float* term, float* r) {
   const float se = s * eta;
@@ -127,6 +141,8 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t n0 = t * 32;
+    // the tile's row sums in FP32 (2 rows per lane), added to FP64 per tile
+    float lpt[4] = {0.f, 0.f, 0.f, 0.f}, srt[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
       f32x4 h = {0.f, 0.f, 0.f, 0.f};
@@ -150,20 +166,25 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const float eta = h[j] + alpha[j];
-        float term = 0.f, r = 0.f;
-        if (valid) {
-          if (FAM == SX_LOGIT) {
-            logit_row_f32(eta, yv, &term, &r);
-          } else {
-            r = yv - eta;
-            term = r * r;
-          }
-          if (!isfinite(eta)) bad[j] += 1.0;
+        float term, r;
+        if (FAM == SX_LOGIT) {
+          logit_row_fast(eta, yv, &term, &r);
+        } else {
+          r = yv - eta;
+          term = r * r;
         }
-        lp[j] += term;
-        sr[j] += r;
+        term = valid ? term : 0.f;
+        r = valid ? r : 0.f;
+        if (valid && !isfinite(eta)) bad[j] += 1.0;
+        lpt[j] += term;
+        srt[j] += r;
         rs[w][4 * lg + j][16 * rb + lr] = r;
       }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      lp[j] += static_cast<double>(lpt[j]);
+      sr[j] += static_cast<double>(srt[j]);
     }
     // rs[w] belongs to this wave: LDS accesses of a wave complete in order
     __builtin_amdgcn_wave_barrier();
@@ -216,6 +237,7 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f32x16 mma3_32(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x16 c) {
   c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
@@ -279,88 +301,105 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
 
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
-  const __bf16* src[4] = {reinterpret_cast<const __bf16*>(a.X1h), reinterpret_cast<const __bf16*>(a.X1l),
-                          reinterpret_cast<const __bf16*>(a.X2h), reinterpret_cast<const __bf16*>(a.X2l)};
-  uint4 stage[PER];
+  // tile staging: thread tid moves chunks tid + 256 i (16 bytes each) of
+  // the four tile arrays; chunk -> (array, offset) is fixed per thread, so
+  // the loads and stores below use compile-time slots and stay in registers
+  const __bf16* X1h = reinterpret_cast<const __bf16*>(a.X1h);
+  const __bf16* X1l = reinterpret_cast<const __bf16*>(a.X1l);
+  const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
+  const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
+  u32x4 stage[PER];  // native vectors: kept in registers
   float ystage = 0.f;
-  auto gload = [&](int64_t t) {
-    const int64_t base = t * 32 * Kp;  // both layouts: a tile is 32 Kp contiguous elements
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int c = tid + 256 * i;
-      const int arr = c / (4 * Kp), o = (c % (4 * Kp)) * 8;
-      stage[i] = *reinterpret_cast<const uint4*>(src[arr] + base + o);
-    }
-    if (tid < 32) {
-      const int64_t n = t * 32 + tid;
-      float v = 0.f;
-      if (n < a.N) {
-        if (FAM == SX_LOGIT)
-          v = static_cast<const int8_t*>(a.y)[n] ? 1.0f : -1.0f;
-        else
-          v = static_cast<float>(static_cast<const double*>(a.y)[n]);
-      }
-      ystage = v;
-    }
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int c = tid + 256 * i;
-      const int arr = c / (4 * Kp), o = (c % (4 * Kp)) * 8;
-      if (arr < 2) {
-        const int row = o / Kp, col = o % Kp;
-        __bf16* d = (arr == 0 ? x1h[buf] : x1l[buf]) + row * X1S + col;
-        *reinterpret_cast<uint4*>(d) = stage[i];
-      } else {
-        __bf16* d = (arr == 2 ? x2h[buf] : x2l[buf]) + o;
-        *reinterpret_cast<uint4*>(d) = stage[i];
-      }
-    }
-    if (tid < 32) ys[buf][tid] = ystage;
-  };
+#define SX_GLOAD(T)                                                              \
+  {                                                                              \
+    const int64_t base_ = (T) * 32 * Kp;                                         \
+    _Pragma("unroll") for (int i = 0; i < PER; ++i) {                            \
+      const int c_ = tid + 256 * i;                                              \
+      const int arr_ = c_ / (4 * Kp), o_ = (c_ % (4 * Kp)) * 8;                  \
+      const __bf16* p_ = arr_ == 0 ? X1h : arr_ == 1 ? X1l : arr_ == 2 ? X2h : X2l; \
+      stage[i] = *reinterpret_cast<const u32x4*>(p_ + base_ + o_);               \
+    }                                                                            \
+    if (tid < 32) {                                                              \
+      const int64_t n_ = (T) * 32 + tid;                                         \
+      float v_ = 0.f;                                                            \
+      if (n_ < a.N) {                                                            \
+        if (FAM == SX_LOGIT)                                                     \
+          v_ = static_cast<const int8_t*>(a.y)[n_] ? 1.0f : -1.0f;               \
+        else                                                                     \
+          v_ = static_cast<float>(static_cast<const double*>(a.y)[n_]);          \
+      }                                                                          \
+      ystage = v_;                                                               \
+    }                                                                            \
+  }
+#define SX_LSTORE(BUF)                                                           \
+  {                                                                              \
+    _Pragma("unroll") for (int i = 0; i < PER; ++i) {                            \
+      const int c_ = tid + 256 * i;                                              \
+      const int arr_ = c_ / (4 * Kp), o_ = (c_ % (4 * Kp)) * 8;                  \
+      __bf16* d_;                                                                \
+      if (arr_ < 2)                                                              \
+        d_ = (arr_ == 0 ? x1h[BUF] : x1l[BUF]) + (o_ / Kp) * X1S + (o_ % Kp);    \
+      else                                                                       \
+        d_ = (arr_ == 2 ? x2h[BUF] : x2l[BUF]) + o_;                             \
+      *reinterpret_cast<u32x4*>(d_) = stage[i];                                  \
+    }                                                                            \
+    if (tid < 32) ys[BUF][tid] = ystage;                                         \
+  }
   if (t0 < t1) {
-    gload(t0);
-    lstore(0);
+    SX_GLOAD(t0);
+    SX_LSTORE(0);
   }
   __syncthreads();
   for (int64_t t = t0; t < t1; ++t) {
     const int buf = static_cast<int>((t - t0) & 1);
-    if (t + 1 < t1) gload(t + 1);
-    // forward
-    f32x16 hacc;
+    if (t + 1 < t1) SX_GLOAD(t + 1);
+    // forward: the three products accumulate separately, so consecutive
+    // MFMAs are independent and issue at throughput rather than latency
+    f32x16 hhh, hhl, hlh;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) hacc[j] = 0.f;
+    for (int j = 0; j < 16; ++j) hhh[j] = hhl[j] = hlh[j] = 0.f;
 #pragma unroll
     for (int s = 0; s < S16; ++s) {
       const int off = li * X1S + 16 * s + 8 * h;
       const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1h[buf][off]);
       const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1l[buf][off]);
-      hacc = mma3_32(xh, xl, bh[s], bl[s], hacc);
+      hhh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], hhh, 0, 0, 0);
+      hhl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], hhl, 0, 0, 0);
+      hlh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], hlh, 0, 0, 0);
     }
-    // link: row of element j is 8 (j/4) + 4 h + j%4
+    f32x16 hacc;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) hacc[j] = (hhh[j] + hhl[j]) + hlh[j];
+    // link, branch-free (rows past N are masked): row of element j is
+    // 8 (j/4) + 4 h + j%4.  The tile's sums are FP32 (16 terms per lane) and
+    // are added to the FP64 totals once per tile.
     float r[16];
     const int64_t n0 = t * 32;
+    float lpt = 0.f, srt = 0.f;
+    int badt = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int row = 8 * (j >> 2) + 4 * h + (j & 3);
       const bool valid = n0 + row < a.N;
       const float yv = ys[buf][row];
       const float eta = hacc[j] + alpha;
-      float term = 0.f, rr = 0.f;
-      if (valid) {
-        if (FAM == SX_LOGIT) {
-          logit_row_f32(eta, yv, &term, &rr);
-        } else {
-          rr = yv - eta;
-          term = rr * rr;
-        }
-        if (!isfinite(eta)) bad += 1.0;
+      float term, rr;
+      if (FAM == SX_LOGIT) {
+        logit_row_fast(eta, yv, &term, &rr);
+      } else {
+        rr = yv - eta;
+        term = rr * rr;
       }
-      lp += term;
-      sr += rr;
+      term = valid ? term : 0.f;
+      rr = valid ? rr : 0.f;
+      badt += (valid && !isfinite(eta)) ? 1 : 0;
+      lpt += term;
+      srt += rr;
       r[j] = rr;
     }
+    lp += static_cast<double>(lpt);
+    sr += static_cast<double>(srt);
+    bad += static_cast<double>(badt);
     // adjoint
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
@@ -372,6 +411,7 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
         rh[j] = hh;
         rl[j] = ll;
       }
+      bf16x8 xh[KQ], xl[KQ];
 #pragma unroll
       for (int kb = 0; kb < KQ; ++kb) {
         const int k = 32 * kb + li;
@@ -381,20 +421,31 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
         const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2h[buf][o1]);
         const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2l[buf][o0]);
         const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2l[buf][o1]);
-        bf16x8 xh, xl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          xh[j] = ah0[j];
-          xh[4 + j] = ah1[j];
-          xl[j] = al0[j];
-          xl[4 + j] = al1[j];
+          xh[kb][j] = ah0[j];
+          xh[kb][4 + j] = ah1[j];
+          xl[kb][j] = al0[j];
+          xl[kb][4 + j] = al1[j];
         }
-        g[kb] = mma3_32(rh, rl, xh, xl, g[kb]);
       }
+      // product-major order: the KQ accumulators make consecutive MFMAs
+      // independent
+#pragma unroll
+      for (int kb = 0; kb < KQ; ++kb)
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xh[kb], g[kb], 0, 0, 0);
+#pragma unroll
+      for (int kb = 0; kb < KQ; ++kb)
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xl[kb], g[kb], 0, 0, 0);
+#pragma unroll
+      for (int kb = 0; kb < KQ; ++kb)
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, xh[kb], g[kb], 0, 0, 0);
     }
-    if (t + 1 < t1) lstore(buf ^ 1);
+    if (t + 1 < t1) SX_LSTORE(buf ^ 1);
     __syncthreads();
   }
+#undef SX_GLOAD
+#undef SX_LSTORE
   // g[kb][j] = G^T[chain cbase + 8 (j/4) + 4 h + j%4][k = 32 kb + li]
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll

@@ -103,11 +103,14 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
 
 // Bernoulli-logit row term and residual in FP32 for sign s = 2y - 1; s = 0
 // (a padding row) gives term = r = 0.
+// Hardware transcendentals (v_exp_f32, v_log_f32, v_rcp_f32): absolute
+// error of term ~1e-7, a few ulp relative in r.
 __device__ __forceinline__ void logit_row_f32s(float e, float s, float* term, float* r) {
   const float se = s * e;
   const float t = __expf(-fabsf(se));
-  const float inv = 1.0f / (1.0f + t);
-  *term = s == 0.f ? 0.f : -(fmaxf(-se, 0.0f) + log1pf(t));
+  const float u = 1.0f + t;
+  const float inv = __builtin_amdgcn_rcpf(u);
+  *term = s == 0.f ? 0.f : -(fmaxf(-se, 0.0f) + __logf(u));
   *r = s * (se >= 0.0f ? t * inv : inv);
 }
 
@@ -142,7 +145,8 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) acc[k] = 0.f;
   float s1 = 0.f;
-  double s0 = 0.0, bad = 0.0;
+  double s0 = 0.0;
+  int bad = 0;
   for (int p0 = 0; p0 < rows; p0 += 64) {
     const int cnt = rows - p0 < 64 ? rows - p0 : 64;
     const float* src = a.X32 + (r0 + p0) * K;
@@ -161,6 +165,7 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
     }
+    float s0p = 0.f;  // the piece's row terms in FP32, added to FP64 per piece
     for (int i = 0; i < cnt; i += 2) {
       const float* x0 = &sx[w][i * K];
       const float* x1 = x0 + K;
@@ -175,16 +180,17 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
       float term0, r0, term1, r1;
       logit_row_f32s(e0, sg0, &term0, &r0);
       logit_row_f32s(e1, sg1, &term1, &r1);
-      if (!isfinite(e0)) bad += 1.0;
-      if (sg1 != 0.f && !isfinite(e1)) bad += 1.0;
-      s0 += term0;
-      s0 += term1;
+      bad += isfinite(e0) ? 0 : 1;
+      bad += (sg1 != 0.f && !isfinite(e1)) ? 1 : 0;
+      s0p += term0;
+      s0p += term1;
       s1 += r0;
       s1 += r1;
 #pragma unroll
       for (int k = 0; k < KMAX; ++k)
         if (k < K) acc[k] = fmaf(x1[k], r1, fmaf(x0[k], r0, acc[k]));
     }
+    s0 += static_cast<double>(s0p);
   }
   if (!live) return;
   double* out = a.cpart + (ch * a.chains + c) * a.E;
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
   for (int k = 0; k < KMAX; ++k)
     if (k < K) out[1 + k] = acc[k];
   out[K + 1] = s0;
-  out[K + 2] = bad;
+  out[K + 2] = static_cast<double>(bad);
 }
 
 // Per group and chain: sum the group's chunks in order; z gradients;
