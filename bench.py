@@ -126,7 +126,7 @@ def serving_data(job):
     return X, y
 
 
-def serving_section(n_jobs, target_ess=1000.0, warmup=200):
+def serving_section(n_jobs, target_ess=1000.0, warmup=200, deadline_s=120.0):
     """C5: a bounded prefix of the arrival stream through the serving loop."""
     import stan_amd as S
     from stan_amd import workloads as W
@@ -148,22 +148,38 @@ def serving_section(n_jobs, target_ess=1000.0, warmup=200):
                               num_warmup=warmup, num_samples=first, random_seed=j["seed"],
                               chain=1))
     del data
-    last = time.perf_counter()
+    t_submitted = time.perf_counter()
+    last = t_submitted
     pending = list(ids)
-    while pending:
-        pending = [i for i in pending if not srv.wait(i, 0.5)]
+    while pending and time.perf_counter() - t_submitted < deadline_s:
+        still = []
+        for i in pending:
+            if srv.wait(i, 0.0):
+                inf = srv.info(i)
+                log(f"  job {i}: {inf['t_done'] - inf['t_submit']:.1f} s, {inf['steps']} steps, "
+                    f"min ESS {inf['min_ess']:.0f}, {inf['extensions']} extensions")
+            else:
+                still.append(i)
+        pending = still
+        time.sleep(0.2)
         if time.perf_counter() - last > 10:
             last = time.perf_counter()
             log(f"  ... serving: {len(ids) - len(pending)}/{len(ids)} jobs done")
-    infos = [srv.info(i) for i in ids]
-    srv.close()
+    infos_all = {i: srv.info(i) for i in ids}
+    srv.close()  # cancels what is still running at the deadline
+    unfinished = len(pending)
+    infos = [infos_all[i] for i in ids if i not in pending]
+    if not infos:
+        return dict(jobs=n_jobs, completed=0, unfinished_at_deadline=unfinished,
+                    deadline_s=deadline_s)
     lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
     queue = np.array([i["t_admit"] - i["t_submit"] for i in infos])
     t_first = min(i["t_submit"] for i in infos)
     makespan = max(i["t_done"] for i in infos) - t_first
     n_grad = sum(i["n_grad"] for i in infos)
     ess = [i["min_ess"] for i in infos]
-    return dict(jobs=n_jobs, arrival_rate_per_s=20.0, seed=7, target_min_ess=target_ess,
+    return dict(jobs=n_jobs, completed=len(infos), unfinished_at_deadline=unfinished,
+                deadline_s=deadline_s, arrival_rate_per_s=20.0, seed=7, target_min_ess=target_ess,
                 warmup=warmup, p50_s=float(np.percentile(lat, 50)),
                 p99_s=float(np.percentile(lat, 99)), max_s=float(lat.max()),
                 p50_queue_s=float(np.percentile(queue, 50)), makespan_s=makespan,
@@ -172,7 +188,7 @@ def serving_section(n_jobs, target_ess=1000.0, warmup=200):
                 jobs_reaching_target=int(sum(e >= target_ess for e in ess)),
                 min_ess_min=float(np.nanmin(ess)),
                 extensions=int(sum(i["extensions"] for i in infos)),
-                mfma_jobs=int(sum(1 for j in jobs if j["K"] <= 256)))
+                mfma_jobs=int(sum(1 for j in jobs if j["K"] <= 256 and j["C"] >= 64)))
 
 
 def mixed_error(model, ref_data, family, thetas):
@@ -278,7 +294,7 @@ def main():
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--no-hier", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=30)
+    ap.add_argument("--serve-jobs", type=int, default=16)
     ap.add_argument("--no-c1", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -295,6 +295,9 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m,
                          const double* inv_metric);
 /* Advance all chains by up to max_steps lockstep gradient steps (stops
  * early once every chain has finished).                                 */
+/* Steps run as replays of a captured HIP graph of 16 steps; with the
+ * environment variable STANX_BATCH_NO_GRAPH=1 at create they are launched
+ * directly instead (same kernels and results; for kernel tracers).     */
 int sx_nuts_batch_run(sx_nuts_batch* b, int64_t max_steps,
                       int64_t* steps_run, int64_t* chains_done);
 /* Rows of draws each chain can hold ((save_warmup ? warmup : 0) + samples,
@@ -343,8 +346,9 @@ typedef struct sx_job_spec {
   int64_t J;
   sx_prior prior;
   int64_t chains;
-  int grad_precision;         /* SX_BF16X2 (used when K <= 256), SX_FP32
-                                 (SX_HIER_LOGIT) or SX_FP64                    */
+  int grad_precision;         /* SX_BF16X2 (used when K <= 256 and chains >=
+                                 64; FP64 otherwise), SX_FP32 (SX_HIER_LOGIT)
+                                 or SX_FP64                                    */
   sx_nuts_config cfg;         /* chain ids cfg.chain, cfg.chain + 1, ...;
                                  cfg.num_samples: first sampling phase       */
   double target_min_ess;      /* <= 0: run cfg.num_samples only              */
@@ -380,7 +384,8 @@ int sx_server_results(sx_server* srv, int64_t job_id, double* draws,
                       int64_t* rows, int* status);
 /* forget a finished job and its host copy of the draws                  */
 int sx_server_release(sx_server* srv, int64_t job_id);
-/* drops queued jobs, waits for resident ones, stops the scheduler       */
+/* drops queued jobs and cancels resident ones (both end SX_JOB_FAILED),
+ * then stops the scheduler                                              */
 void sx_server_destroy(sx_server* srv);
 
 #ifdef __cplusplus

@@ -25,9 +25,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <future>
@@ -74,9 +77,14 @@ struct job {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// The MFMA pass computes whole chain tiles (64 or 128 chains): with fewer
+// chains the FP64 per-chain passes cost less, and are exact.
+constexpr int64_t kMinMfmaChains = 64;
+
 int gradient_precision(const sx_job_spec& s) {
   if (s.family == SX_HIER_LOGIT) return s.grad_precision == SX_FP32 ? SX_FP32 : SX_FP64;
-  return (s.grad_precision == SX_BF16X2 && s.K <= 256) ? SX_BF16X2 : SX_FP64;
+  return (s.grad_precision == SX_BF16X2 && s.K <= 256 && s.chains >= kMinMfmaChains) ? SX_BF16X2
+                                                                                      : SX_FP64;
 }
 
 // Device bytes a job holds while resident: data layouts, the sampler's
@@ -133,7 +141,8 @@ struct sx_server {
   std::vector<job*> active;
   std::map<int64_t, std::unique_ptr<job>> jobs;
   std::thread th;
-  bool stop = false;
+  std::atomic<bool> stop{false};
+  bool log = false;  // STANX_SERVER_LOG: scheduler events on stderr
   int64_t next_id = 1;
   clock_t_::time_point t0 = clock_t_::now();
   std::string fatal;
@@ -142,7 +151,16 @@ struct sx_server {
 
 namespace {
 
+void log_event(sx_server* srv, const job* j, const char* what) {
+  if (!srv->log) return;
+  std::fprintf(stderr, "[server %8.3f] job %lld %s (N=%lld K=%lld chains=%lld samples=%lld ext=%d ess=%.1f)\n",
+               srv->now(), static_cast<long long>(j->id), what, static_cast<long long>(j->spec.N),
+               static_cast<long long>(j->spec.K), static_cast<long long>(j->spec.chains),
+               static_cast<long long>(j->samples), j->extensions, j->min_ess);
+}
+
 void finish_job(sx_server* srv, job* j, int rc, const std::string& err) {
+  log_event(srv, j, rc == SX_OK ? "done" : "failed");
   if (j->b) {
     sx_nuts_batch_destroy(j->b);
     j->b = nullptr;
@@ -232,7 +250,7 @@ void scheduler(sx_server* srv) {
       if (srv->active.empty() && srv->pending.empty())
         srv->cv.wait_for(lk, std::chrono::milliseconds(5));
       if (srv->stop) {
-        // queued jobs are dropped; resident ones run to completion
+        // queued jobs are dropped, resident ones cancelled (below)
         for (job* j : srv->pending) {
           j->state = SX_JOB_FAILED;
           j->rc = SX_EINVAL;
@@ -242,6 +260,21 @@ void scheduler(sx_server* srv) {
         srv->cv.notify_all();
         if (srv->active.empty()) return;
       }
+    }
+    if (srv->stop) {
+      std::vector<job*> act;
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        act = srv->active;
+      }
+      for (job* j : act) {
+        if (j->analysing) j->ess.wait();
+        finish_job(srv, j, SX_EINVAL, "cancelled: server stopped");
+      }
+      continue;
+    }
+    {
+      std::unique_lock<std::mutex> lk(srv->mu);
       for (auto it = srv->pending.begin(); it != srv->pending.end();) {
         job* j = *it;
         if (static_cast<int>(srv->active.size() + admit.size()) >= srv->max_active) break;
@@ -258,7 +291,9 @@ void scheduler(sx_server* srv) {
       for (job* j : admit) srv->active.push_back(j);
     }
     for (job* j : admit) {
+      log_event(srv, j, "admitted");
       const int rc = start_job(srv, j);
+      log_event(srv, j, "started");
       if (rc) finish_job(srv, j, rc, sx_last_error());
     }
     std::vector<job*> act;
@@ -292,6 +327,7 @@ void scheduler(sx_server* srv) {
           }
           j->samples += extra;
           ++j->extensions;
+          log_event(srv, j, "extended");
           j->draws.clear();
           continue;
         }
@@ -301,6 +337,7 @@ void scheduler(sx_server* srv) {
       bool finished = false;
       int rc = sx::nb::poll(j->b, &finished);
       if (!rc && finished) {
+        log_event(srv, j, "collect");
         rc = collect(j);
         progressed = true;
         if (rc) finish_job(srv, j, rc, sx_last_error());
@@ -359,6 +396,8 @@ int sx_server_create(sx_server** out, int device, int64_t hbm_budget_bytes, int 
   srv->device = device;
   srv->budget = hbm_budget_bytes;
   srv->max_active = max_active;
+  const char* lg = std::getenv("STANX_SERVER_LOG");
+  srv->log = lg && lg[0] && lg[0] != '0';
   srv->th = std::thread(scheduler, srv);
   *out = srv;
   return SX_OK;

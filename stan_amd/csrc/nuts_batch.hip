@@ -142,6 +142,47 @@ __device__ __forceinline__ double vsum(const ctx& k, Ld term) {
   return acc;
 }
 
+// vpass whose write step also adds to NS thread-local sums, in the index
+// order vsum uses (so a fused pass gives the sums of separate passes
+// bitwise)
+template <int NS, class T, class Ld, class St>
+__device__ __forceinline__ void vpass_acc(const ctx& k, double (&acc)[NS], Ld ld, St st) {
+  const int D = k.a->D, nt = k.nt;
+  for (int i0 = k.lane; i0 < D; i0 += kU * nt) {
+    T v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * nt;
+      if (i < D) v[u] = ld(i);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * nt;
+      if (i < D) st(i, v[u], acc);
+    }
+  }
+}
+// block_sum of NS values with one exchange (per value: the same order)
+template <int NS>
+__device__ __forceinline__ void block_sum_n(const ctx& k, double (&x)[NS]) {
+#pragma unroll
+  for (int j = 0; j < NS; ++j) x[j] = wave_sum(x[j]);
+  if (k.nt == 64) return;
+  const int w = k.lane >> 6;
+  __syncthreads();
+  if ((k.lane & 63) == 0)
+#pragma unroll
+    for (int j = 0; j < NS; ++j) k.red[8 * j + w] = x[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    double t = 0.0;
+    for (int i = 0; i < (k.nt >> 6); ++i) t += k.red[8 * j + i];
+    x[j] = t;
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void vcopy(const ctx& k, int dst, int src) {
   double* d = k.slot(dst);
   const double* s = k.slot(src);
@@ -247,11 +288,26 @@ __device__ void request_gradient(const ctx& k, int next_phase) {
 }
 // base_hamiltonian::update_potential_gradient: V = -lp, g = -grad; on a
 // domain error V = +inf and g = -g_old (the reference flips the stale g)
-__device__ void consume_gradient(const ctx& k) {
+// With lf, the second half of the leapfrog (p -= eps/2 g) follows in the
+// same pass: the phases that requested this gradient (PH_LEAF, PH_SS_B)
+// begin with it.
+__device__ void consume_gradient(const ctx& k, bool lf, double lf_eps) {
   const double* o = k.a->out + static_cast<int64_t>(k.c) * k.a->out_stride;
   const bool ok = o[k.a->D + 1] == 0.0 && isfinite(o[0]);
   double* g = k.slot(ZG);
-  vpass<double>(k, [&](int i) { return ok ? o[1 + i] : g[i]; }, [&](int i, double v) { g[i] = -v; });
+  if (lf) {
+    double* p = k.slot(ZP);
+    const double he = 0.5 * lf_eps;
+    struct gp { double g, p; };
+    vpass<gp>(k, [&](int i) { return gp{ok ? o[1 + i] : g[i], p[i]}; },
+              [&](int i, const gp& v) {
+                const double gn = -v.g;
+                g[i] = gn;
+                p[i] = v.p - he * gn;
+              });
+  } else {
+    vpass<double>(k, [&](int i) { return ok ? o[1 + i] : g[i]; }, [&](int i, double v) { g[i] = -v; });
+  }
   if (k.lane == 0) {
     k.s->V_z = ok ? -o[0] : kInf;
     k.s->last_lp = o[0];
@@ -273,6 +329,29 @@ __device__ void leapfrog_begin(const ctx& k, double eps) {
                 p[i] = pn;
                 q[i] = v.q + eps * (v.m * pn);
               });
+}
+// leapfrog_begin then request_gradient, in one pass
+__device__ void leapfrog_request(const ctx& k, double eps, int next_phase) {
+  double* p = k.slot(ZP);
+  double* q = k.slot(ZQ);
+  const double* g = k.slot(ZG);
+  const double* m = k.slot(MINV);
+  double* th = k.a->theta + static_cast<int64_t>(k.c) * k.a->D;
+  const double he = 0.5 * eps;
+  struct pgqm { double p, g, q, m; };
+  vpass<pgqm>(k, [&](int i) { return pgqm{p[i], g[i], q[i], m[i]}; },
+              [&](int i, const pgqm& v) {
+                const double pn = v.p - he * v.g;
+                const double qn = v.q + eps * (v.m * pn);
+                p[i] = pn;
+                q[i] = qn;
+                th[i] = qn;
+              });
+  if (k.lane == 0) {
+    k.s->phase = next_phase;
+    k.s->pending = 1;
+    k.s->n_grad++;
+  }
 }
 __device__ void leapfrog_end(const ctx& k, double eps) {
   double* p = k.slot(ZP);
@@ -309,7 +388,28 @@ __device__ void start_frame(const ctx& k, int l) {
 // base case of build_tree, after the leapfrog's gradient
 __device__ bool leaf(const ctx& k, const dest& d) {
   chain_state* s = k.s;
-  const double T = kinetic(k, ZP);
+  // one pass: kinetic energy, proposal position, rho += p, subtree ends
+  double acc[1] = {0.0};
+  {
+    const double* p = k.slot(ZP);
+    const double* m = k.slot(MINV);
+    const double* q = k.slot(ZQ);
+    double* prop = k.slot(prop_slot(d.zprop));
+    double* r = k.slot(d.rho);
+    double* pb = k.slot(d.pbeg);
+    double* pe = k.slot(d.pend);
+    struct lf { double p, m, q, r; };
+    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{p[i], m[i], q[i], r[i]}; },
+                     [&](int i, const lf& v, double (&ac)[1]) {
+                       ac[0] += v.p * (v.m * v.p);
+                       prop[i] = v.q;
+                       r[i] = v.r + v.p;
+                       pb[i] = v.p;
+                       pe[i] = v.p;
+                     });
+  }
+  block_sum_n<1>(k, acc);
+  const double T = 0.5 * acc[0];
   double h = T + s->V_z;
   if (isnan(h)) h = kInf;
   const double H0 = s->H0;
@@ -324,20 +424,6 @@ __device__ bool leaf(const ctx& k, const dest& d) {
       s->smp += exp(H0 - h);
     prop_V(k, d.zprop) = s->V_z;
     prop_T(k, d.zprop) = T;
-  }
-  vcopy(k, prop_slot(d.zprop), ZQ);
-  {
-    const double* p = k.slot(ZP);
-    double* r = k.slot(d.rho);
-    double* pb = k.slot(d.pbeg);
-    double* pe = k.slot(d.pend);
-    struct rp { double r, p; };
-    vpass<rp>(k, [&](int i) { return rp{r[i], p[i]}; },
-              [&](int i, const rp& v) {
-                r[i] = v.r + v.p;
-                pb[i] = v.p;
-                pe[i] = v.p;
-              });
   }
   __syncthreads();
   return !s->divergent;
@@ -357,27 +443,55 @@ __device__ bool merge(const ctx& k, int l, const dest& d) {
     take = uniform(k) < accept_prob;
   }
   __syncthreads();
-  if (k.lane == 0) lw = new_lw;
-  if (take) copy_prop(k, d.zprop, l);
-  // rho_subtree = rho_init + rho_final; rho(d) += rho_subtree
+  if (k.lane == 0) {
+    lw = new_lw;
+    if (take) {
+      prop_V(k, d.zprop) = prop_V(k, l);
+      prop_T(k, d.zprop) = prop_T(k, l);
+    }
+  }
+  // one pass: the proposal copy, rho_subtree = rho_init + rho_final,
+  // rho(d) += rho_subtree, and the six sums of the three U-turn checks
+  // (p_sharp = minv p): merged (pend, pbeg against rho_subtree), left join
+  // (pfbg, pbeg against rho_init + pfbg), right join (pend, pie against
+  // rho_final + pie)
+  double acc[6] = {0, 0, 0, 0, 0, 0};
   {
     const double* ri = k.slot(RI(l));
     const double* rf = k.slot(RF(l));
+    const double* m = k.slot(MINV);
+    const double* pend = k.slot(d.pend);
+    const double* pbeg = k.slot(d.pbeg);
+    const double* pfbg = k.slot(PFBG(l));
+    const double* pie = k.slot(PIE(l));
+    const double* psrc = k.slot(prop_slot(l));
+    double* pdst = k.slot(prop_slot(d.zprop));
     double* t = k.slot(TMP);
     double* r = k.slot(d.rho);
-    struct abr { double a, b, r; };
-    vpass<abr>(k, [&](int i) { return abr{ri[i], rf[i], r[i]}; },
-               [&](int i, const abr& v) {
-                 const double tv = v.a + v.b;
-                 t[i] = tv;
-                 r[i] = v.r + tv;
-               });
+    struct mg { double ri, rf, r, m, pend, pbeg, pfbg, pie, prop; };
+    vpass_acc<6, mg>(
+        k, acc,
+        [&](int i) {
+          return mg{ri[i], rf[i], r[i], m[i], pend[i], pbeg[i], pfbg[i], pie[i],
+                    take ? psrc[i] : 0.0};
+        },
+        [&](int i, const mg& v, double (&ac)[6]) {
+          if (take) pdst[i] = v.prop;
+          const double tv = v.ri + v.rf;
+          t[i] = tv;
+          r[i] = v.r + tv;
+          ac[0] += (v.m * v.pend) * tv;
+          ac[1] += (v.m * v.pbeg) * tv;
+          ac[2] += (v.m * v.pfbg) * (v.ri + v.pfbg);
+          ac[3] += (v.m * v.pbeg) * (v.ri + v.pfbg);
+          ac[4] += (v.m * v.pend) * (v.rf + v.pie);
+          ac[5] += (v.m * v.pie) * (v.rf + v.pie);
+        });
   }
-  bool persist = dot_sharp(k, d.pend, TMP) > 0 && dot_sharp(k, d.pbeg, TMP) > 0;
-  persist &= dot_sharp2(k, PFBG(l), RI(l), PFBG(l)) > 0 &&
-             dot_sharp2(k, d.pbeg, RI(l), PFBG(l)) > 0;
-  persist &= dot_sharp2(k, d.pend, RF(l), PIE(l)) > 0 &&
-             dot_sharp2(k, PIE(l), RF(l), PIE(l)) > 0;
+  block_sum_n<6>(k, acc);
+  bool persist = acc[0] > 0 && acc[1] > 0;
+  persist &= acc[2] > 0 && acc[3] > 0;
+  persist &= acc[4] > 0 && acc[5] > 0;
   __syncthreads();
   return persist;
 }
@@ -513,7 +627,7 @@ __device__ void begin_stepsize_heuristic(const ctx& k, int ret_phase) {
 __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   __shared__ chain_state s;
   __shared__ batch_args sa;  // helpers read the arguments through ctx::a
-  __shared__ double red[8];
+  __shared__ double red[48];  // block_sum_n: 8 waves x up to 6 values
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int nt = blockDim.x;
@@ -528,7 +642,10 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         a.chain0 + static_cast<uint32_t>(c)};
   if (s.phase == PH_DONE) return;
   if (s.pending) {
-    consume_gradient(k);
+    // PH_LEAF and PH_SS_B begin with the leapfrog's second half
+    const bool lf = s.phase == PH_LEAF || s.phase == PH_SS_B;
+    const double lf_eps = s.phase == PH_LEAF ? (s.fwd ? 1.0 : -1.0) * s.eps : s.nom_eps;
+    consume_gradient(k, lf, lf_eps);
     __syncthreads();
     if (lane == 0) s.pending = 0;
   }
@@ -582,14 +699,10 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
       case PH_SS_A: {  // heuristic: gradient at z with fresh p
         const double T = kinetic(k, ZP);
         if (lane == 0) s.ss_H0 = T + s.V_z;
-        leapfrog_begin(k, s.nom_eps);
-        __syncthreads();
-        request_gradient(k, PH_SS_B);
+        leapfrog_request(k, s.nom_eps, PH_SS_B);
         break;
       }
-      case PH_SS_B: {
-        leapfrog_end(k, s.nom_eps);
-        __syncthreads();
+      case PH_SS_B: {  // the leapfrog's second half ran with the gradient
         const double T = kinetic(k, ZP);
         double h = T + s.V_z;
         if (isnan(h)) h = kInf;
@@ -718,15 +831,11 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         __syncthreads();
         for (int l = s.tree_depth; l >= 1; --l) start_frame(k, l);
         __syncthreads();
-        leapfrog_begin(k, (s.fwd ? 1.0 : -1.0) * s.eps);
-        __syncthreads();
-        request_gradient(k, PH_LEAF);
+        leapfrog_request(k, (s.fwd ? 1.0 : -1.0) * s.eps, PH_LEAF);
         break;
       }
-      case PH_LEAF: {
+      case PH_LEAF: {  // the leapfrog's second half ran with the gradient
         const double eps = (s.fwd ? 1.0 : -1.0) * s.eps;
-        leapfrog_end(k, eps);
-        __syncthreads();
         bool valid = leaf(k, dest_of(k, 0));
         bool descended = false;
         int l = 1;
@@ -734,9 +843,24 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
           if (!valid) break;
           if (s.stage[l] == 0) {
             // initial half done: start the final half from the current point
-            const double T = kinetic(k, ZP);
-            vcopy(k, ZPF(l), ZQ);
-            vzero(k, RF(l));
+            // (one pass: kinetic energy, proposal position, rho_final = 0)
+            double kacc[1] = {0.0};
+            {
+              const double* p = k.slot(ZP);
+              const double* m = k.slot(MINV);
+              const double* q = k.slot(ZQ);
+              double* zpf = k.slot(ZPF(l));
+              double* rf = k.slot(RF(l));
+              struct pmq { double p, m, q; };
+              vpass_acc<1, pmq>(k, kacc, [&](int i) { return pmq{p[i], m[i], q[i]}; },
+                                [&](int i, const pmq& v, double (&ac)[1]) {
+                                  ac[0] += v.p * (v.m * v.p);
+                                  zpf[i] = v.q;
+                                  rf[i] = 0.0;
+                                });
+            }
+            block_sum_n<1>(k, kacc);
+            const double T = 0.5 * kacc[0];
             __syncthreads();
             if (lane == 0) {
               s.stage[l] = 1;
@@ -747,9 +871,7 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
             __syncthreads();
             for (int m = l - 1; m >= 1; --m) start_frame(k, m);
             __syncthreads();
-            leapfrog_begin(k, eps);
-            __syncthreads();
-            request_gradient(k, PH_LEAF);
+            leapfrog_request(k, eps, PH_LEAF);
             descended = true;
             break;
           }
@@ -758,12 +880,12 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         }
         if (descended) break;
         // the subtree of depth tree_depth is complete (or invalid)
-        if (s.fwd)
-          copy_point(k, FQ, ZQ, s.V_fwd, s.V_z);
-        else
-          copy_point(k, BQ, ZQ, s.V_bck, s.V_z);
-        __syncthreads();
         if (!valid) {
+          if (s.fwd)
+            copy_point(k, FQ, ZQ, s.V_fwd, s.V_z);
+          else
+            copy_point(k, BQ, ZQ, s.V_bck, s.V_z);
+          __syncthreads();
           if (lane == 0) s.phase = PH_TRANS_END;
           break;
         }
@@ -774,17 +896,62 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
           take = uniform(k) < exp(s.lsw_sub - s.lsw);
         }
         __syncthreads();
-        if (take) copy_prop(k, 0, -1);
+        const bool fwd = s.fwd != 0;
+        if (lane == 0) {
+          if (fwd)
+            s.V_fwd = s.V_z;
+          else
+            s.V_bck = s.V_z;
+          if (take) {
+            s.V_samp = s.V_prop;
+            s.T_samp = s.T_prop;
+          }
+        }
+        // one pass: the new trajectory end (z), the sample copy, rho =
+        // rho_bck + rho_fwd and the six sums of the top-level U-turn checks
+        double acc[6] = {0, 0, 0, 0, 0, 0};
         {
+          const double* zq = k.slot(ZQ);
+          const double* zp = k.slot(ZP);
+          const double* zg = k.slot(ZG);
+          const double* pq = k.slot(PQ);
           const double* rb = k.slot(RHOB);
           const double* rf = k.slot(RHOF);
-          double* r = k.slot(RHO);
-          vpass<double>(k, [&](int i) { return rb[i] + rf[i]; }, [&](int i, double v) { r[i] = v; });
+          const double* m = k.slot(MINV);
+          const double* oth = k.slot(fwd ? BP : FP);
+          const double* pfb = k.slot(PFB);
+          const double* pbf = k.slot(PBF);
+          double* eq = k.slot(fwd ? FQ : BQ);
+          double* sq = k.slot(SQ);
+          double* rho = k.slot(RHO);
+          struct tl { double zq, zp, zg, pq, rb, rf, m, oth, pfb, pbf; };
+          vpass_acc<6, tl>(
+              k, acc,
+              [&](int i) {
+                return tl{zq[i], zp[i], zg[i], take ? pq[i] : 0.0, rb[i], rf[i], m[i], oth[i],
+                          pfb[i], pbf[i]};
+              },
+              [&](int i, const tl& v, double (&ac)[6]) {
+                eq[i] = v.zq;
+                eq[i + a.Dpad] = v.zp;
+                eq[i + 2 * a.Dpad] = v.zg;
+                if (take) sq[i] = v.pq;
+                const double r = v.rb + v.rf;
+                rho[i] = r;
+                const double fpv = fwd ? v.zp : v.oth;
+                const double bpv = fwd ? v.oth : v.zp;
+                ac[0] += (v.m * fpv) * r;
+                ac[1] += (v.m * bpv) * r;
+                ac[2] += (v.m * v.pfb) * (v.rb + v.pfb);
+                ac[3] += (v.m * bpv) * (v.rb + v.pfb);
+                ac[4] += (v.m * fpv) * (v.rf + v.pbf);
+                ac[5] += (v.m * v.pbf) * (v.rf + v.pbf);
+              });
         }
-        __syncthreads();
-        bool persist = dot_sharp(k, FP, RHO) > 0 && dot_sharp(k, BP, RHO) > 0;
-        persist &= dot_sharp2(k, PFB, RHOB, PFB) > 0 && dot_sharp2(k, BP, RHOB, PFB) > 0;
-        persist &= dot_sharp2(k, FP, RHOF, PBF) > 0 && dot_sharp2(k, PBF, RHOF, PBF) > 0;
+        block_sum_n<6>(k, acc);
+        bool persist = acc[0] > 0 && acc[1] > 0;
+        persist &= acc[2] > 0 && acc[3] > 0;
+        persist &= acc[4] > 0 && acc[5] > 0;
         __syncthreads();
         if (lane == 0) {
           s.depth++;
