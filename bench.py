@@ -283,6 +283,34 @@ def c1_section(with_cpu=True):
     return out
 
 
+def kernels_section(steps, sampler_iters=30):
+    """Short runs of the batched kernels alone, sized for a kernel trace:
+    `steps` batched evaluations of each batched workload (logistic N=1e6
+    K=100 with 256 chains and normal N=2e5 K=256 with 256 chains on BF16
+    hi+lo MFMA; hierarchical logistic J=1000 K=20 with 64 chains in FP32),
+    then `sampler_iters` warm-up + `sampler_iters` draws of the device
+    sampler on each.  Prints one JSON line of milliseconds per batched
+    evaluation; no CPU leg."""
+    import stan_amd as S
+    from stan_amd import workloads as W
+    out = {}
+    cases = [("logistic_n1e6_k100_c256", S.LOGIT, W.logistic_large, 256, S.BF16X2),
+             ("normal_n2e5_k256_c256", S.NORMAL, W.normal_batched, 256, S.BF16X2),
+             ("hier_logistic_j1000_k20_c64", S.HIER_LOGIT, W.hier_logistic, 64, S.FP32)]
+    for name, fam, gen, C, prec in cases:
+        log(f"kernels: {name}")
+        d = gen()
+        m = S.GlmModel(fam, d["X"], d["y"], offsets=d.get("offsets"), prior=d["prior"],
+                       precision=prec)
+        th = W.random_thetas(m.D, C, seed=7) * 0.5
+        ms, _ = m.time_log_prob_grad(th, iters=steps, warmup=3, chains=C)
+        r = batched_run(m, C, sampler_iters, sampler_iters, prec, max_depth=6)
+        out[name] = dict(ms_per_batched_eval=ms, chains=C, sampler_steps=r["steps"],
+                         sampler_ms_per_step=r["ms_per_step"])
+        m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,11 +324,16 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=16)
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--kernels", action="store_true",
+                    help="only short runs of the batched kernels (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(the multi-GPU design shards rows or chains, see DESIGN.md)"}))
         sys.exit(2)
+    if args.kernels:
+        print(json.dumps({"kernels": kernels_section(args.steps)}))
+        return
 
     import stan_amd as S
     from stan_amd import workloads as W

@@ -30,6 +30,10 @@
  *                          stan::services::sample::hmc_nuts_diag_e
  *                          (src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91):
  *                          the same transition without adaptation.
+ *   sx_hmc_nuts_dense_e_adapt(_cb, _csv, _cb_csv), sx_hmc_nuts_dense_e(_cb)
+ *                          the dense_e services
+ *                          (src/stan/services/sample/hmc_nuts_dense_e_adapt.hpp:57-112,
+ *                          src/stan/services/sample/hmc_nuts_dense_e.hpp:48-93).
  *
  * Conventions
  *   - Host buffers are caller-owned.  X (row-major N x K doubles) and y are
@@ -278,6 +282,40 @@ int sx_hmc_nuts_diag_e(sx_glm* m, const sx_nuts_config* cfg,
 int sx_hmc_nuts_diag_e_cb(sx_lpgrad_fn f, void* ctx, int D,
                           const sx_nuts_config* cfg, const double* init,
                           const double* init_inv_metric, sx_nuts_result* out);
+
+/* The dense-metric services:
+ *   sx_hmc_nuts_dense_e_adapt  stan::services::sample::hmc_nuts_dense_e_adapt
+ *                              (src/stan/services/sample/hmc_nuts_dense_e_adapt.hpp:57-112)
+ *   sx_hmc_nuts_dense_e        stan::services::sample::hmc_nuts_dense_e
+ *                              (src/stan/services/sample/hmc_nuts_dense_e.hpp:48-93)
+ * The same arguments as the diag_e entry points, except that
+ * init_inv_metric is a D x D matrix (row-major; NULL = identity, as
+ * util::create_unit_e_dense_inv_metric) that must be finite, symmetric to
+ * 1e-8 and positive definite (util::validate_dense_inv_metric; otherwise
+ * SX_CODE_CONFIG), and that out->inv_metric must hold D x D doubles: the
+ * metric adapted by windowed covariance estimation
+ * (src/stan/mcmc/covar_adaptation.hpp:17-47), or the fixed one.         */
+int sx_hmc_nuts_dense_e_adapt(sx_glm* m, const sx_nuts_config* cfg,
+                              const double* init, const double* init_inv_metric,
+                              sx_nuts_result* out);
+int sx_hmc_nuts_dense_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D,
+                                 const sx_nuts_config* cfg, const double* init,
+                                 const double* init_inv_metric, sx_nuts_result* out);
+int sx_hmc_nuts_dense_e_adapt_csv(sx_glm* m, const sx_nuts_config* cfg,
+                                  const double* init, const double* init_inv_metric,
+                                  const char* csv_path, int sig_figs,
+                                  sx_nuts_result* out);
+int sx_hmc_nuts_dense_e_adapt_cb_csv(sx_lpgrad_fn f, void* ctx, int D,
+                                     const sx_nuts_config* cfg, const double* init,
+                                     const double* init_inv_metric,
+                                     const char* csv_path, int sig_figs,
+                                     sx_nuts_result* out);
+int sx_hmc_nuts_dense_e(sx_glm* m, const sx_nuts_config* cfg,
+                        const double* init, const double* init_inv_metric,
+                        sx_nuts_result* out);
+int sx_hmc_nuts_dense_e_cb(sx_lpgrad_fn f, void* ctx, int D,
+                           const sx_nuts_config* cfg, const double* init,
+                           const double* init_inv_metric, sx_nuts_result* out);
 
 /* ---------------- batched device-resident sampler ---------------- */
 /* `chains` chains of adaptive diag_e NUTS on the device, chain ids

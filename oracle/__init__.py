@@ -44,6 +44,15 @@ def lib():
                                                 C.POINTER(C.c_double), C.c_void_p,
                                                 C.POINTER(C.c_int64)]
         L.orc_hmc_nuts_diag_e_adapt.restype = C.c_int
+        L.orc_hmc_nuts_dense_e_adapt.argtypes = L.orc_hmc_nuts_diag_e_adapt.argtypes
+        L.orc_hmc_nuts_dense_e_adapt.restype = C.c_int
+        for nm in ("orc_hmc_nuts_diag_e", "orc_hmc_nuts_dense_e"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                       C.c_void_p, C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]
+            getattr(L, nm).restype = C.c_int
+        L.orc_cholesky.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_cholesky.restype = C.c_int
         L.orc_leapfrog.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
         L.orc_leapfrog.restype = C.c_int
         L.orc_hamiltonian.argtypes = [C.c_void_p]
@@ -83,7 +92,8 @@ class GlmCtx(C.Structure):
 
 class Point(C.Structure):
     _fields_ = [("D", C.c_int), ("q", C.c_void_p), ("p", C.c_void_p),
-                ("g", C.c_void_p), ("minv", C.c_void_p), ("V", C.c_double)]
+                ("g", C.c_void_p), ("minv", C.c_void_p), ("V", C.c_double),
+                ("dense", C.c_int)]
 
 
 class Rng(C.Structure):
@@ -172,9 +182,7 @@ def default_args(**kw):
     return a
 
 
-def hmc_nuts_diag_e_adapt(fn_ptr, ctx, D, init=None, init_inv_metric=None, **kw):
-    """Run the oracle sampler.  Returns (rc, draws[rows, 7+D], stepsize,
-    inv_metric, n_grad)."""
+def _run(entry, fn_ptr, ctx, D, init, init_inv_metric, metric_size, adapt, kw):
     a = default_args(**kw)
     init_a = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
     minv_a = None if init_inv_metric is None else np.ascontiguousarray(init_inv_metric,
@@ -189,9 +197,47 @@ def hmc_nuts_diag_e_adapt(fn_ptr, ctx, D, init=None, init_inv_metric=None, **kw)
     draws = np.zeros((max(max_rows, 1), 7 + D))
     rows = C.c_int64(0)
     eps = C.c_double(0.0)
-    minv = np.zeros(D)
+    minv = np.zeros(metric_size)
     ng = C.c_int64(0)
-    rc = lib().orc_hmc_nuts_diag_e_adapt(fn_ptr, C.byref(ctx) if ctx is not None else None,
-                                         D, C.byref(args), _ptr(draws), C.byref(rows),
-                                         C.byref(eps), _ptr(minv), C.byref(ng))
-    return rc, draws[: rows.value], eps.value, minv, ng.value
+    cx = C.byref(ctx) if ctx is not None else None
+    if adapt:
+        rc = getattr(lib(), entry)(fn_ptr, cx, D, C.byref(args), _ptr(draws), C.byref(rows),
+                                   C.byref(eps), _ptr(minv), C.byref(ng))
+        return rc, draws[: rows.value], eps.value, minv, ng.value
+    rc = getattr(lib(), entry)(fn_ptr, cx, D, C.byref(args), _ptr(draws), C.byref(rows),
+                               C.byref(ng))
+    return rc, draws[: rows.value], ng.value
+
+
+def hmc_nuts_diag_e_adapt(fn_ptr, ctx, D, init=None, init_inv_metric=None, **kw):
+    """Run the oracle sampler.  Returns (rc, draws[rows, 7+D], stepsize,
+    inv_metric, n_grad)."""
+    return _run("orc_hmc_nuts_diag_e_adapt", fn_ptr, ctx, D, init, init_inv_metric, D, True,
+                kw)
+
+
+def hmc_nuts_dense_e_adapt(fn_ptr, ctx, D, init=None, init_inv_metric=None, **kw):
+    """Dense-metric adaptive sampler.  Returns (rc, draws, stepsize,
+    inv_metric[D, D], n_grad)."""
+    rc, d, eps, m, ng = _run("orc_hmc_nuts_dense_e_adapt", fn_ptr, ctx, D, init,
+                             init_inv_metric, D * D, True, kw)
+    return rc, d, eps, m.reshape(D, D), ng
+
+
+def hmc_nuts_diag_e(fn_ptr, ctx, D, init=None, init_inv_metric=None, **kw):
+    """Non-adaptive diag_e service.  Returns (rc, draws, n_grad)."""
+    return _run("orc_hmc_nuts_diag_e", fn_ptr, ctx, D, init, init_inv_metric, D, False, kw)
+
+
+def hmc_nuts_dense_e(fn_ptr, ctx, D, init=None, init_inv_metric=None, **kw):
+    """Non-adaptive dense_e service.  Returns (rc, draws, n_grad)."""
+    return _run("orc_hmc_nuts_dense_e", fn_ptr, ctx, D, init, init_inv_metric, D * D, False,
+                kw)
+
+
+def cholesky(A):
+    """Lower Cholesky factor of a symmetric positive-definite matrix, or
+    None if it is not positive definite."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    L = np.zeros_like(A)
+    return L if lib().orc_cholesky(_ptr(A), A.shape[0], _ptr(L)) == 0 else None

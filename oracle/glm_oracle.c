@@ -373,3 +373,30 @@ int orc_funnel_model_cb(void* ctx, const double* q, int D, double* lp,
   *lp = lpv;
   return 0;
 }
+
+/* Correlated Gaussian test model: x ~ N(0, S) with S_ij = s_i s_j rho^|i-j|
+ * (AR(1) correlation, rho = 0.9, scales s_i = 1 + i), whose precision is
+ * tridiagonal: with z_i = x_i / s_i, -2 log p = (sum_i z_i^2 (1 + rho^2 c_i)
+ * - 2 rho sum_i z_i z_{i+1}) / (1 - rho^2), c_i = 1 for interior i, 0 at the
+ * two ends.  A dense metric equal to S makes the problem isotropic. */
+int orc_corr_gauss_model_cb(void* ctx, const double* q, int D, double* lp,
+                            double* grad) {
+  (void)ctx;
+  const double rho = 0.9, w = 1.0 / (1.0 - rho * rho);
+  double s = 0.0;
+  for (int i = 0; i < D; ++i) {
+    const double si = 1.0 + i, zi = q[i] / si;
+    const double di = (i == 0 || i == D - 1) ? 1.0 : 1.0 + rho * rho;
+    double gz = di * zi;
+    s += di * zi * zi;
+    if (i + 1 < D) {
+      const double zn = q[i + 1] / (2.0 + i);
+      s -= 2.0 * rho * zi * zn;
+      gz -= rho * zn;
+    }
+    if (i > 0) gz -= rho * q[i - 1] / (double)i;
+    grad[i] = -w * gz / si;
+  }
+  *lp = -0.5 * w * s;
+  return 0;
+}

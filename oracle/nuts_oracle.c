@@ -1,8 +1,8 @@
 /*
  * nuts_oracle.c -- test infrastructure (checker only): recursive multinomial
- * NUTS with a diagonal Euclidean metric, its adaptation and the service
- * driver, restated line by line from the reference (files listed in
- * oracle.h).  This is the "reference sampler": posterior moments of the
+ * NUTS with a diagonal or dense Euclidean metric, its adaptation and the
+ * service driver, restated line by line from the reference (files listed
+ * in oracle.h).  This is the "reference sampler": posterior moments of the
  * product's samplers are compared with runs of this code, and its speed on
  * host cores is the CPU baseline.
  */
@@ -33,8 +33,9 @@ static double dot(const double* a, const double* b, int D) {
 }
 
 /* ---------------- point helpers ---------------- */
-static void pt_alloc(orc_point* z, int D, double* minv) {
+static void pt_alloc(orc_point* z, int D, double* minv, int dense) {
   z->D = D;
+  z->dense = dense;
   z->q = (double*)calloc((size_t)D + 1, sizeof(double));
   z->p = (double*)calloc((size_t)D + 1, sizeof(double));
   z->g = (double*)calloc((size_t)D + 1, sizeof(double));
@@ -54,14 +55,68 @@ static void pt_copy(orc_point* dst, const orc_point* src) {
 }
 static double* vec(int D) { return (double*)calloc((size_t)D + 1, sizeof(double)); }
 
-/* diag_e_metric::T (diag_e_metric.hpp:20-22) */
+/* p_sharp = M^-1 p: diag_e_metric::dtau_dp (diag_e_metric.hpp:36-38)
+ * elementwise; dense_e_metric::dtau_dp (dense_e_metric.hpp:37) as a
+ * matrix-vector product over the row-major matrix, j ascending.        */
+static void dtau_dp(const orc_point* z, double* out) {
+  if (z->dense) {
+    for (int i = 0; i < z->D; ++i) {
+      const double* row = z->minv + (size_t)i * (size_t)z->D;
+      double s = 0.0;
+      for (int j = 0; j < z->D; ++j) s += row[j] * z->p[j];
+      out[i] = s;
+    }
+    return;
+  }
+  for (int i = 0; i < z->D; ++i) out[i] = z->minv[i] * z->p[i];
+}
+/* H = T + V with T = 1/2 p . (M^-1 p): diag_e_metric::T
+ * (diag_e_metric.hpp:20-22), dense_e_metric::T (dense_e_metric.hpp:21-23) */
 double orc_hamiltonian(const orc_point* z) {
   double t = 0.0;
-  for (int i = 0; i < z->D; ++i) t += z->p[i] * (z->minv[i] * z->p[i]);
+  if (z->dense) {
+    for (int i = 0; i < z->D; ++i) {
+      const double* row = z->minv + (size_t)i * (size_t)z->D;
+      double s = 0.0;
+      for (int j = 0; j < z->D; ++j) s += row[j] * z->p[j];
+      t += z->p[i] * s;
+    }
+  } else {
+    for (int i = 0; i < z->D; ++i) t += z->p[i] * (z->minv[i] * z->p[i]);
+  }
   return 0.5 * t + z->V;
 }
-static void dtau_dp(const orc_point* z, double* out) {
-  for (int i = 0; i < z->D; ++i) out[i] = z->minv[i] * z->p[i];
+
+/* Cholesky factor A = L L^T of a row-major symmetric matrix (lower L,
+ * row-major; column by column, k ascending in every sum).  Returns 0, or
+ * -1 when a pivot is not positive (A not positive definite).           */
+int orc_cholesky(const double* A, int D, double* L) {
+  memset(L, 0, sizeof(double) * (size_t)D * (size_t)D);
+  for (int j = 0; j < D; ++j) {
+    double s = A[(size_t)j * D + j];
+    for (int k = 0; k < j; ++k) s -= L[(size_t)j * D + k] * L[(size_t)j * D + k];
+    if (!(s > 0.0)) return -1;
+    const double d = sqrt(s);
+    L[(size_t)j * D + j] = d;
+    for (int i = j + 1; i < D; ++i) {
+      double t = A[(size_t)i * D + j];
+      for (int k = 0; k < j; ++k) t -= L[(size_t)i * D + k] * L[(size_t)j * D + k];
+      L[(size_t)i * D + j] = t / d;
+    }
+  }
+  return 0;
+}
+
+/* dense_e_metric::sample_p (dense_e_metric.hpp:43-54): u_i ~ N(0,1), then
+ * p = U^-1 u with U the upper Cholesky factor of M^-1 (M^-1 = U^T U,
+ * U = L^T), by back substitution (j ascending in each row's sum).  p then
+ * has covariance (U^T U)^-1 = M.  L is the factor of the current metric. */
+static void solve_upper(const double* L, int D, const double* u, double* p) {
+  for (int i = D - 1; i >= 0; --i) {
+    double t = u[i];
+    for (int j = i + 1; j < D; ++j) t -= L[(size_t)j * D + i] * p[j];
+    p[i] = t / L[(size_t)i * D + i];
+  }
 }
 
 /* base_hamiltonian::update_potential_gradient (base_hamiltonian.hpp:61-70).
@@ -91,7 +146,14 @@ static int evolve(orc_lpgrad_fn f, void* ctx, orc_point* z, double eps,
   const int D = z->D;
   const double he = 0.5 * eps;
   for (int i = 0; i < D; ++i) z->p[i] -= he * z->g[i];
-  for (int i = 0; i < D; ++i) z->q[i] += eps * (z->minv[i] * z->p[i]);
+  if (z->dense) {
+    double* ps = vec(D);
+    dtau_dp(z, ps);
+    for (int i = 0; i < D; ++i) z->q[i] += eps * ps[i];
+    free(ps);
+  } else {
+    for (int i = 0; i < D; ++i) z->q[i] += eps * (z->minv[i] * z->p[i]);
+  }
   int st = update_potential_gradient(f, ctx, z, n_grad);
   if (st) return st;
   for (int i = 0; i < D; ++i) z->p[i] -= he * z->g[i];
@@ -114,7 +176,9 @@ typedef struct sampler {
   int depth, n_leapfrog, divergent;
   double energy;
   orc_point z;
-  double* minv;
+  double* minv;  /* D (diag) or D x D row-major (dense) */
+  int dense;
+  double* chol;  /* dense: Cholesky factor of minv, refreshed with it */
   int64_t n_grad;
   int fatal;
   /* stepsize_adaptation */
@@ -172,7 +236,7 @@ static int build_tree_impl(sampler* s, int depth, orc_point* z_propose,
                                    &log_sum_weight_init, sum_metro_prob);
   int result = 0;
   orc_point z_propose_final;
-  pt_alloc(&z_propose_final, D, s->minv);
+  pt_alloc(&z_propose_final, D, s->minv, s->dense);
   double* p_final_beg = vec(D);
   double* p_sharp_final_beg = vec(D);
   double* rho_final = vec(D);
@@ -222,7 +286,7 @@ int orc_build_tree(orc_lpgrad_fn f, void* ctx, orc_rng* rng, double eps,
   sampler s;
   memset(&s, 0, sizeof(s));
   s.f = f; s.ctx = ctx; s.D = z->D; s.rng = *rng; s.eps = eps;
-  s.max_deltaH = max_deltaH; s.minv = z->minv;
+  s.max_deltaH = max_deltaH; s.minv = z->minv; s.dense = z->dense;
   s.z = *z; /* shares storage with the caller's point */
   s.divergent = *divergent;
   int v = build_tree_impl(&s, depth, z_propose, p_sharp_beg, p_sharp_end,
@@ -235,6 +299,13 @@ int orc_build_tree(orc_lpgrad_fn f, void* ctx, orc_rng* rng, double eps,
 }
 
 static void sample_p(sampler* s, orc_point* z) {
+  if (s->dense) {
+    double* u = vec(s->D);
+    for (int i = 0; i < s->D; ++i) u[i] = orc_rng_normal(&s->rng);
+    solve_upper(s->chol, s->D, u, z->p);
+    free(u);
+    return;
+  }
   for (int i = 0; i < s->D; ++i)
     z->p[i] = orc_rng_normal(&s->rng) / sqrt(z->minv[i]);
 }
@@ -259,10 +330,10 @@ static int nuts_transition(sampler* s, double* q_inout, double* out_lp,
   if (st) return st;
 
   orc_point z_fwd, z_bck, z_sample, z_propose;
-  pt_alloc(&z_fwd, D, s->minv); pt_copy(&z_fwd, z);
-  pt_alloc(&z_bck, D, s->minv); pt_copy(&z_bck, &z_fwd);
-  pt_alloc(&z_sample, D, s->minv); pt_copy(&z_sample, &z_fwd);
-  pt_alloc(&z_propose, D, s->minv); pt_copy(&z_propose, &z_fwd);
+  pt_alloc(&z_fwd, D, s->minv, s->dense); pt_copy(&z_fwd, z);
+  pt_alloc(&z_bck, D, s->minv, s->dense); pt_copy(&z_bck, &z_fwd);
+  pt_alloc(&z_sample, D, s->minv, s->dense); pt_copy(&z_sample, &z_fwd);
+  pt_alloc(&z_propose, D, s->minv, s->dense); pt_copy(&z_propose, &z_fwd);
 
   double* p_fwd_fwd = vec(D); memcpy(p_fwd_fwd, z->p, sizeof(double) * D);
   double* p_sharp_fwd_fwd = vec(D); dtau_dp(z, p_sharp_fwd_fwd);
@@ -357,7 +428,7 @@ static int init_stepsize(sampler* s) {
   if (D == 0) { s->nom_eps = NAN; return 0; }
   if (s->nom_eps == 0 || s->nom_eps > 1e7 || isnan(s->nom_eps)) return 0;
   orc_point z_init;
-  pt_alloc(&z_init, D, s->minv);
+  pt_alloc(&z_init, D, s->minv, s->dense);
   pt_copy(&z_init, z);
   int st;
   sample_p(s, z);
@@ -485,14 +556,62 @@ static int learn_variance(sampler* s, double* var, const double* q) {
   return 0;
 }
 
-/* adapt_diag_e_nuts::transition (adapt_diag_e_nuts.hpp:25-43) */
+/* covar_adaptation::learn_covariance (covar_adaptation.hpp:17-47) with the
+ * Welford running covariance of Stan Math's welford_covar_estimator
+ * (absent from the tree; restated from its definition: delta = q - m,
+ * m += delta / n, m2 += (q - m) delta^T, covariance m2 / (n - 1)).  The
+ * regularised estimate n/(n+5) C + 1e-3 5/(n+5) I replaces the metric
+ * (row-major) and its Cholesky factor.  Returns 1 when the metric was
+ * updated, 0 if not, -1 on a non-finite metric.                          */
+static int learn_covariance(sampler* s, double* covar, const double* q) {
+  const int D = s->D;
+  if (wa_adaptation_window(s)) {
+    s->wn += 1.0;
+    double* delta = vec(D);
+    for (int i = 0; i < D; ++i) delta[i] = q[i] - s->wm[i];
+    for (int i = 0; i < D; ++i) s->wm[i] += delta[i] / s->wn;
+    for (int i = 0; i < D; ++i) {
+      const double d = q[i] - s->wm[i];
+      for (int j = 0; j < D; ++j) s->wm2[(size_t)i * D + j] += d * delta[j];
+    }
+    free(delta);
+  }
+  if (wa_end_adaptation_window(s)) {
+    wa_compute_next_window(s);
+    const size_t DD = (size_t)D * (size_t)D;
+    if (s->wn > 1)
+      for (size_t k = 0; k < DD; ++k) covar[k] = s->wm2[k] / (s->wn - 1.0);
+    double n = s->wn;
+    const double a = n / (n + 5.0), b = 1e-3 * (5.0 / (n + 5.0));
+    int finite = 1;
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        double* c = &covar[(size_t)i * D + j];
+        *c = a * *c + b * (i == j ? 1.0 : 0.0);
+        if (!isfinite(*c)) finite = 0;
+      }
+    if (!finite) return -1;
+    if (orc_cholesky(covar, D, s->chol) != 0) return -1;
+    s->wn = 0;
+    memset(s->wm, 0, sizeof(double) * D);
+    memset(s->wm2, 0, sizeof(double) * DD);
+    ++s->window_counter;
+    return 1;
+  }
+  ++s->window_counter;
+  return 0;
+}
+
+/* adapt_diag_e_nuts::transition (adapt_diag_e_nuts.hpp:25-43) and
+ * adapt_dense_e_nuts::transition (adapt_dense_e_nuts.hpp:25-43) */
 static int adaptive_transition(sampler* s, double* q, double* lp,
                                double* accept) {
   int st = nuts_transition(s, q, lp, accept);
   if (st) return st;
   if (s->adapt_flag) {
     orc_learn_stepsize(s->sa, &s->nom_eps, *accept);
-    int update = learn_variance(s, s->minv, s->z.q);
+    int update = s->dense ? learn_covariance(s, s->minv, s->z.q)
+                          : learn_variance(s, s->minv, s->z.q);
     if (update < 0) return -1;
     if (update) {
       st = init_stepsize(s);
@@ -516,15 +635,19 @@ static void write_row(const sampler* s, const double* q, double lp,
   memcpy(row + 7, q, sizeof(double) * s->D);
 }
 
-int orc_hmc_nuts_diag_e_adapt(orc_lpgrad_fn f, void* ctx, int D,
-                              const orc_nuts_args* a, double* out_draws,
-                              int64_t* out_rows, double* out_stepsize,
-                              double* out_inv_metric, int64_t* n_grad) {
+/* The services: hmc_nuts_diag_e_adapt (hmc_nuts_diag_e_adapt.hpp:57-112),
+ * hmc_nuts_dense_e_adapt (hmc_nuts_dense_e_adapt.hpp:57-112) through
+ * run_adaptive_sampler, and hmc_nuts_dense_e (hmc_nuts_dense_e.hpp:48-93) through
+ * run_sampler (no step-size heuristic, no adaptation). */
+static int run_service(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                       int dense, int adapt, double* out_draws, int64_t* out_rows,
+                       double* out_stepsize, double* out_inv_metric, int64_t* n_grad) {
   sampler s;
   memset(&s, 0, sizeof(s));
-  s.f = f; s.ctx = ctx; s.D = D;
+  s.f = f; s.ctx = ctx; s.D = D; s.dense = dense;
   orc_rng_init(&s.rng, a->seed, a->chain);
   *out_rows = 0;
+  const size_t DD = dense ? (size_t)D * (size_t)D : (size_t)D;
   double* q = vec(D);
   double* gbuf = vec(D);
   /* util::initialize (initialize.hpp:68-227) */
@@ -552,15 +675,35 @@ int orc_hmc_nuts_diag_e_adapt(orc_lpgrad_fn f, void* ctx, int D,
   }
   free(gbuf);
   if (!ok) { free(q); return 78; }
-  s.minv = vec(D);
-  for (int i = 0; i < D; ++i) {
-    s.minv[i] = a->init_inv_metric ? a->init_inv_metric[i] : 1.0;
-    if (!(s.minv[i] > 0) || !isfinite(s.minv[i])) {
-      free(q); free(s.minv); return 78;
+  s.minv = (double*)calloc(DD + 1, sizeof(double));
+  if (dense) {
+    /* read_dense_inv_metric + validate_dense_inv_metric: symmetric (to
+     * 1e-8), finite and positive definite, else CONFIG */
+    s.chol = (double*)calloc(DD + 1, sizeof(double));
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j)
+        s.minv[(size_t)i * D + j] =
+            a->init_inv_metric ? a->init_inv_metric[(size_t)i * D + j] : (i == j ? 1.0 : 0.0);
+    int bad = 0;
+    for (int i = 0; i < D && !bad; ++i)
+      for (int j = 0; j < D; ++j) {
+        const double x = s.minv[(size_t)i * D + j], y = s.minv[(size_t)j * D + i];
+        if (!isfinite(x) || fabs(x - y) > 1e-8) { bad = 1; break; }
+      }
+    if (bad || orc_cholesky(s.minv, D, s.chol) != 0) {
+      free(q); free(s.minv); free(s.chol); return 78;
+    }
+  } else {
+    for (int i = 0; i < D; ++i) {
+      s.minv[i] = a->init_inv_metric ? a->init_inv_metric[i] : 1.0;
+      if (!(s.minv[i] > 0) || !isfinite(s.minv[i])) {
+        free(q); free(s.minv); return 78;
+      }
     }
   }
-  pt_alloc(&s.z, D, s.minv);
-  s.wm = vec(D); s.wm2 = vec(D);
+  pt_alloc(&s.z, D, s.minv, s.dense);
+  s.wm = vec(D);
+  s.wm2 = (double*)calloc(DD + 1, sizeof(double));
   s.nom_eps = 0.1; /* base_hmc default nom_epsilon_ */
   if (a->stepsize > 0) s.nom_eps = a->stepsize;
   if (a->stepsize_jitter > 0 && a->stepsize_jitter < 1) s.jitter = a->stepsize_jitter;
@@ -579,21 +722,24 @@ int orc_hmc_nuts_diag_e_adapt(orc_lpgrad_fn f, void* ctx, int D,
                 a->window);
 
   int rc = 0;
-  /* run_adaptive_sampler (run_adaptive_sampler.hpp:46-103) */
-  s.adapt_flag = 1;
-  memcpy(s.z.q, q, sizeof(double) * D);
-  if (init_stepsize(&s) != 0) goto out; /* logged and returned in the reference */
   const int ncol = 7 + D;
   int64_t rows = 0;
   double lp = 0, acc = 0;
+  memcpy(s.z.q, q, sizeof(double) * D);
+  /* run_adaptive_sampler (run_adaptive_sampler.hpp:46-103): the step-size
+   * heuristic first; run_sampler (run_sampler.hpp:39-85) has none */
+  s.adapt_flag = adapt;
+  if (adapt && init_stepsize(&s) != 0) goto out; /* logged and returned in the reference */
   for (int m = 0; m < a->num_warmup; ++m) {
     int st = adaptive_transition(&s, q, &lp, &acc);
     if (st) { rc = 70; goto out; }
     if (a->save_warmup && (m % a->num_thin) == 0)
       write_row(&s, q, lp, acc, out_draws + (rows++) * ncol);
   }
-  s.adapt_flag = 0;
-  s.nom_eps = exp(s.sa[2]); /* complete_adaptation */
+  if (adapt) {
+    s.adapt_flag = 0;
+    s.nom_eps = exp(s.sa[2]); /* complete_adaptation */
+  }
   for (int m = 0; m < a->num_samples; ++m) {
     int st = adaptive_transition(&s, q, &lp, &acc);
     if (st) { rc = 70; goto out; }
@@ -603,9 +749,30 @@ int orc_hmc_nuts_diag_e_adapt(orc_lpgrad_fn f, void* ctx, int D,
   *out_rows = rows;
 out:
   if (out_stepsize) *out_stepsize = s.nom_eps;
-  if (out_inv_metric) memcpy(out_inv_metric, s.minv, sizeof(double) * D);
+  if (out_inv_metric) memcpy(out_inv_metric, s.minv, sizeof(double) * DD);
   if (n_grad) *n_grad = s.n_grad;
   pt_free(&s.z);
-  free(s.wm); free(s.wm2); free(s.minv); free(q);
+  free(s.wm); free(s.wm2); free(s.minv); free(s.chol); free(q);
   return rc;
+}
+
+int orc_hmc_nuts_diag_e_adapt(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                              double* out_draws, int64_t* out_rows, double* out_stepsize,
+                              double* out_inv_metric, int64_t* n_grad) {
+  return run_service(f, ctx, D, a, 0, 1, out_draws, out_rows, out_stepsize, out_inv_metric,
+                     n_grad);
+}
+int orc_hmc_nuts_diag_e(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                        double* out_draws, int64_t* out_rows, int64_t* n_grad) {
+  return run_service(f, ctx, D, a, 0, 0, out_draws, out_rows, NULL, NULL, n_grad);
+}
+int orc_hmc_nuts_dense_e_adapt(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                               double* out_draws, int64_t* out_rows, double* out_stepsize,
+                               double* out_inv_metric, int64_t* n_grad) {
+  return run_service(f, ctx, D, a, 1, 1, out_draws, out_rows, out_stepsize, out_inv_metric,
+                     n_grad);
+}
+int orc_hmc_nuts_dense_e(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                         double* out_draws, int64_t* out_rows, int64_t* n_grad) {
+  return run_service(f, ctx, D, a, 1, 0, out_draws, out_rows, NULL, NULL, n_grad);
 }

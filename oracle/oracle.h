@@ -18,7 +18,17 @@
  *       src/stan/mcmc/var_adaptation.hpp:17-46 (Welford estimator from Stan
  *       Math, absent; restated from Welford 1962)
  *   - adaptive transition: src/stan/mcmc/hmc/nuts/adapt_diag_e_nuts.hpp:25-48
+ *   - dense_e metric: src/stan/mcmc/hmc/hamiltonians/dense_e_metric.hpp:21-54
+ *       (Cholesky factor and triangular solve restated from their
+ *       definitions; Eigen's LLT is absent), windowed covariance adaptation
+ *       src/stan/mcmc/covar_adaptation.hpp:17-47 (Welford covariance
+ *       estimator from Stan Math, absent), adaptive transition
+ *       src/stan/mcmc/hmc/nuts/adapt_dense_e_nuts.hpp:25-48, metric checks
+ *       src/stan/services/util/validate_dense_inv_metric.hpp:18-28
  *   - service driver: src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:57-112,
+ *       src/stan/services/sample/hmc_nuts_dense_e_adapt.hpp:57-112,
+ *       src/stan/services/sample/hmc_nuts_dense_e.hpp:48-93,
+ *       src/stan/services/util/run_sampler.hpp:39-85,
  *       src/stan/services/util/run_adaptive_sampler.hpp:46-103,
  *       src/stan/services/util/generate_transitions.hpp:42-77,
  *       src/stan/services/util/initialize.hpp:68-227
@@ -102,7 +112,8 @@ typedef struct orc_nuts_args {
   double delta, gamma, kappa, t0;
   unsigned init_buffer, term_buffer, window;
   const double* init;          /* optional D-vector of unconstrained inits */
-  const double* init_inv_metric; /* optional D-vector */
+  const double* init_inv_metric; /* optional: D-vector (diag), or D x D
+                                    row-major (dense) */
 } orc_nuts_args;
 
 /* Draw table: one row per saved iteration, columns
@@ -116,12 +127,28 @@ int orc_hmc_nuts_diag_e_adapt(orc_lpgrad_fn f, void* ctx, int D,
                               const orc_nuts_args* a, double* out_draws,
                               int64_t* out_rows, double* out_stepsize,
                               double* out_inv_metric, int64_t* n_grad);
+/* hmc_nuts_diag_e: the same transitions without adaptation (no step-size
+ * heuristic; stepsize and metric fixed). */
+int orc_hmc_nuts_diag_e(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                        double* out_draws, int64_t* out_rows, int64_t* n_grad);
+/* hmc_nuts_dense_e_adapt / hmc_nuts_dense_e: dense metric (D x D row-major
+ * init_inv_metric, identity if NULL; out_inv_metric receives D x D). */
+int orc_hmc_nuts_dense_e_adapt(orc_lpgrad_fn f, void* ctx, int D,
+                               const orc_nuts_args* a, double* out_draws,
+                               int64_t* out_rows, double* out_stepsize,
+                               double* out_inv_metric, int64_t* n_grad);
+int orc_hmc_nuts_dense_e(orc_lpgrad_fn f, void* ctx, int D, const orc_nuts_args* a,
+                         double* out_draws, int64_t* out_rows, int64_t* n_grad);
+/* Cholesky factor A = L L^T (row-major, lower L); 0, or -1 if A is not
+ * positive definite. */
+int orc_cholesky(const double* A, int D, double* L);
 
 /* Single pieces exposed for golden tests. */
 typedef struct orc_point {
   int D;
-  double *q, *p, *g, *minv;
+  double *q, *p, *g, *minv; /* minv: D (diag) or D x D row-major (dense) */
   double V;
+  int dense;
 } orc_point;
 
 /* leapfrog on a point with a model callback; returns 0 or fatal code */

@@ -29,6 +29,8 @@ EXPORTED = [
     "sx_server_destroy", "sx_analyze_ess", "sx_hmc_nuts_diag_e_adapt_csv",
     "sx_hmc_nuts_diag_e_adapt_cb_csv", "sx_glm_create_json", "sx_json_var", "sx_nuts_batch_run",
     "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
+    "sx_hmc_nuts_dense_e_adapt", "sx_hmc_nuts_dense_e_adapt_cb", "sx_hmc_nuts_dense_e_adapt_csv",
+    "sx_hmc_nuts_dense_e", "sx_hmc_nuts_dense_e_cb", "sx_hmc_nuts_dense_e_adapt_cb_csv",
 ]
 
 _lib = None
@@ -131,6 +133,12 @@ def lib():
         L.sx_hmc_nuts_diag_e_adapt_cb_csv.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                                       vp, vp, C.c_char_p, C.c_int,
                                                       C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_dense_e_adapt.argtypes = L.sx_hmc_nuts_diag_e_adapt.argtypes
+        L.sx_hmc_nuts_dense_e_adapt_cb.argtypes = L.sx_hmc_nuts_diag_e_adapt_cb.argtypes
+        L.sx_hmc_nuts_dense_e_adapt_csv.argtypes = L.sx_hmc_nuts_diag_e_adapt_csv.argtypes
+        L.sx_hmc_nuts_dense_e_adapt_cb_csv.argtypes = L.sx_hmc_nuts_diag_e_adapt_cb_csv.argtypes
+        L.sx_hmc_nuts_dense_e.argtypes = L.sx_hmc_nuts_diag_e.argtypes
+        L.sx_hmc_nuts_dense_e_cb.argtypes = L.sx_hmc_nuts_diag_e_cb.argtypes
         L.sx_glm_create_json.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int, C.c_int,
                                          C.POINTER(Prior), C.c_int]
         L.sx_json_var.argtypes = [C.c_char_p, C.c_char_p, vp, i64, C.POINTER(i64), C.c_int,
@@ -283,10 +291,10 @@ def nuts_config(**kw):
     return NutsConfig(**a)
 
 
-def _run_service(call, D, cfg, init, inv_metric):
+def _run_service(call, D, cfg, init, inv_metric, dense=False):
     rows_cap = (cfg.num_warmup if cfg.save_warmup else 0) + cfg.num_samples
     draws = np.zeros((max(rows_cap, 1), 7 + D))
-    minv = np.zeros(D)
+    minv = np.zeros((D, D) if dense else D)
     res = NutsResult(_ptr(draws).value, 0, 0.0, _ptr(minv).value, 0, 0.0, 0.0)
     init_a = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
     minv_a = None if inv_metric is None else np.ascontiguousarray(inv_metric,
@@ -361,6 +369,67 @@ def hmc_nuts_diag_e_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
         lambda i, m, r: lib().sx_hmc_nuts_diag_e_cb(fn_ptr, ctx, D, C.byref(cfg), _ptr(i),
                                                     _ptr(m), C.byref(r)),
         D, cfg, init, inv_metric)
+
+
+def hmc_nuts_dense_e_adapt(model, init=None, inv_metric=None, **kw):
+    """Adaptive NUTS with a dense metric on a GlmModel: inv_metric (D x D,
+    identity if None) is adapted by windowed covariance estimation; the
+    returned inv_metric is D x D.  Otherwise as hmc_nuts_diag_e_adapt."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_dense_e_adapt(model.handle, C.byref(cfg), _ptr(i),
+                                                        _ptr(m), C.byref(r)),
+        model.D, cfg, init, inv_metric, dense=True)
+
+
+def hmc_nuts_dense_e_adapt_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
+    """hmc_nuts_dense_e_adapt for a model given as a C callback."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_dense_e_adapt_cb(fn_ptr, ctx, D, C.byref(cfg),
+                                                           _ptr(i), _ptr(m), C.byref(r)),
+        D, cfg, init, inv_metric, dense=True)
+
+
+def hmc_nuts_dense_e_adapt_csv(model, path, sig_figs=-1, init=None, inv_metric=None, **kw):
+    """hmc_nuts_dense_e_adapt writing a Stan CSV file at `path`."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_dense_e_adapt_csv(model.handle, C.byref(cfg),
+                                                            _ptr(i), _ptr(m),
+                                                            str(path).encode(),
+                                                            int(sig_figs), C.byref(r)),
+        model.D, cfg, init, inv_metric, dense=True)
+
+
+def hmc_nuts_dense_e_adapt_cb_csv(fn_ptr, ctx, D, path, sig_figs=-1, init=None,
+                                  inv_metric=None, **kw):
+    """The dense CSV-writing service for a model given as a C callback."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_dense_e_adapt_cb_csv(fn_ptr, ctx, D, C.byref(cfg),
+                                                               _ptr(i), _ptr(m),
+                                                               str(path).encode(),
+                                                               int(sig_figs), C.byref(r)),
+        D, cfg, init, inv_metric, dense=True)
+
+
+def hmc_nuts_dense_e(model, init=None, inv_metric=None, **kw):
+    """Dense-metric NUTS without adaptation on a GlmModel."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_dense_e(model.handle, C.byref(cfg), _ptr(i), _ptr(m),
+                                                  C.byref(r)),
+        model.D, cfg, init, inv_metric, dense=True)
+
+
+def hmc_nuts_dense_e_cb(fn_ptr, ctx, D, init=None, inv_metric=None, **kw):
+    """hmc_nuts_dense_e for a model given as a C callback."""
+    cfg = nuts_config(**kw)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_dense_e_cb(fn_ptr, ctx, D, C.byref(cfg), _ptr(i),
+                                                     _ptr(m), C.byref(r)),
+        D, cfg, init, inv_metric, dense=True)
 
 
 class NutsBatch:

@@ -1,6 +1,7 @@
-// C-ABI of the sampler services: runs the C++ diag_e NUTS, adaptive or
-// not (stanx/services.hpp), on a GPU-backed GLM model or on a model given
-// as a C callback, and returns the draws as a table.
+// C-ABI of the sampler services: runs the C++ NUTS with a diag_e or a
+// dense_e metric, adaptive or not (stanx/services.hpp), on a GPU-backed GLM
+// model or on a model given as a C callback, and returns the draws as a
+// table.
 #include <cstring>
 #include <exception>
 #include <fstream>
@@ -37,7 +38,7 @@ class tee_writer : public stanx::callbacks::writer {
 // the configuration block at the top of a Stan CSV file, in CmdStan's
 // layout (key = value lines, nesting by indentation)
 void write_csv_config(std::ostream& o, const std::string& model_name, const sx_nuts_config* c,
-                      bool adapt, int sig_figs) {
+                      bool adapt, bool dense, int sig_figs) {
   o << "# stan_version_major = 2\n# stan_version_minor = 35\n# stan_version_patch = 0\n";
   o << "# model = " << model_name << "\n";
   o << "# method = sample (Default)\n#   sample\n";
@@ -52,7 +53,7 @@ void write_csv_config(std::ostream& o, const std::string& model_name, const sx_n
     << c->term_buffer << "\n#       window = " << c->window << "\n";
   o << "#     algorithm = hmc (Default)\n#       hmc\n#         engine = nuts (Default)\n";
   o << "#           nuts\n#             max_depth = " << c->max_depth << "\n";
-  o << "#         metric = diag_e (Default)\n";
+  o << (dense ? "#         metric = dense_e\n" : "#         metric = diag_e (Default)\n");
   o << "#         stepsize = " << c->stepsize << "\n";
   o << "#         stepsize_jitter = " << c->stepsize_jitter << "\n";
   o << "#     num_chains = 1 (Default)\n";
@@ -64,16 +65,18 @@ void write_csv_config(std::ostream& o, const std::string& model_name, const sx_n
 template <class Model>
 int run_service(const Model& model, const sx_nuts_config* cfg, const double* init,
                 const double* init_inv_metric, sx_nuts_result* out, bool adapt = true,
-                const char* csv_path = nullptr, int sig_figs = -1) {
+                const char* csv_path = nullptr, int sig_figs = -1, bool dense = false) {
+  const char* name = dense ? (adapt ? "sx_hmc_nuts_dense_e_adapt" : "sx_hmc_nuts_dense_e")
+                           : (adapt ? "sx_hmc_nuts_diag_e_adapt" : "sx_hmc_nuts_diag_e");
   if (!cfg || !out || !out->draws) {
-    sx::set_error(adapt ? "sx_hmc_nuts_diag_e_adapt: NULL argument"
-                        : "sx_hmc_nuts_diag_e: NULL argument");
+    sx::set_error(std::string(name) + ": NULL argument");
     return SX_CODE_CONFIG;
   }
   const size_t D = model.num_params_r();
+  const size_t M = dense ? D * D : D;  // metric size
   std::vector<double> init_v, minv_v;
   if (init) init_v.assign(init, init + D);
-  if (init_inv_metric) minv_v.assign(init_inv_metric, init_inv_metric + D);
+  if (init_inv_metric) minv_v.assign(init_inv_metric, init_inv_metric + M);
   stanx::callbacks::interrupt interrupt;
   stanx::callbacks::counting_logger logger;
   stanx::callbacks::writer init_writer, diag_writer;
@@ -86,7 +89,7 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
       return SX_CODE_CONFIG;
     }
     if (sig_figs > 0) csv << std::setprecision(sig_figs);
-    write_csv_config(csv, model.model_name(), cfg, adapt, sig_figs);
+    write_csv_config(csv, model.model_name(), cfg, adapt, dense, sig_figs);
   }
   stanx::callbacks::stream_writer csv_writer(csv, "# ");
   stanx::callbacks::writer null_writer;
@@ -96,25 +99,45 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
   std::vector<double> minv_out;
   long long n_grad = 0;
   int rc = SX_CODE_SOFTWARE;
+  const int thin = cfg->num_thin > 0 ? cfg->num_thin : 1;
+  const std::vector<double>* iv = init ? &init_v : nullptr;
+  const std::vector<double>* mv = init_inv_metric ? &minv_v : nullptr;
   try {
-    if (adapt) {
-      rc = stanx::services::sample::hmc_nuts_diag_e_adapt(
-          model, init ? &init_v : nullptr, init_inv_metric ? &minv_v : nullptr,
-          cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
-          cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
-          cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma,
-          cfg->kappa, cfg->t0, cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt,
-          logger, init_writer, out_writer, diag_writer, &eps, &minv_out, &n_grad, &warm,
-          &samp);
+    namespace smp = stanx::services::sample;
+    if (adapt && !dense) {
+      rc = smp::hmc_nuts_diag_e_adapt(
+          model, iv, mv, cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup,
+          cfg->num_samples, thin, cfg->save_warmup != 0, cfg->refresh, cfg->stepsize,
+          cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma, cfg->kappa, cfg->t0,
+          cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt, logger, init_writer,
+          out_writer, diag_writer, &eps, &minv_out, &n_grad, &warm, &samp);
+    } else if (adapt) {
+      rc = smp::hmc_nuts_dense_e_adapt(
+          model, iv, mv, cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup,
+          cfg->num_samples, thin, cfg->save_warmup != 0, cfg->refresh, cfg->stepsize,
+          cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma, cfg->kappa, cfg->t0,
+          cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt, logger, init_writer,
+          out_writer, diag_writer, &eps, &minv_out, &n_grad, &warm, &samp);
     } else {
-      rc = stanx::services::sample::hmc_nuts_diag_e(
-          model, init ? &init_v : nullptr, init_inv_metric ? &minv_v : nullptr,
-          cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup, cfg->num_samples,
-          cfg->num_thin > 0 ? cfg->num_thin : 1, cfg->save_warmup != 0, cfg->refresh,
-          cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth, interrupt, logger,
-          init_writer, out_writer, diag_writer, &n_grad, &warm, &samp);
+      rc = dense ? smp::hmc_nuts_dense_e(model, iv, mv, cfg->random_seed, cfg->chain,
+                                         cfg->init_radius, cfg->num_warmup, cfg->num_samples,
+                                         thin, cfg->save_warmup != 0, cfg->refresh,
+                                         cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth,
+                                         interrupt, logger, init_writer, out_writer,
+                                         diag_writer, &n_grad, &warm, &samp)
+                 : smp::hmc_nuts_diag_e(model, iv, mv, cfg->random_seed, cfg->chain,
+                                        cfg->init_radius, cfg->num_warmup, cfg->num_samples,
+                                        thin, cfg->save_warmup != 0, cfg->refresh,
+                                        cfg->stepsize, cfg->stepsize_jitter, cfg->max_depth,
+                                        interrupt, logger, init_writer, out_writer,
+                                        diag_writer, &n_grad, &warm, &samp);
       eps = cfg->stepsize;
-      minv_out = init_inv_metric ? minv_v : std::vector<double>(D, 1.0);
+      if (init_inv_metric) {
+        minv_out = minv_v;
+      } else {
+        minv_out.assign(M, 0.0);
+        for (size_t i = 0; i < D; ++i) minv_out[dense ? i * D + i : i] = 1.0;
+      }
     }
   } catch (const std::exception& e) {
     sx::set_error(e.what());
@@ -122,7 +145,7 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
   }
   if (!logger.errors.empty()) sx::set_error(logger.errors.back());
   // rows carry: lp__, accept_stat__, 5 sampler params, then constrained
-  // params from write_array; report unconstrained q for the table instead
+  // params from write_array
   out->rows = static_cast<int64_t>(sample_writer.rows.size());
   const size_t ncol = 7 + D;
   for (size_t i = 0; i < sample_writer.rows.size(); ++i) {
@@ -130,8 +153,8 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
     std::memcpy(out->draws + i * ncol, r.data(), sizeof(double) * std::min(r.size(), ncol));
   }
   out->stepsize = eps;
-  if (out->inv_metric && minv_out.size() == D)
-    std::memcpy(out->inv_metric, minv_out.data(), sizeof(double) * D);
+  if (out->inv_metric && minv_out.size() == M)
+    std::memcpy(out->inv_metric, minv_out.data(), sizeof(double) * M);
   out->n_grad = n_grad;
   out->warmup_seconds = warm;
   out->sampling_seconds = samp;
@@ -219,6 +242,86 @@ int sx_hmc_nuts_diag_e_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config
   }
   stanx::model::callback_model model(f, ctx, D);
   return run_service(model, cfg, init, init_inv_metric, out, false);
+}
+
+// ---- dense_e metric ----
+int sx_hmc_nuts_dense_e_adapt(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                              const double* init_inv_metric, sx_nuts_result* out) {
+  if (!m) {
+    sx::set_error("sx_hmc_nuts_dense_e_adapt: NULL model");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, true, nullptr, -1, true);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_dense_e_adapt_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config* cfg,
+                                 const double* init, const double* init_inv_metric,
+                                 sx_nuts_result* out) {
+  if (!f || D < 0) {
+    sx::set_error("sx_hmc_nuts_dense_e_adapt_cb: invalid model");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out, true, nullptr, -1, true);
+}
+
+int sx_hmc_nuts_dense_e_adapt_csv(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                                  const double* init_inv_metric, const char* csv_path,
+                                  int sig_figs, sx_nuts_result* out) {
+  if (!m || !csv_path) {
+    sx::set_error("sx_hmc_nuts_dense_e_adapt_csv: NULL model or path");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, true, csv_path, sig_figs, true);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_dense_e_adapt_cb_csv(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config* cfg,
+                                     const double* init, const double* init_inv_metric,
+                                     const char* csv_path, int sig_figs, sx_nuts_result* out) {
+  if (!f || D < 0 || !csv_path) {
+    sx::set_error("sx_hmc_nuts_dense_e_adapt_cb_csv: invalid model or path");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out, true, csv_path, sig_figs, true);
+}
+
+int sx_hmc_nuts_dense_e(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                        const double* init_inv_metric, sx_nuts_result* out) {
+  if (!m) {
+    sx::set_error("sx_hmc_nuts_dense_e: NULL model");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, false, nullptr, -1, true);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_dense_e_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config* cfg,
+                           const double* init, const double* init_inv_metric,
+                           sx_nuts_result* out) {
+  if (!f || D < 0) {
+    sx::set_error("sx_hmc_nuts_dense_e_cb: invalid model");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out, false, nullptr, -1, true);
 }
 
 }  // extern "C"

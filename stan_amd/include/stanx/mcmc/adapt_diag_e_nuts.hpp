@@ -23,21 +23,6 @@ template <class Model>
 class diag_e_nuts : public base_nuts<Model> {
  public:
   diag_e_nuts(const Model& model, rng_t& rng) : base_nuts<Model>(model, rng) {}
-
-  void get_sampler_param_names(std::vector<std::string>& names) const {
-    names.push_back("stepsize__");
-    names.push_back("treedepth__");
-    names.push_back("n_leapfrog__");
-    names.push_back("divergent__");
-    names.push_back("energy__");
-  }
-  void get_sampler_params(std::vector<double>& values) const {
-    values.push_back(this->epsilon_);
-    values.push_back(this->depth_);
-    values.push_back(this->n_leapfrog_);
-    values.push_back(this->divergent_);
-    values.push_back(this->energy_);
-  }
 };
 
 template <class Model>

@@ -2,8 +2,10 @@
 //   stepsize_adaptation   src/stan/mcmc/stepsize_adaptation.hpp:11-84
 //   windowed_adaptation   src/stan/mcmc/windowed_adaptation.hpp:12-123
 //   var_adaptation        src/stan/mcmc/var_adaptation.hpp:12-50
-// The running variance is Welford's (1962) update, the algorithm of Stan
-// Math's welford_var_estimator (absent from the tree).
+//   covar_adaptation      src/stan/mcmc/covar_adaptation.hpp:12-51
+// The running variance and covariance are Welford's (1962) update, the
+// algorithm of Stan Math's welford_var_estimator and
+// welford_covar_estimator (absent from the tree).
 #ifndef STANX_MCMC_ADAPTATION_HPP
 #define STANX_MCMC_ADAPTATION_HPP
 
@@ -220,6 +222,79 @@ class var_adaptation : public windowed_adaptation {
 
  protected:
   welford_var_estimator estimator_;
+};
+
+// Running mean and covariance: delta = q - m, m += delta / n,
+// m2 += (q - m) delta^T (row-major D x D), covariance m2 / (n - 1).
+class welford_covar_estimator {
+ public:
+  explicit welford_covar_estimator(int n)
+      : n_(n), m_(n, 0.0), m2_(static_cast<size_t>(n) * n, 0.0), delta_(n, 0.0) {}
+  void restart() {
+    num_samples_ = 0;
+    std::fill(m_.begin(), m_.end(), 0.0);
+    std::fill(m2_.begin(), m2_.end(), 0.0);
+  }
+  void add_sample(const std::vector<double>& q) {
+    num_samples_ += 1.0;
+    for (int i = 0; i < n_; ++i) delta_[i] = q[i] - m_[i];
+    for (int i = 0; i < n_; ++i) m_[i] += delta_[i] / num_samples_;
+    for (int i = 0; i < n_; ++i) {
+      const double d = q[i] - m_[i];
+      double* row = m2_.data() + static_cast<size_t>(i) * n_;
+      for (int j = 0; j < n_; ++j) row[j] += d * delta_[j];
+    }
+  }
+  void sample_covariance(std::vector<double>& covar) const {
+    if (num_samples_ > 1)
+      for (size_t k = 0; k < m2_.size(); ++k) covar[k] = m2_[k] / (num_samples_ - 1.0);
+  }
+  double num_samples() const { return num_samples_; }
+
+ private:
+  int n_;
+  double num_samples_ = 0;
+  std::vector<double> m_, m2_, delta_;
+};
+
+// Windowed covariance estimation of a dense inverse metric (row-major):
+// at each window end the regularised estimate n/(n+5) C + 1e-3 5/(n+5) I.
+class covar_adaptation : public windowed_adaptation {
+ public:
+  explicit covar_adaptation(int n)
+      : windowed_adaptation("covariance"), n_(n), estimator_(n) {}
+  bool learn_covariance(std::vector<double>& covar, const std::vector<double>& q) {
+    if (adaptation_window()) estimator_.add_sample(q);
+    if (end_adaptation_window()) {
+      compute_next_window();
+      estimator_.sample_covariance(covar);
+      const double n = estimator_.num_samples();
+      const double a = n / (n + 5.0), b = 1e-3 * (5.0 / (n + 5.0));
+      bool finite = true;
+      for (int i = 0; i < n_; ++i)
+        for (int j = 0; j < n_; ++j) {
+          double& c = covar[static_cast<size_t>(i) * n_ + j];
+          c = a * c + b * (i == j ? 1.0 : 0.0);
+          if (!std::isfinite(c)) finite = false;
+        }
+      if (!finite)
+        throw std::runtime_error(
+            "Numerical overflow in metric adaptation. "
+            "This occurs when the sampler encounters extreme values on the "
+            "unconstrained space; this may happen when the posterior density "
+            "function is too wide or improper. "
+            "There may be problems with your model specification.");
+      estimator_.restart();
+      ++adapt_window_counter_;
+      return true;
+    }
+    ++adapt_window_counter_;
+    return false;
+  }
+
+ protected:
+  int n_;
+  welford_covar_estimator estimator_;
 };
 
 }  // namespace mcmc

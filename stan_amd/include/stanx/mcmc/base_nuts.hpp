@@ -1,4 +1,4 @@
-// Multinomial NUTS with a diagonal Euclidean metric, written as an
+// Multinomial NUTS with a Euclidean metric (diagonal or dense), written as an
 // explicit-stack (iterative) tree builder so that the same state machine
 // can run one leapfrog step at a time; the GPU batched sampler uses the
 // same frame layout per chain.  Semantics follow the recursive reference
@@ -67,10 +67,11 @@ class sample {
   double log_prob_, accept_stat_;
 };
 
-template <class Model>
+template <class Model, class Metric = diag_e_metric<Model>>
 class base_nuts {
  public:
-  using hamiltonian_t = diag_e_metric<Model>;
+  using hamiltonian_t = Metric;
+  using point_t = typename Metric::point_t;
 
   base_nuts(const Model& model, rng_t& rng)
       : z_(static_cast<int>(model.num_params_r())),
@@ -88,8 +89,8 @@ class base_nuts {
   virtual ~base_nuts() {}
 
   // ---- base_hmc surface ----
-  diag_e_point& z() { return z_; }
-  const diag_e_point& z() const { return z_; }
+  point_t& z() { return z_; }
+  const point_t& z() const { return z_; }
   hamiltonian_t& hamiltonian() { return hamiltonian_; }
   void set_metric(const vec_t& inv_e_metric) { z_.set_metric(inv_e_metric); }
   void set_nominal_stepsize(double e) {
@@ -265,6 +266,22 @@ class base_nuts {
     return sample(z_.q, -z_.V, accept_prob);
   }
 
+  // sampler parameters of a draw (base_nuts.hpp:206-220)
+  void get_sampler_param_names(std::vector<std::string>& names) const {
+    names.push_back("stepsize__");
+    names.push_back("treedepth__");
+    names.push_back("n_leapfrog__");
+    names.push_back("divergent__");
+    names.push_back("energy__");
+  }
+  void get_sampler_params(std::vector<double>& values) const {
+    values.push_back(this->epsilon_);
+    values.push_back(this->depth_);
+    values.push_back(this->n_leapfrog_);
+    values.push_back(this->divergent_);
+    values.push_back(this->energy_);
+  }
+
   static bool compute_criterion(const vec_t& p_sharp_minus,
                                 const vec_t& p_sharp_plus, const vec_t& rho) {
     return dot(p_sharp_plus, rho) > 0 && dot(p_sharp_minus, rho) > 0;
@@ -425,7 +442,7 @@ class base_nuts {
     return persist;
   }
 
-  diag_e_point z_;
+  point_t z_;
   expl_leapfrog<hamiltonian_t> integrator_;
   hamiltonian_t hamiltonian_;
   rng_t& rng_;

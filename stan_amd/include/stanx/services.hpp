@@ -11,6 +11,15 @@
 //                          src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:57-112
 //   sample::hmc_nuts_diag_e
 //                          src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91
+//   sample::hmc_nuts_dense_e_adapt
+//                          src/stan/services/sample/hmc_nuts_dense_e_adapt.hpp:57-112
+//   sample::hmc_nuts_dense_e
+//                          src/stan/services/sample/hmc_nuts_dense_e.hpp:48-93
+//   util::read_dense_inv_metric, util::validate_dense_inv_metric,
+//   util::create_unit_e_dense_inv_metric
+//                          src/stan/services/util/read_dense_inv_metric.hpp:25-42,
+//                          validate_dense_inv_metric.hpp:18-28,
+//                          create_unit_e_dense_inv_metric.hpp:18-27
 #ifndef STANX_SERVICES_HPP
 #define STANX_SERVICES_HPP
 
@@ -24,6 +33,7 @@
 
 #include "callbacks.hpp"
 #include "io.hpp"
+#include "mcmc/adapt_dense_e_nuts.hpp"
 #include "mcmc/adapt_diag_e_nuts.hpp"
 #include "model.hpp"
 #include "philox.hpp"
@@ -141,11 +151,7 @@ class mcmc_writer {
     std::stringstream ss;
     ss << "Step size = " << sampler.get_nominal_stepsize();
     sample_writer_(ss.str());
-    sample_writer_("Diagonal elements of inverse mass matrix:");
-    std::stringstream m;
-    const auto& im = sampler.z().inv_e_metric_;
-    for (size_t i = 0; i < im.size(); ++i) m << (i ? ", " : "") << im[i];
-    sample_writer_(m.str());
+    sampler.z().write_metric(sample_writer_);
   }
   // blank line, three aligned lines, blank line (mcmc_writer.hpp:203-238)
   void write_timing(double warm, double samp, callbacks::writer& w) {
@@ -384,6 +390,136 @@ int hmc_nuts_diag_e_adapt(const Model& model, const std::vector<double>* init,
   if (stepsize_out) *stepsize_out = sampler->get_nominal_stepsize();
   if (inv_metric_out) *inv_metric_out = sampler->z().inv_e_metric_;
   if (n_grad_out) *n_grad_out = init_evals + sampler->hamiltonian().n_grad();
+  return rc;
+}
+
+// The dense inverse metric (row-major D x D): the given one, or the
+// identity (create_unit_e_dense_inv_metric); it must be D x D, finite,
+// symmetric to 1e-8 and positive definite (validate_dense_inv_metric:
+// "Inverse Euclidean metric not positive definite." and an
+// "Initialization failure" domain error otherwise).
+inline std::vector<double> dense_inv_metric(const std::vector<double>* given, size_t D,
+                                            callbacks::logger& logger) {
+  std::vector<double> m(D * D, 0.0);
+  if (given) {
+    if (given->size() != D * D) {
+      logger.error("Cannot get inverse metric from input file.");
+      throw std::domain_error("Initialization failure");
+    }
+    m = *given;
+  } else {
+    for (size_t i = 0; i < D; ++i) m[i * D + i] = 1.0;
+  }
+  bool ok = true;
+  for (size_t i = 0; i < D && ok; ++i)
+    for (size_t j = 0; j < D; ++j)
+      if (!std::isfinite(m[i * D + j]) || std::fabs(m[i * D + j] - m[j * D + i]) > 1e-8) {
+        ok = false;
+        break;
+      }
+  std::vector<double> L;
+  if (!ok || !mcmc::cholesky_lower(m, static_cast<int>(D), L)) {
+    logger.error("Inverse Euclidean metric not positive definite.");
+    throw std::domain_error("Initialization failure");
+  }
+  return m;
+}
+
+// hmc_nuts_dense_e_adapt: windowed covariance adaptation of a dense
+// metric; inv_metric_out receives the adapted D x D matrix.
+template <class Model>
+int hmc_nuts_dense_e_adapt(const Model& model, const std::vector<double>* init,
+                           const std::vector<double>* init_inv_metric,
+                           unsigned int random_seed, unsigned int chain,
+                           double init_radius, int num_warmup, int num_samples,
+                           int num_thin, bool save_warmup, int refresh, double stepsize,
+                           double stepsize_jitter, int max_depth, double delta,
+                           double gamma, double kappa, double t0,
+                           unsigned int init_buffer, unsigned int term_buffer,
+                           unsigned int window, callbacks::interrupt& interrupt,
+                           callbacks::logger& logger, callbacks::writer& init_writer,
+                           callbacks::writer& sample_writer,
+                           callbacks::writer& diagnostic_writer,
+                           double* stepsize_out = nullptr,
+                           std::vector<double>* inv_metric_out = nullptr,
+                           long long* n_grad_out = nullptr,
+                           double* warm_s = nullptr, double* samp_s = nullptr) {
+  rng_t rng = util::create_rng(random_seed, chain);
+  std::vector<double> cont_vector;
+  std::vector<double> inv_metric;
+  long long init_evals = 0;
+  try {
+    cont_vector = util::initialize(model, init, rng, init_radius, true, logger,
+                                   init_writer, &init_evals);
+    inv_metric = dense_inv_metric(init_inv_metric, model.num_params_r(), logger);
+  } catch (const std::exception& e) {
+    logger.error(e.what());
+    return error_codes::CONFIG;
+  }
+  mcmc::adapt_dense_e_nuts<Model> sampler(model, rng);
+  sampler.set_metric(inv_metric);
+  sampler.set_nominal_stepsize(stepsize);
+  sampler.set_stepsize_jitter(stepsize_jitter);
+  sampler.set_max_depth(max_depth);
+  sampler.get_stepsize_adaptation().set_mu(std::log(10 * stepsize));
+  sampler.get_stepsize_adaptation().set_delta(delta);
+  sampler.get_stepsize_adaptation().set_gamma(gamma);
+  sampler.get_stepsize_adaptation().set_kappa(kappa);
+  sampler.get_stepsize_adaptation().set_t0(t0);
+  sampler.set_window_params(num_warmup, init_buffer, term_buffer, window, logger);
+  int rc = error_codes::OK;
+  try {
+    util::run_adaptive_sampler(sampler, model, cont_vector, num_warmup, num_samples,
+                               num_thin, refresh, save_warmup, rng, interrupt, logger,
+                               sample_writer, diagnostic_writer, warm_s, samp_s);
+  } catch (const std::exception& e) {
+    logger.error(e.what());
+    rc = error_codes::SOFTWARE;
+  }
+  if (stepsize_out) *stepsize_out = sampler.get_nominal_stepsize();
+  if (inv_metric_out) *inv_metric_out = sampler.z().inv_e_metric_;
+  if (n_grad_out) *n_grad_out = init_evals + sampler.hamiltonian().n_grad();
+  return rc;
+}
+
+// hmc_nuts_dense_e: the dense-metric transition, no adaptation.
+template <class Model>
+int hmc_nuts_dense_e(const Model& model, const std::vector<double>* init,
+                     const std::vector<double>* init_inv_metric, unsigned int random_seed,
+                     unsigned int chain, double init_radius, int num_warmup,
+                     int num_samples, int num_thin, bool save_warmup, int refresh,
+                     double stepsize, double stepsize_jitter, int max_depth,
+                     callbacks::interrupt& interrupt, callbacks::logger& logger,
+                     callbacks::writer& init_writer, callbacks::writer& sample_writer,
+                     callbacks::writer& diagnostic_writer, long long* n_grad_out = nullptr,
+                     double* warm_s = nullptr, double* samp_s = nullptr) {
+  rng_t rng = util::create_rng(random_seed, chain);
+  std::vector<double> cont_vector;
+  std::vector<double> inv_metric;
+  long long init_evals = 0;
+  try {
+    cont_vector = util::initialize(model, init, rng, init_radius, true, logger,
+                                   init_writer, &init_evals);
+    inv_metric = dense_inv_metric(init_inv_metric, model.num_params_r(), logger);
+  } catch (const std::exception& e) {
+    logger.error(e.what());
+    return error_codes::CONFIG;
+  }
+  mcmc::dense_e_nuts<Model> sampler(model, rng);
+  sampler.set_metric(inv_metric);
+  sampler.set_nominal_stepsize(stepsize);
+  sampler.set_stepsize_jitter(stepsize_jitter);
+  sampler.set_max_depth(max_depth);
+  int rc = error_codes::OK;
+  try {
+    util::run_sampler(sampler, model, cont_vector, num_warmup, num_samples, num_thin,
+                      refresh, save_warmup, rng, interrupt, logger, sample_writer,
+                      diagnostic_writer, warm_s, samp_s);
+  } catch (const std::exception& e) {
+    logger.error(e.what());
+    rc = error_codes::SOFTWARE;
+  }
+  if (n_grad_out) *n_grad_out = init_evals + sampler.hamiltonian().n_grad();
   return rc;
 }
 

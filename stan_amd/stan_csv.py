@@ -5,8 +5,9 @@ read_stan_csv restates the reader of src/stan/io/stan_csv_reader.hpp
 read_adaptation :233-284, read_samples :286-348): the configuration
 comments before the header become metadata, the header gives the column
 names, the comment block after the header gives the adapted step size and
-inverse metric, numeric lines are the draws, and the trailing comment block
-gives the elapsed times.  It reads files written by CmdStan and by
+inverse metric (one line of diagonal elements for diag_e, one line per
+row for dense_e), numeric lines are the draws, and the trailing comment
+block gives the elapsed times.  It reads files written by CmdStan and by
 sx_hmc_nuts_diag_e_adapt_csv alike.
 """
 import re
@@ -40,7 +41,8 @@ def read_stan_csv(path):
         lines = f.read().splitlines()
     meta, names, rows = {}, None, []
     step_size, metric, timing = None, None, {}
-    expect_metric = False
+    expect_metric = 0  # metric lines still to read
+    dense_rows = []
     for line in lines:
         if line.startswith("#"):
             body = line[1:].strip()
@@ -49,16 +51,27 @@ def read_stan_csv(path):
                 if m and m.group(1) in _META_KEYS:
                     meta[m.group(1)] = _value(_META_KEYS[m.group(1)], m.group(2))
                 continue
-            if expect_metric:
-                metric = np.array([float(v) for v in body.split(",")])
-                expect_metric = False
+            if expect_metric and body:
+                row = np.array([float(v) for v in body.split(",")])
+                if expect_metric == -1:  # diagonal: one line
+                    metric = row
+                    expect_metric = 0
+                else:  # dense: as many lines as the first row has entries
+                    dense_rows.append(row)
+                    if len(dense_rows) == len(dense_rows[0]):
+                        metric = np.stack(dense_rows)
+                        expect_metric = 0
                 continue
             m = re.match(r"^Step size = (.*)$", body)
             if m:
                 step_size = float(m.group(1))
                 continue
             if body.startswith("Diagonal elements of inverse mass matrix"):
-                expect_metric = True
+                expect_metric = -1
+                continue
+            if body.startswith("Elements of inverse mass matrix"):
+                expect_metric = 1
+                dense_rows = []
                 continue
             m = re.match(r"^(?:Elapsed Time:)?\s*([0-9.eE+-]+) seconds \((Warm-up|Sampling|Total)\)",
                          body)

@@ -62,3 +62,41 @@ def test_gpu_nuts_rejects_invalid_metric():
                                   num_samples=10)
     assert out["rc"] == 78
     m.close()
+
+
+def test_gpu_dense_nuts_posterior_matches_oracle_sampler(tmp_path):
+    """hmc_nuts_dense_e_adapt on the GPU gradient, correlated predictors:
+    posterior moments against the oracle's dense sampler (4 MCSE), the
+    adapted metric against the posterior covariance, and the CSV output."""
+    print("gpu dense nuts vs oracle", flush=True)
+    d = W.logistic(2000, 4, 0.5, 0.5, seed=91)
+    X = d["X"].copy()
+    X[:, 1] = 0.9 * X[:, 0] + 0.1 * X[:, 1]
+    m = S.GlmModel(S.LOGIT, X, d["y"], prior=d["prior"])
+    ref = O.GlmData(S.LOGIT, X, d["y"], prior=d["prior"])
+    f = O.model_fn("orc_glm_model_cb")
+    ctx = ref.ctx(mode=0)
+    gpu, cpu = [], []
+    for chain in range(1, 5):
+        out = S.hmc_nuts_dense_e_adapt(m, random_seed=2024, chain=chain, num_warmup=500,
+                                       num_samples=1000)
+        assert out["rc"] == 0, out["error"]
+        assert out["inv_metric"].shape == (m.D, m.D)
+        gpu.append(out["draws"][:, 7:])
+        rc, dr, _, _, _ = O.hmc_nuts_dense_e_adapt(f, ctx, m.D, seed=777, chain=chain,
+                                                   num_warmup=500, num_samples=1000)
+        assert rc == 0
+        cpu.append(dr[:, 7:])
+    a = np.stack(gpu, 1)
+    _moments_agree(a, np.stack(cpu, 1), "dense")
+    cov = np.cov(a.reshape(-1, m.D).T)
+    rel = np.linalg.norm(out["inv_metric"] - cov) / np.linalg.norm(cov)
+    assert rel < 0.35, rel
+    path = tmp_path / "dense_glm.csv"
+    out = S.hmc_nuts_dense_e_adapt_csv(m, path, random_seed=3, num_warmup=150,
+                                       num_samples=20)
+    assert out["rc"] == 0
+    assert "Elements of inverse mass matrix" in path.read_text()
+    out = S.hmc_nuts_dense_e(m, random_seed=4, num_warmup=10, num_samples=20, stepsize=0.2)
+    assert out["rc"] == 0 and np.all(out["draws"][:, 2] == 0.2)
+    m.close()
