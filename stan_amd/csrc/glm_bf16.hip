@@ -237,20 +237,15 @@ __global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ f32x16 mma3_32(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x16 c) {
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c, 0, 0, 0);
-  return c;
-}
+typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of global_load_lds
 
 // K <= 128 (KQ = Kp / 32 <= 4): workgroup = 4 waves x 32 chains = 128
 // chains.  Each 32-row tile of X (both layouts, hi and lo) is staged once
-// per workgroup in LDS (double-buffered, the next tile's global loads in
-// flight during the current tile's MFMAs) and read by all four waves, so X
-// crosses L2 once per 128 chains.  v_mfma_f32_32x32x16_bf16 operands (lane
+// per workgroup in LDS, straight from global memory (global_load_lds), and
+// read by all four waves, so X crosses L2 once per 128 chains.  Three LDS
+// buffers carry a software pipeline: while the link and adjoint of tile t
+// run, the forward MFMAs of tile t + 1 (independent of them) interleave
+// with the link's VALU work and tile t + 2 streams in.  v_mfma_f32_32x32x16_bf16 operands (lane
 // l, j < 8): A[l % 32][8 (l / 32) + j], B[8 (l / 32) + j][l % 32]; result
 // d[j] = D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
 //   forward  H (32 rows x 32 chains) = X1 tile (A, from LDS) . B (registers)
@@ -264,14 +259,14 @@ template <int KQ, int FAM>
 __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;          // forward k-steps of 16
-  constexpr int X1S = Kp + 8;          // padded LDS row stride of the X1 tile
-  constexpr int X1E = 32 * X1S;        // elements of one X1 tile array
-  constexpr int X2E = 32 * Kp;         // elements of one X2 tile array
-  constexpr int CH = 16 * Kp;          // 16-byte chunks per tile (4 arrays)
-  constexpr int PER = CH / 256;        // chunks per thread
-  __shared__ __attribute__((aligned(16))) __bf16 x1h[2][X1E], x1l[2][X1E];
-  __shared__ __attribute__((aligned(16))) __bf16 x2h[2][X2E], x2l[2][X2E];
-  __shared__ float ys[2][32];
+  constexpr int CPR = Kp / 8;          // 16-byte chunks per X1 row
+  // XOR swizzle of X1 chunks: the largest power of two (<= 16) dividing
+  // CPR, so c ^ (r & SWZ) stays inside the row
+  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int TE = 32 * Kp;          // elements of one tile array
+  constexpr int BPA = Kp / 16;         // 1 KB blocks per tile array
+  __shared__ __attribute__((aligned(16))) __bf16 xt[3][4][TE];  // X1h X1l X2h X2l
+  __shared__ float ys[3][32];  // y of the tile's rows (+-1 for the logit)
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -301,23 +296,32 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
 
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
-  // tile staging: thread tid moves chunks tid + 256 i (16 bytes each) of
-  // the four tile arrays; chunk -> (array, offset) is fixed per thread, so
-  // the loads and stores below use compile-time slots and stay in registers
   const __bf16* X1h = reinterpret_cast<const __bf16*>(a.X1h);
   const __bf16* X1l = reinterpret_cast<const __bf16*>(a.X1l);
   const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
   const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
-  u32x4 stage[PER];  // native vectors: kept in registers
   float ystage = 0.f;
-#define SX_GLOAD(T)                                                              \
+  // Tile staging straight into LDS (global_load_lds_dwordx4: lane l of a
+  // wave writes 16 bytes at the wave's LDS base + 16 l).  Wave w moves the
+  // 1 KB blocks w, w + 4, ... of the tile (block b: array b / BPA); X1 rows
+  // are stored with chunk c of row r at c ^ (r & SWZ), so the forward's
+  // row-per-lane reads spread over the banks; X2 is copied as is.
+#define SX_DMA(T, BUF)                                                           \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < PER; ++i) {                            \
-      const int c_ = tid + 256 * i;                                              \
-      const int arr_ = c_ / (4 * Kp), o_ = (c_ % (4 * Kp)) * 8;                  \
-      const __bf16* p_ = arr_ == 0 ? X1h : arr_ == 1 ? X1l : arr_ == 2 ? X2h : X2l; \
-      stage[i] = *reinterpret_cast<const u32x4*>(p_ + base_ + o_);               \
+    _Pragma("unroll") for (int i = 0; i < KQ * 2; ++i) {                         \
+      const int b_ = w + 4 * i;                                                  \
+      const int arr_ = b_ / BPA;                                                 \
+      const int p_ = (b_ % BPA) * 64 + lane;                                     \
+      int src_ = p_;                                                             \
+      if (arr_ < 2) {                                                            \
+        const int r_ = p_ / CPR;                                                 \
+        src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                             \
+      }                                                                          \
+      const __bf16* g_ = arr_ == 0 ? X1h : arr_ == 1 ? X1l : arr_ == 2 ? X2h : X2l; \
+      __builtin_amdgcn_global_load_lds(                                          \
+          g_ + base_ + 8 * src_,                                                 \
+          (lds_ptr)(&xt[BUF][arr_][(b_ % BPA) * 512]), 16, 0, 0);                \
     }                                                                            \
     if (tid < 32) {                                                              \
       const int64_t n_ = (T) * 32 + tid;                                         \
@@ -331,121 +335,113 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
       ystage = v_;                                                               \
     }                                                                            \
   }
-#define SX_LSTORE(BUF)                                                           \
+#define SX_DMA_DONE(BUF)                                                         \
   {                                                                              \
-    _Pragma("unroll") for (int i = 0; i < PER; ++i) {                            \
-      const int c_ = tid + 256 * i;                                              \
-      const int arr_ = c_ / (4 * Kp), o_ = (c_ % (4 * Kp)) * 8;                  \
-      __bf16* d_;                                                                \
-      if (arr_ < 2)                                                              \
-        d_ = (arr_ == 0 ? x1h[BUF] : x1l[BUF]) + (o_ / Kp) * X1S + (o_ % Kp);    \
-      else                                                                       \
-        d_ = (arr_ == 2 ? x2h[BUF] : x2l[BUF]) + o_;                             \
-      *reinterpret_cast<u32x4*>(d_) = stage[i];                                  \
-    }                                                                            \
+    __builtin_amdgcn_s_waitcnt(0);                                               \
     if (tid < 32) ys[BUF][tid] = ystage;                                         \
   }
+  // Software pipeline over tiles, three LDS buffers: iteration t runs the
+  // forward MFMAs of tile t + 1 (independent work) alongside the link and
+  // adjoint of tile t, while tile t + 2 streams into the third buffer.
+  // The forward of the tile after the last reads a stale buffer and its
+  // result is discarded.
+  f32x16 hcur, hnext;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
+#define SX_FORWARD(BUF, ACC)                                                     \
+  {                                                                              \
+    _Pragma("unroll") for (int j = 0; j < 16; ++j) ACC[j] = 0.f;                 \
+    _Pragma("unroll") for (int s = 0; s < S16; ++s) {                            \
+      const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));               \
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&xt[BUF][0][off]);      \
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&xt[BUF][1][off]);      \
+      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], ACC, 0, 0, 0);    \
+      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], ACC, 0, 0, 0);    \
+      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], ACC, 0, 0, 0);    \
+    }                                                                            \
+  }
   if (t0 < t1) {
-    SX_GLOAD(t0);
-    SX_LSTORE(0);
+    SX_DMA(t0, 0);
+    SX_DMA_DONE(0);
+    if (t0 + 1 < t1) {
+      SX_DMA(t0 + 1, 1);
+      SX_DMA_DONE(1);
+    }
   }
   __syncthreads();
+  if (t0 < t1) SX_FORWARD(0, hcur);
   for (int64_t t = t0; t < t1; ++t) {
-    const int buf = static_cast<int>((t - t0) & 1);
-    if (t + 1 < t1) SX_GLOAD(t + 1);
-    // forward: the three products accumulate separately, so consecutive
-    // MFMAs are independent and issue at throughput rather than latency
-    f32x16 hhh, hhl, hlh;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) hhh[j] = hhl[j] = hlh[j] = 0.f;
-#pragma unroll
-    for (int s = 0; s < S16; ++s) {
-      const int off = li * X1S + 16 * s + 8 * h;
-      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1h[buf][off]);
-      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1l[buf][off]);
-      hhh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], hhh, 0, 0, 0);
-      hhl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], hhl, 0, 0, 0);
-      hlh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], hlh, 0, 0, 0);
-    }
-    f32x16 hacc;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) hacc[j] = (hhh[j] + hhl[j]) + hlh[j];
-    // link, branch-free (rows past N are masked): row of element j is
-    // 8 (j/4) + 4 h + j%4.  The tile's sums are FP32 (16 terms per lane) and
-    // are added to the FP64 totals once per tile.
-    float r[16];
+    const int buf = static_cast<int>((t - t0) % 3);
+    const int nbuf = buf == 2 ? 0 : buf + 1;
+    const int fbuf = nbuf == 2 ? 0 : nbuf + 1;
+    if (t + 2 < t1) SX_DMA(t + 2, fbuf);
+    SX_FORWARD(nbuf, hnext);
+    // link and adjoint of tile t in two halves of the lane's 16 rows: half
+    // st holds elements j = 8 st .. 8 st + 7 (rows 8 (j/4) + 4 h + j%4),
+    // the rows the adjoint's step st contracts.  Rows past N are masked.
+    // The tile's sums are FP32 and are added to the FP64 totals per tile.
     const int64_t n0 = t * 32;
     float lpt = 0.f, srt = 0.f;
     int badt = 0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int row = 8 * (j >> 2) + 4 * h + (j & 3);
-      const bool valid = n0 + row < a.N;
-      const float yv = ys[buf][row];
-      const float eta = hacc[j] + alpha;
-      float term, rr;
-      if (FAM == SX_LOGIT) {
-        logit_row_fast(eta, yv, &term, &rr);
-      } else {
-        rr = yv - eta;
-        term = rr * rr;
-      }
-      term = valid ? term : 0.f;
-      rr = valid ? rr : 0.f;
-      badt += (valid && !isfinite(eta)) ? 1 : 0;
-      lpt += term;
-      srt += rr;
-      r[j] = rr;
-    }
-    lp += static_cast<double>(lpt);
-    sr += static_cast<double>(srt);
-    bad += static_cast<double>(badt);
-    // adjoint
-#pragma unroll
     for (int st = 0; st < 2; ++st) {
       bf16x8 rh, rl;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int jj = 0; jj < 8; ++jj) {
+        const int j = 8 * st + jj;
+        const int row = 8 * (j >> 2) + 4 * h + (j & 3);
+        const bool valid = n0 + row < a.N;
+        const float yv = ys[buf][row];
+        const float eta = hcur[j] + alpha;
+        float term, rr;
+        if (FAM == SX_LOGIT) {
+          logit_row_fast(eta, yv, &term, &rr);
+        } else {
+          rr = yv - eta;
+          term = rr * rr;
+        }
+        term = valid ? term : 0.f;
+        rr = valid ? rr : 0.f;
+        badt += (valid && !isfinite(eta)) ? 1 : 0;
+        lpt += term;
+        srt += rr;
         __bf16 hh, ll;
-        split(r[8 * st + j], &hh, &ll);
-        rh[j] = hh;
-        rl[j] = ll;
+        split(rr, &hh, &ll);
+        rh[jj] = hh;
+        rl[jj] = ll;
       }
-      bf16x8 xh[KQ], xl[KQ];
 #pragma unroll
       for (int kb = 0; kb < KQ; ++kb) {
         const int k = 32 * kb + li;
         const int o0 = ((2 * st) * Kp + k) * 8 + 4 * h;
         const int o1 = ((2 * st + 1) * Kp + k) * 8 + 4 * h;
-        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&x2h[buf][o0]);
-        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2h[buf][o1]);
-        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2l[buf][o0]);
-        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2l[buf][o1]);
+        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&xt[buf][2][o0]);
+        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&xt[buf][2][o1]);
+        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&xt[buf][3][o0]);
+        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&xt[buf][3][o1]);
+        bf16x8 xh, xl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          xh[kb][j] = ah0[j];
-          xh[kb][4 + j] = ah1[j];
-          xl[kb][j] = al0[j];
-          xl[kb][4 + j] = al1[j];
+          xh[j] = ah0[j];
+          xh[4 + j] = ah1[j];
+          xl[j] = al0[j];
+          xl[4 + j] = al1[j];
         }
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xh, g[kb], 0, 0, 0);
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xl, g[kb], 0, 0, 0);
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, xh, g[kb], 0, 0, 0);
       }
-      // product-major order: the KQ accumulators make consecutive MFMAs
-      // independent
-#pragma unroll
-      for (int kb = 0; kb < KQ; ++kb)
-        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xh[kb], g[kb], 0, 0, 0);
-#pragma unroll
-      for (int kb = 0; kb < KQ; ++kb)
-        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xl[kb], g[kb], 0, 0, 0);
-#pragma unroll
-      for (int kb = 0; kb < KQ; ++kb)
-        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, xh[kb], g[kb], 0, 0, 0);
     }
-    if (t + 1 < t1) SX_LSTORE(buf ^ 1);
+    lp += static_cast<double>(lpt);
+    sr += static_cast<double>(srt);
+    bad += static_cast<double>(badt);
+    if (t + 2 < t1) SX_DMA_DONE(fbuf);
     __syncthreads();
+    hcur = hnext;
   }
-#undef SX_GLOAD
-#undef SX_LSTORE
+#undef SX_FORWARD
+#undef SX_DMA
+#undef SX_DMA_DONE
   // g[kb][j] = G^T[chain cbase + 8 (j/4) + 4 h + j%4][k = 32 kb + li]
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll
