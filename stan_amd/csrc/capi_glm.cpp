@@ -519,10 +519,10 @@ void free_workspace(hier_workspace& w) {
   w = hier_workspace();
 }
 
-// chains padded to the chain tile of the MFMA kernel: 128 for K <= 128
+// chains padded to the chain tile of the MFMA kernel: 128 for K <= 256
 // (32x32 tiles, four waves of 32 chains), else 64
 int mx_chain_pad(const sx_glm* m, int C) {
-  const int tile = m->Kp <= 128 ? 128 : 64;
+  const int tile = m->Kp <= 256 ? 128 : 64;
   return (C + tile - 1) / tile * tile;
 }
 

@@ -7,18 +7,18 @@
 // chain pass is a matrix-vector product and stays on the HBM roofline).
 // Each product of a split pair is formed as hi*hi + hi*lo + lo*hi (the
 // lo*lo term is below the split's own representation error) with
-// v_mfma_f32_16x16x32_bf16.
+// v_mfma_f32_32x32x16_bf16.
 //
 // The two contractions run over different indices of X (k forward, n in
 // the adjoint) and an MFMA operand holds 8 consecutive inner-index values
 // per lane, so X is stored twice, in HBM, as BF16 hi+lo:
 //   X1[n][k]            k-packed rows      (forward operand)
 //   X2[n/8][k][n % 8]   8-row-packed       (adjoint operand)
-// Workgroup = 4 waves x 16 chains; the grid's x dimension splits the rows,
-// y the chain tiles; row splits are multiples of 8 so the blocks that share
-// a row range (different chain tiles) land on the same XCD and share X in
-// its L2.  Per-block FP32 partials are summed across blocks in FP64 by a
-// second launch that also adds priors and Jacobian (shared finalize).
+// Workgroup = 4 waves x 32 chains (glm_mfma32); a 1-D grid of row splits x
+// chain blocks, mapped so that the blocks sharing a row range land on the
+// same XCD and share X in its L2.  Per-block FP32 partials are summed
+// across blocks in FP64 by a second launch that also adds priors and
+// Jacobian (shared finalize).
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -33,7 +33,6 @@ namespace sx {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
 
@@ -41,13 +40,6 @@ __device__ __forceinline__ void split(float x, __bf16* hi, __bf16* lo) {
   const __bf16 h = to_bf16(x);
   *hi = h;
   *lo = to_bf16(x - static_cast<float>(h));
-}
-
-__device__ __forceinline__ f32x4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x4 c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-  return c;
 }
 
 // FP32 Bernoulli-logit row on the hardware transcendentals (v_exp_f32,
@@ -61,15 +53,6 @@ __device__ __forceinline__ void logit_row_fast(float eta, float s, float* term, 
   const float u = 1.0f + t;
   const float inv = __builtin_amdgcn_rcpf(u);
   *term = -(fmaxf(-se, 0.0f) + __logf(u));
-  *r = s * (se >= 0.0f ? t * inv : inv);
-}
-
-// FP32 Bernoulli-logit row: term = -log1p(exp(-s eta)), r = s inv_logit(-s eta)
-__device__ __forceinline__ void logit_row_f32(float eta, float s, float* term, float* r) {
-  const float se = s * eta;
-  const float t = __expf(-fabsf(se));
-  const float inv = 1.0f / (1.0f + t);
-  *term = -(fmaxf(-se, 0.0f) + log1pf(t));
   *r = s * (se >= 0.0f ? t * inv : inv);
 }
 
@@ -104,159 +87,30 @@ __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, 
   if (threadIdx.x == 0) alpha[c] = c < C ? static_cast<float>(theta[c * D]) : 0.0f;
 }
 
-template <int KS, int FAM>
-__global__ __launch_bounds__(256) void glm_mfma(mx_args a) {
-  __shared__ float rs[4][16][33];
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lr = lane & 15, lg = lane >> 4;
-  const int Kp = a.Kp;
-  // 1-D grid, XCD-aware: blocks are dealt to the 8 XCDs round-robin, so the
-  // nCB chain-blocks of one row split are given ids 8 apart: they run on
-  // the same XCD at about the same time and read the split's X tiles
-  // through one L2.
-  const int nCB = a.Cp / 64;
-  const int bid = blockIdx.x;
-  const int xcd = bid & 7, widx = bid >> 3;
-  const int spl = (widx / nCB) * 8 + xcd;
-  const int cblk = widx % nCB;
-  const int cb = cblk * 64 + 16 * w;  // first chain of this wave
-  // forward A operands: B^T rows (chains cb + lr), k = 32 s + 8 lg + j
-  bf16x8 bh[KS], bl[KS];
-#pragma unroll
-  for (int s = 0; s < KS; ++s) {
-    const int64_t off = static_cast<int64_t>(cb + lr) * Kp + 32 * s + 8 * lg;
-    bh[s] = *reinterpret_cast<const bf16x8*>(a.Bh + off);
-    bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
-  }
-  float alpha[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) alpha[j] = a.alpha[cb + 4 * lg + j];
-  f32x4 g[2 * KS];
-#pragma unroll
-  for (int i = 0; i < 2 * KS; ++i) g[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-  double lp[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0}, bad[4] = {0, 0, 0, 0};
-
-  const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
-  const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
-  for (int64_t t = t0; t < t1; ++t) {
-    const int64_t n0 = t * 32;
-    // the tile's row sums in FP32 (2 rows per lane), added to FP64 per tile
-    float lpt[4] = {0.f, 0.f, 0.f, 0.f}, srt[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-      f32x4 h = {0.f, 0.f, 0.f, 0.f};
-      const int64_t n = n0 + 16 * rb + lr;
-      const int64_t xoff = n * Kp + 8 * lg;
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const bf16x8 xh = *reinterpret_cast<const bf16x8*>(a.X1h + xoff + 32 * s);
-        const bf16x8 xl = *reinterpret_cast<const bf16x8*>(a.X1l + xoff + 32 * s);
-        h = mma3(bh[s], bl[s], xh, xl, h);
-      }
-      // h[j] = (X B)[n][cb + 4 lg + j]
-      const bool valid = n < a.N;
-      float yv = 0.f;
-      if (valid) {
-        if (FAM == SX_LOGIT)
-          yv = static_cast<const int8_t*>(a.y)[n] ? 1.0f : -1.0f;
-        else
-          yv = static_cast<float>(static_cast<const double*>(a.y)[n]);
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float eta = h[j] + alpha[j];
-        float term, r;
-        if (FAM == SX_LOGIT) {
-          logit_row_fast(eta, yv, &term, &r);
-        } else {
-          r = yv - eta;
-          term = r * r;
-        }
-        term = valid ? term : 0.f;
-        r = valid ? r : 0.f;
-        if (valid && !isfinite(eta)) bad[j] += 1.0;
-        lpt[j] += term;
-        srt[j] += r;
-        rs[w][4 * lg + j][16 * rb + lr] = r;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      lp[j] += static_cast<double>(lpt[j]);
-      sr[j] += static_cast<double>(srt[j]);
-    }
-    // rs[w] belongs to this wave: LDS accesses of a wave complete in order
-    __builtin_amdgcn_wave_barrier();
-    // adjoint A operand: R^T[c = lr][n = 8 lg + j] as hi/lo
-    bf16x8 rh, rl;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      __bf16 hh, ll;
-      split(rs[w][lr][8 * lg + j], &hh, &ll);
-      rh[j] = hh;
-      rl[j] = ll;
-    }
-    const int64_t x2base = ((n0 >> 3) + lg) * Kp;
-#pragma unroll
-    for (int kb = 0; kb < 2 * KS; ++kb) {
-      const int64_t off = (x2base + 16 * kb + lr) * 8;
-      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(a.X2h + off);
-      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(a.X2l + off);
-      g[kb] = mma3(rh, rl, xh, xl, g[kb]);
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  // g[kb][j] = G^T[cb + 4 lg + j][16 kb + lr]
-  const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
-#pragma unroll
-  for (int kb = 0; kb < 2 * KS; ++kb)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      a.gpart[(pbase + cb + 4 * lg + j) * Kp + 16 * kb + lr] = g[kb][j];
-  // row terms: reduce over the 16 lanes that held different rows
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-      lp[j] += __shfl_xor(lp[j], off, 64);
-      sr[j] += __shfl_xor(sr[j], off, 64);
-      bad[j] += __shfl_xor(bad[j], off, 64);
-    }
-  }
-  if (lr == 0) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double* sp = a.spart + (pbase + cb + 4 * lg + j) * 3;
-      sp[0] = lp[j];
-      sp[1] = sr[j];
-      sp[2] = bad[j];
-    }
-  }
-}
-
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of global_load_lds
 
-// K <= 128 (KQ = Kp / 32 <= 4): workgroup = 4 waves x 32 chains = 128
+// K <= 256 (KQ = Kp / 32 <= 8): workgroup = 4 waves x 32 chains = 128
 // chains.  Each 32-row tile of X (both layouts, hi and lo) is staged once
 // per workgroup in LDS, straight from global memory (global_load_lds), and
-// read by all four waves, so X crosses L2 once per 128 chains.  Three LDS
-// buffers carry a software pipeline: while the link and adjoint of tile t
-// run, the forward MFMAs of tile t + 1 (independent of them) interleave
-// with the link's VALU work and tile t + 2 streams in.  v_mfma_f32_32x32x16_bf16 operands (lane
-// l, j < 8): A[l % 32][8 (l / 32) + j], B[8 (l / 32) + j][l % 32]; result
+// read by all four waves, so X crosses L2 once per 128 chains.  A software
+// pipeline over tiles: while the link and adjoint of tile t run, the
+// forward MFMAs of tile t + 1 (independent of them) interleave with the
+// link's VALU work, X1 of tile t + 2 and X2 of tile t + 1 stream in; two
+// buffers per layout suffice.  v_mfma_f32_32x32x16_bf16 operands (lane l,
+// j < 8): A[l % 32][8 (l / 32) + j], B[8 (l / 32) + j][l % 32]; result
 // d[j] = D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
 //   forward  H (32 rows x 32 chains) = X1 tile (A, from LDS) . B (registers)
 //            lane l holds chain l % 32 and rows 8 (j/4) + 4 (l/32) + j%4
 //   adjoint  G^T (32 chains x 32 k) += R^T . X: the contraction runs over
-//            rows in the order the forward left them in the lane (step t
-//            uses the lane's own r[8t .. 8t+7]), and the X operand takes the
-//            same rows from the 8-row-packed tile: rows 16t + 4 (l/32) + 0..3
-//            of groups 2t and 2t+1 (two 8-byte LDS reads).
+//            rows in the order the forward left them in the lane (step st
+//            uses the lane's own r[8st .. 8st+7]), and the X operand takes
+//            the same rows from the 8-row-packed tile: rows 16st + 4 (l/32)
+//            + 0..3 of groups 2st and 2st+1 (two 8-byte LDS reads).
 template <int KQ, int FAM>
 __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
+  static_assert(KQ >= 1 && KQ <= 8, "glm_mfma32 handles 32 <= Kp <= 256");
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;          // forward k-steps of 16
   constexpr int CPR = Kp / 8;          // 16-byte chunks per X1 row
@@ -265,13 +119,15 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
   constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
   constexpr int TE = 32 * Kp;          // elements of one tile array
   constexpr int BPA = Kp / 16;         // 1 KB blocks per tile array
-  __shared__ __attribute__((aligned(16))) __bf16 xt[3][4][TE];  // X1h X1l X2h X2l
-  __shared__ float ys[3][32];  // y of the tile's rows (+-1 for the logit)
+  __shared__ __attribute__((aligned(16))) __bf16 x1[2][2][TE];  // [buffer][hi, lo]
+  __shared__ __attribute__((aligned(16))) __bf16 x2[2][2][TE];
+  __shared__ float ys[2][32];  // y of the tile's rows (+-1 for the logit)
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 31, h = lane >> 5;
-  // XCD-aware 1-D grid (see glm_mfma)
+  // XCD-aware 1-D grid: the splits of one chain block are 8 block ids
+  // apart, so blocks sharing a row range run on the same XCD and L2
   const int nCB = a.Cp / 128;
   const int bid = blockIdx.x;
   const int xcd = bid & 7, widx = bid >> 3;
@@ -301,27 +157,31 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
   const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
   const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
   float ystage = 0.f;
-  // Tile staging straight into LDS (global_load_lds_dwordx4: lane l of a
-  // wave writes 16 bytes at the wave's LDS base + 16 l).  Wave w moves the
-  // 1 KB blocks w, w + 4, ... of the tile (block b: array b / BPA); X1 rows
-  // are stored with chunk c of row r at c ^ (r & SWZ), so the forward's
-  // row-per-lane reads spread over the banks; X2 is copied as is.
-#define SX_DMA(T, BUF)                                                           \
+  // Staging (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at
+  // the wave's LDS base + 16 l).  One layout's tile is 2 BPA blocks of 1 KB
+  // (hi, then lo); wave w moves blocks w, w + 4, ...  X1 rows are stored
+  // with chunk c of row r at c ^ (r & SWZ), so the forward's row-per-lane
+  // reads spread over the banks; X2 is copied as is, with the tile's y.
+#define SX_STAGE_X1(T, BUF)                                                      \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < KQ * 2; ++i) {                         \
+    _Pragma("unroll") for (int i = 0; i < KQ; ++i) {                             \
       const int b_ = w + 4 * i;                                                  \
-      const int arr_ = b_ / BPA;                                                 \
       const int p_ = (b_ % BPA) * 64 + lane;                                     \
-      int src_ = p_;                                                             \
-      if (arr_ < 2) {                                                            \
-        const int r_ = p_ / CPR;                                                 \
-        src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                             \
-      }                                                                          \
-      const __bf16* g_ = arr_ == 0 ? X1h : arr_ == 1 ? X1l : arr_ == 2 ? X2h : X2l; \
-      __builtin_amdgcn_global_load_lds(                                          \
-          g_ + base_ + 8 * src_,                                                 \
-          (lds_ptr)(&xt[BUF][arr_][(b_ % BPA) * 512]), 16, 0, 0);                \
+      const int r_ = p_ / CPR;                                                   \
+      const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? X1h : X1l) + base_ + 8 * src_, \
+          (lds_ptr)(&x1[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
+    }                                                                            \
+  }
+#define SX_STAGE_X2(T, BUF)                                                      \
+  {                                                                              \
+    const int64_t base_ = (T) * 32 * Kp;                                         \
+    _Pragma("unroll") for (int i = 0; i < KQ; ++i) {                             \
+      const int b_ = w + 4 * i;                                                  \
+      const int p_ = (b_ % BPA) * 64 + lane;                                     \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? X2h : X2l) + base_ + 8 * p_,  \
+          (lds_ptr)(&x2[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
     }                                                                            \
     if (tid < 32) {                                                              \
       const int64_t n_ = (T) * 32 + tid;                                         \
@@ -335,47 +195,46 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
       ystage = v_;                                                               \
     }                                                                            \
   }
-#define SX_DMA_DONE(BUF)                                                         \
+  // the wave's staged loads have landed (before the barrier that publishes
+  // them) and the staged y goes to LDS
+#define SX_STAGE_DONE(YBUF, WITH_Y)                                              \
   {                                                                              \
     __builtin_amdgcn_s_waitcnt(0);                                               \
-    if (tid < 32) ys[BUF][tid] = ystage;                                         \
+    if ((WITH_Y) && tid < 32) ys[YBUF][tid] = ystage;                            \
   }
-  // Software pipeline over tiles, three LDS buffers: iteration t runs the
-  // forward MFMAs of tile t + 1 (independent work) alongside the link and
-  // adjoint of tile t, while tile t + 2 streams into the third buffer.
-  // The forward of the tile after the last reads a stale buffer and its
-  // result is discarded.
-  f32x16 hcur, hnext;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
 #define SX_FORWARD(BUF, ACC)                                                     \
   {                                                                              \
     _Pragma("unroll") for (int j = 0; j < 16; ++j) ACC[j] = 0.f;                 \
     _Pragma("unroll") for (int s = 0; s < S16; ++s) {                            \
       const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));               \
-      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&xt[BUF][0][off]);      \
-      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&xt[BUF][1][off]);      \
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1[BUF][0][off]);      \
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1[BUF][1][off]);      \
       ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], ACC, 0, 0, 0);    \
       ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], ACC, 0, 0, 0);    \
       ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], ACC, 0, 0, 0);    \
     }                                                                            \
   }
+  f32x16 hcur, hnext;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
+  // prologue: X1 and X2 of tile t0, X1 of tile t0 + 1; forward of tile t0
   if (t0 < t1) {
-    SX_DMA(t0, 0);
-    SX_DMA_DONE(0);
-    if (t0 + 1 < t1) {
-      SX_DMA(t0 + 1, 1);
-      SX_DMA_DONE(1);
-    }
+    SX_STAGE_X1(t0, 0);
+    SX_STAGE_X2(t0, 0);
+    if (t0 + 1 < t1) SX_STAGE_X1(t0 + 1, 1);
+    SX_STAGE_DONE(0, true);
   }
   __syncthreads();
   if (t0 < t1) SX_FORWARD(0, hcur);
   for (int64_t t = t0; t < t1; ++t) {
-    const int buf = static_cast<int>((t - t0) % 3);
-    const int nbuf = buf == 2 ? 0 : buf + 1;
-    const int fbuf = nbuf == 2 ? 0 : nbuf + 1;
-    if (t + 2 < t1) SX_DMA(t + 2, fbuf);
-    SX_FORWARD(nbuf, hnext);
+    const int cur = static_cast<int>((t - t0) & 1), nxt = cur ^ 1;
+    // x1[cur] held X1 of tile t (its forward ran an iteration ago) and
+    // x2[nxt] X2 of tile t - 1: both free since the last barrier.  The
+    // forward of the tile after the last reads a stale buffer; its result
+    // is discarded.
+    if (t + 2 < t1) SX_STAGE_X1(t + 2, cur);
+    if (t + 1 < t1) SX_STAGE_X2(t + 1, nxt);
+    SX_FORWARD(nxt, hnext);
     // link and adjoint of tile t in two halves of the lane's 16 rows: half
     // st holds elements j = 8 st .. 8 st + 7 (rows 8 (j/4) + 4 h + j%4),
     // the rows the adjoint's step st contracts.  Rows past N are masked.
@@ -391,7 +250,7 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
         const int j = 8 * st + jj;
         const int row = 8 * (j >> 2) + 4 * h + (j & 3);
         const bool valid = n0 + row < a.N;
-        const float yv = ys[buf][row];
+        const float yv = ys[cur][row];
         const float eta = hcur[j] + alpha;
         float term, rr;
         if (FAM == SX_LOGIT) {
@@ -415,10 +274,10 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
         const int k = 32 * kb + li;
         const int o0 = ((2 * st) * Kp + k) * 8 + 4 * h;
         const int o1 = ((2 * st + 1) * Kp + k) * 8 + 4 * h;
-        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&xt[buf][2][o0]);
-        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&xt[buf][2][o1]);
-        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&xt[buf][3][o0]);
-        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&xt[buf][3][o1]);
+        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o0]);
+        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o1]);
+        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o0]);
+        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o1]);
         bf16x8 xh, xl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -435,13 +294,14 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
     lp += static_cast<double>(lpt);
     sr += static_cast<double>(srt);
     bad += static_cast<double>(badt);
-    if (t + 2 < t1) SX_DMA_DONE(fbuf);
+    SX_STAGE_DONE(nxt, t + 1 < t1);
     __syncthreads();
     hcur = hnext;
   }
 #undef SX_FORWARD
-#undef SX_DMA
-#undef SX_DMA_DONE
+#undef SX_STAGE_X1
+#undef SX_STAGE_X2
+#undef SX_STAGE_DONE
   // g[kb][j] = G^T[chain cbase + 8 (j/4) + 4 h + j%4][k = 32 kb + li]
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll
@@ -463,52 +323,76 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
   }
 }
 
-// Per chain: FP64 sums of the split partials (four fixed interleaved slices
-// of splits, eight loads in flight per thread, combined in order), then
-// priors and output.  Entries: gradient partials k < Kp, then lp, sum r, bad.
+// Per chain: FP64 sums of the split partials, then priors and output.  A
+// split's gradient partials are a contiguous row of Kp floats; the block's
+// 1024 threads cover the rows as Kp/4 float4 columns x 4096/Kp slices of
+// splits, every thread active and eight 16-byte loads in flight.  Each
+// thread sums its splits in ascending order, the slices are combined in
+// slice order: a fixed order that depends on N, K and the split count only.
 template <int FAM>
 __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   extern __shared__ double st[];
-  __shared__ double red[4][256];
+  __shared__ double red[4096];  // [slice][k], Kp x slices = 4096
+  __shared__ double rs[3][16];
   const int c = blockIdx.x;
   pa.theta += static_cast<int64_t>(c) * pa.theta_stride;
   pa.out += static_cast<int64_t>(c) * pa.out_stride;
-  const int K = pa.K;
-  const int sl = threadIdx.x >> 8;
-  const int nE = K + 3;
-  for (int e0 = 0; e0 < nE; e0 += 256) {
-    const int e = e0 + (threadIdx.x & 255);
-    if (e < nE) {
-      // entry e: gradient k = e (e < K), or spart component e - K
-      const bool g = e < K;
-      const float* gp = a.gpart + static_cast<int64_t>(c) * a.Kp + e;
-      const double* sp = a.spart + static_cast<int64_t>(c) * 3 + (g ? 0 : e - K);
-      const int64_t gstride = static_cast<int64_t>(a.Cp) * a.Kp, sstride = 3 * static_cast<int64_t>(a.Cp);
-      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      int sidx = sl;
-      for (; sidx + 28 < a.splits; sidx += 32) {
+  const int K = pa.K, Kp = a.Kp;
+  const int q = Kp / 4;               // float4 columns of a row
+  const int nsl = 1024 / q;           // slices of splits (q * nsl <= 1024)
+  const int tid = threadIdx.x;
+  const int kq = tid % q, sl = tid / q;
+  const int64_t gstride = static_cast<int64_t>(a.Cp) * Kp;
+  const float* gp = a.gpart + static_cast<int64_t>(c) * Kp + 4 * kq;
+  double v[4] = {0, 0, 0, 0};
+  // threads past q * nsl (Kp not dividing 4096) take no splits
+  int si = sl < nsl ? sl : a.splits;
+  for (; si + 7 * nsl < a.splits; si += 8 * nsl) {
+    float4 x[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int64_t si = sidx + 4 * u;
-          v[u] += g ? static_cast<double>(gp[si * gstride]) : sp[si * sstride];
-        }
-      }
-      for (int u = 0; sidx < a.splits; sidx += 4, ++u) {
-        const int64_t si = sidx;
-        v[u] += g ? static_cast<double>(gp[si * gstride]) : sp[si * sstride];
-      }
-      red[sl][threadIdx.x & 255] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    for (int u = 0; u < 8; ++u)
+      x[u] = *reinterpret_cast<const float4*>(gp + static_cast<int64_t>(si + u * nsl) * gstride);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      v[0] += static_cast<double>(x[u].x);
+      v[1] += static_cast<double>(x[u].y);
+      v[2] += static_cast<double>(x[u].z);
+      v[3] += static_cast<double>(x[u].w);
     }
-    __syncthreads();
-    if (threadIdx.x < 256 && e0 + static_cast<int>(threadIdx.x) < nE) {
-      const int ee = e0 + threadIdx.x;
-      const double t = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-      // finalize layout: st[0] = lp, st[1] = sum r, st[2 + k] = grad k, st[K + 2] = bad
-      st[ee < K ? 2 + ee : (ee == K ? 0 : (ee == K + 1 ? 1 : K + 2))] = t;
-    }
-    __syncthreads();
   }
-  if (threadIdx.x < 64) finalize<FAM>(pa, st);
+  for (; si < a.splits; si += nsl) {
+    const float4 x = *reinterpret_cast<const float4*>(gp + static_cast<int64_t>(si) * gstride);
+    v[0] += static_cast<double>(x.x);
+    v[1] += static_cast<double>(x.y);
+    v[2] += static_cast<double>(x.z);
+    v[3] += static_cast<double>(x.w);
+  }
+  if (sl < nsl)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[sl * Kp + 4 * kq + i] = v[i];
+  // row terms (lp, sum r, bad): wave 0, entry e on lanes 16 e .. 16 e + 15,
+  // each lane over splits lane, lane + 16, ... then lanes in order
+  if (tid < 48) {
+    const int e = tid >> 4, l = tid & 15;
+    const double* sp = a.spart + static_cast<int64_t>(c) * 3 + e;
+    const int64_t sstride = 3 * static_cast<int64_t>(a.Cp);
+    double t = 0.0;
+    for (int s2 = l; s2 < a.splits; s2 += 16) t += sp[s2 * sstride];
+    rs[e][l] = t;
+  }
+  __syncthreads();
+  if (tid < K) {
+    double t = 0.0;
+    for (int s2 = 0; s2 < nsl; ++s2) t += red[s2 * Kp + tid];
+    st[2 + tid] = t;  // finalize layout: st[0] lp, st[1] sum r, st[2 + k], st[K + 2] bad
+  } else if (tid >= 1021) {
+    const int e = tid - 1021;
+    double t = 0.0;
+    for (int l = 0; l < 16; ++l) t += rs[e][l];
+    st[e == 0 ? 0 : (e == 1 ? 1 : K + 2)] = t;
+  }
+  __syncthreads();
+  if (tid < 64) finalize<FAM>(pa, st);
 }
 
 template <int KQ>
@@ -519,18 +403,6 @@ hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
     hipLaunchKernelGGL((glm_mfma32<KQ, SX_LOGIT>), grid, dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL((glm_mfma32<KQ, SX_NORMAL>), grid, dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-template <int KS>
-hipError_t launch_ks(int family, const mx_args& a, hipStream_t s) {
-  // splits is a multiple of 8 (the XCD-aware block mapping relies on it)
-  if (a.splits % 8 != 0) return hipErrorInvalidValue;
-  dim3 grid(a.splits * (a.Cp / 64));
-  if (family == SX_LOGIT)
-    hipLaunchKernelGGL((glm_mfma<KS, SX_LOGIT>), grid, dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((glm_mfma<KS, SX_NORMAL>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -556,23 +428,17 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
                      const_cast<float*>(a.alpha));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (a.Kp <= 128 && a.Cp % 128 == 0) {
-    switch (a.Kp / 32) {
-      case 1: e = launch_kq32<1>(family, a, s); break;
-      case 2: e = launch_kq32<2>(family, a, s); break;
-      case 3: e = launch_kq32<3>(family, a, s); break;
-      case 4: e = launch_kq32<4>(family, a, s); break;
-      default: return hipErrorInvalidValue;
-    }
-  } else switch (a.Kp / 32) {
-    case 1: e = launch_ks<1>(family, a, s); break;
-    case 2: e = launch_ks<2>(family, a, s); break;
-    case 3: e = launch_ks<3>(family, a, s); break;
-    case 4: e = launch_ks<4>(family, a, s); break;
-    case 5: e = launch_ks<5>(family, a, s); break;
-    case 6: e = launch_ks<6>(family, a, s); break;
-    case 7: e = launch_ks<7>(family, a, s); break;
-    case 8: e = launch_ks<8>(family, a, s); break;
+  // chains come padded to 128 (four waves of 32) and K to a multiple of 32
+  if (a.Cp % 128 != 0) return hipErrorInvalidValue;
+  switch (a.Kp / 32) {
+    case 1: e = launch_kq32<1>(family, a, s); break;
+    case 2: e = launch_kq32<2>(family, a, s); break;
+    case 3: e = launch_kq32<3>(family, a, s); break;
+    case 4: e = launch_kq32<4>(family, a, s); break;
+    case 5: e = launch_kq32<5>(family, a, s); break;
+    case 6: e = launch_kq32<6>(family, a, s); break;
+    case 7: e = launch_kq32<7>(family, a, s); break;
+    case 8: e = launch_kq32<8>(family, a, s); break;
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
