@@ -157,7 +157,8 @@ def serving_section(n_jobs, target_ess=1000.0, warmup=200, deadline_s=120.0):
             if srv.wait(i, 0.0):
                 inf = srv.info(i)
                 log(f"  job {i}: {inf['t_done'] - inf['t_submit']:.1f} s, {inf['steps']} steps, "
-                    f"min ESS {inf['min_ess']:.0f}, {inf['extensions']} extensions")
+                    f"min ESS {inf['min_ess']:.0f}, {inf['extensions']} extensions, "
+                    f"{'BF16 hi+lo' if inf['grad_precision'] == S.BF16X2 else 'FP64'}")
             else:
                 still.append(i)
         pending = still
@@ -188,7 +189,7 @@ def serving_section(n_jobs, target_ess=1000.0, warmup=200, deadline_s=120.0):
                 jobs_reaching_target=int(sum(e >= target_ess for e in ess)),
                 min_ess_min=float(np.nanmin(ess)),
                 extensions=int(sum(i["extensions"] for i in infos)),
-                mfma_jobs=int(sum(1 for j in jobs if j["K"] <= 256 and j["C"] >= 64)))
+                mfma_jobs=int(sum(1 for i in infos if i["grad_precision"] == S.BF16X2)))
 
 
 def mixed_error(model, ref_data, family, thetas):

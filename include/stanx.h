@@ -404,6 +404,9 @@ typedef struct sx_job_info {
   int64_t sampling_iterations;
   int extensions;
   double min_ess;             /* min bulk ESS over parameters at completion    */
+  int grad_precision;         /* SX_FP64 / SX_FP32 / SX_BF16X2 the job runs with:
+                                 BF16 hi+lo when the job allows it and it costs
+                                 less than the FP64 passes                      */
 } sx_job_info;
 
 int sx_server_create(sx_server** out, int device, int64_t hbm_budget_bytes,

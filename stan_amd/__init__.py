@@ -82,7 +82,7 @@ class JobInfo(C.Structure):
                 ("n_grad", C.c_int64), ("steps", C.c_int64), ("D", C.c_int64),
                 ("chains", C.c_int64), ("rows_cap", C.c_int64),
                 ("sampling_iterations", C.c_int64), ("extensions", C.c_int),
-                ("min_ess", C.c_double)]
+                ("min_ess", C.c_double), ("grad_precision", C.c_int)]
 
 
 class NutsResult(C.Structure):

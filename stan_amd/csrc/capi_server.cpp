@@ -77,14 +77,17 @@ struct job {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// The MFMA pass computes whole chain tiles (64 or 128 chains): with fewer
-// chains the FP64 per-chain passes cost less, and are exact.
-constexpr int64_t kMinMfmaChains = 64;
-
+// A job that allows BF16 hi+lo gradients gets them when they cost less
+// than the exact FP64 passes.  The MFMA pass computes whole tiles of 128
+// chains at about 0.09 ns + 2.5 ps x Kp per row and tile (measured at
+// N = 1e6, K = 100 and N = 2e5, K = 256); the FP64 passes stream 8 K
+// bytes per row and chain, 1.25 ps x K per row and chain at 6.4 TB/s.
+// So the MFMA pass wins when chains x K > (72 + 2 Kp) per tile.
 int gradient_precision(const sx_job_spec& s) {
   if (s.family == SX_HIER_LOGIT) return s.grad_precision == SX_FP32 ? SX_FP32 : SX_FP64;
-  return (s.grad_precision == SX_BF16X2 && s.K <= 256 && s.chains >= kMinMfmaChains) ? SX_BF16X2
-                                                                                      : SX_FP64;
+  if (s.grad_precision != SX_BF16X2 || s.K > 256) return SX_FP64;
+  const int64_t Kp = std::max<int64_t>(32, ceil_div(s.K, 32) * 32);
+  return s.chains * s.K > ceil_div(s.chains, 128) * (72 + 2 * Kp) ? SX_BF16X2 : SX_FP64;
 }
 
 // Device bytes a job holds while resident: data layouts, the sampler's
@@ -470,6 +473,7 @@ int sx_server_info(sx_server* srv, int64_t job_id, sx_job_info* info) {
   info->sampling_iterations = j->samples;
   info->extensions = j->extensions;
   info->min_ess = j->min_ess;
+  info->grad_precision = gradient_precision(j->spec);
   return SX_OK;
 }
 
