@@ -115,11 +115,17 @@ __device__ __forceinline__ void logit_row_f32s(float e, float s, float* term, fl
 }
 
 // FP32, many chains: a wave per chunk, chains on lanes; the chunk's rows
-// are staged 64 at a time in the wave's LDS and read as broadcasts.
+// are staged 64 at a time in the wave's LDS (rows padded to KMAX floats)
+// and read as float4 broadcasts.  The next piece's global loads are in
+// flight (in registers) while the current piece is computed; rows go four
+// at a time, so four independent FMA chains hide the FMA and LDS latency.
+// Padding rows (x = 0, sign 0) add nothing.
 template <int KMAX>
 __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
-  __shared__ float sx[4][65 * KMAX];
-  __shared__ float sy[4][65];
+  static_assert(KMAX % 4 == 0, "rows are read as float4");
+  constexpr int PR = 64;  // rows per piece
+  __shared__ __attribute__((aligned(16))) float sx[4][PR * KMAX];
+  __shared__ float sy[4][PR];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int K = a.K;
@@ -147,51 +153,95 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
   float s1 = 0.f;
   double s0 = 0.0;
   int bad = 0;
-  for (int p0 = 0; p0 < rows; p0 += 64) {
-    const int cnt = rows - p0 < 64 ? rows - p0 : 64;
-    const float* src = a.X32 + (r0 + p0) * K;
-    // LDS accesses of one wave complete in order: the reads of the
-    // previous piece are done before these writes land
+  // staging: lane `lane` loads elements lane + 64 j (j < KMAX) of the
+  // piece's contiguous rows (coalesced, one base address) and stores them
+  // at their padded LDS positions (row * KMAX + column), computed
+  // incrementally: adding 64 to an element index adds 64 / K rows and
+  // 64 % K columns (with a carry)
+  float stg[KMAX];
+  float ystg = 0.f;
+  int lim = 0;  // elements in the staged piece (cnt * K)
+  const int dr = 64 / K, dk = 64 % K;
+  const int row0 = lane / K, col0 = lane % K;
+#define SX_LOAD_PIECE(P0)                                                        \
+  {                                                                              \
+    const int cnt_ = rows - (P0) < PR ? rows - (P0) : PR;                        \
+    const float* src_ = a.X32 + (r0 + (P0)) * K + lane;                          \
+    lim = cnt_ * K;                                                              \
+    _Pragma("unroll") for (int j = 0; j < KMAX; ++j)                             \
+      stg[j] = lane + 64 * j < lim ? src_[64 * j] : 0.f;                         \
+    ystg = lane < cnt_ ? (a.y[r0 + (P0) + lane] ? 1.0f : -1.0f) : 0.f;           \
+  }
+  SX_LOAD_PIECE(0);
+  for (int p0 = 0; p0 < rows; p0 += PR) {
+    const int cnt = rows - p0 < PR ? rows - p0 : PR;
+    // LDS accesses of one wave complete in order: the previous piece's
+    // reads are done before these writes land
     __builtin_amdgcn_wave_barrier();
-    for (int i = lane; i < cnt * K; i += 64) sx[w][i] = src[i];
-    if (lane < cnt) sy[w][lane] = a.y[r0 + p0 + lane] ? 1.0f : -1.0f;
+    // zero the padded piece, then place the staged elements
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) sx[w][lane + 64 * j] = 0.f;
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    // two rows per iteration (independent dependency chains); an odd last
-    // row pairs with a zero row of sign 0, which adds nothing
-    if (cnt & 1) {
-      for (int k = lane; k < K; k += 64) sx[w][cnt * K + k] = 0.f;
-      if (lane == 0) sy[w][cnt] = 0.f;
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
-    }
-    float s0p = 0.f;  // the piece's row terms in FP32, added to FP64 per piece
-    for (int i = 0; i < cnt; i += 2) {
-      const float* x0 = &sx[w][i * K];
-      const float* x1 = x0 + K;
-      float e0 = b, e1 = b;
+    {
+      int row = row0, col = col0;
 #pragma unroll
-      for (int k = 0; k < KMAX; ++k)
-        if (k < K) {
-          e0 = fmaf(x0[k], wk[k], e0);
-          e1 = fmaf(x1[k], wk[k], e1);
+      for (int j = 0; j < KMAX; ++j) {
+        if (lane + 64 * j < lim) sx[w][row * KMAX + col] = stg[j];
+        row += dr;
+        col += dk;
+        if (col >= K) {
+          col -= K;
+          ++row;
         }
-      const float sg0 = sy[w][i], sg1 = sy[w][i + 1];
-      float term0, r0, term1, r1;
-      logit_row_f32s(e0, sg0, &term0, &r0);
-      logit_row_f32s(e1, sg1, &term1, &r1);
-      bad += isfinite(e0) ? 0 : 1;
-      bad += (sg1 != 0.f && !isfinite(e1)) ? 1 : 0;
-      s0p += term0;
-      s0p += term1;
-      s1 += r0;
-      s1 += r1;
+      }
+    }
+    sy[w][lane] = ystg;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (p0 + PR < rows) SX_LOAD_PIECE(p0 + PR);  // in flight during this piece
+    float s0p = 0.f;  // the piece's row terms in FP32, added to FP64 per piece
+    for (int i = 0; i < cnt; i += 4) {
+      float e[4] = {b, b, b, b};
 #pragma unroll
-      for (int k = 0; k < KMAX; ++k)
-        if (k < K) acc[k] = fmaf(x1[k], r1, fmaf(x0[k], r0, acc[k]));
+      for (int q = 0; q < KMAX / 4; ++q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float4 xv = *reinterpret_cast<const float4*>(&sx[w][(i + r) * KMAX + 4 * q]);
+          e[r] = fmaf(xv.x, wk[4 * q], e[r]);
+          e[r] = fmaf(xv.y, wk[4 * q + 1], e[r]);
+          e[r] = fmaf(xv.z, wk[4 * q + 2], e[r]);
+          e[r] = fmaf(xv.w, wk[4 * q + 3], e[r]);
+        }
+      }
+      float rr[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float sg = sy[w][i + r];
+        float term;
+        logit_row_f32s(e[r], sg, &term, &rr[r]);
+        bad += (sg != 0.f && !isfinite(e[r])) ? 1 : 0;
+        s0p += term;
+        s1 += rr[r];
+      }
+      // re-read the rows from LDS for the adjoint rather than keep 4 x KMAX
+      // floats live across the link (register pressure sets occupancy)
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int q = 0; q < KMAX / 4; ++q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float4 xv = *reinterpret_cast<const float4*>(&sx[w][(i + r) * KMAX + 4 * q]);
+          acc[4 * q] = fmaf(xv.x, rr[r], acc[4 * q]);
+          acc[4 * q + 1] = fmaf(xv.y, rr[r], acc[4 * q + 1]);
+          acc[4 * q + 2] = fmaf(xv.z, rr[r], acc[4 * q + 2]);
+          acc[4 * q + 3] = fmaf(xv.w, rr[r], acc[4 * q + 3]);
+        }
+      }
     }
     s0 += static_cast<double>(s0p);
   }
+#undef SX_LOAD_PIECE
   if (!live) return;
   double* out = a.cpart + (ch * a.chains + c) * a.E;
   out[0] = s1;
