@@ -97,6 +97,8 @@ void release(sx_nuts_batch* b) {
   (void)hipFree(b->a.vecs);
   (void)hipFree(b->a.draws);
   (void)hipFree(b->a.done_count);
+  (void)hipFree(b->a.xch);
+  (void)hipFree(b->a.xcnt);
   (void)hipFree(b->d_partials);
   (void)hipFree(b->d_out);
   (void)hipFree(b->d_theta);
@@ -117,7 +119,7 @@ namespace nb {
 
 int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int64_t chains,
                  int grad_precision, const double* inits, const double* inv_metric,
-                 int64_t max_samples) {
+                 int64_t max_samples, int max_parts) {
   if (!out || !m || !cfg || chains <= 0) {
     set_error("sx_nuts_batch_create: invalid argument");
     return SX_EINVAL;
@@ -173,6 +175,37 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
       hipMalloc(&b->d_out, sizeof(double) * chains * (D + 2)) != hipSuccess ||
       hipHostMalloc(&b->h_done, sizeof(unsigned)) != hipSuccess)
     return fail(SX_ENOMEM, "device memory");
+  // Parts per chain: a chain with D > 8192 (one 512-thread workgroup per
+  // chain streams its vectors through one CU) may spread over up to 4
+  // workgroups, as long as every workgroup of the launch can be resident
+  // at once: chains x parts <= CUs (one 512-thread workgroup per CU).
+  // STANX_NUTS_PARTS=n sets the count (1 disables).
+  {
+    int parts = 1;
+    if (max_parts != 1 && D > 8192) {
+      int cus = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device) !=
+          hipSuccess)
+        cus = 0;
+      parts = static_cast<int>(std::min<int64_t>(4, cus / std::max<int64_t>(1, chains)));
+      if (const char* e = std::getenv("STANX_NUTS_PARTS")) parts = std::atoi(e);
+      if (max_parts > 0) parts = std::min(parts, max_parts);
+      if (cus > 0 && chains * parts > cus) parts = static_cast<int>(cus / chains);
+      parts = std::max(1, std::min(parts, 4));
+    }
+    b->a.parts = parts;
+    b->a.spin_limit = 1 << 22;
+    if (parts > 1) {
+      const size_t nx = static_cast<size_t>(chains) * 16 * parts;
+      if (hipMalloc(&b->a.xch, sizeof(double) * nx) != hipSuccess ||
+          hipMalloc(&b->a.xcnt, sizeof(unsigned long long) * chains) != hipSuccess)
+        return fail(SX_ENOMEM, "exchange buffers");
+      if (hipMemsetAsync(b->a.xch, 0, sizeof(double) * nx, b->stream) != hipSuccess ||
+          hipMemsetAsync(b->a.xcnt, 0, sizeof(unsigned long long) * chains, b->stream) !=
+              hipSuccess)
+        return fail(SX_EDEVICE, "clear");
+    }
+  }
   if (hipMemsetAsync(b->a.vecs, 0, sizeof(double) * nvec, b->stream) != hipSuccess ||
       hipMemsetAsync(b->a.done_count, 0, sizeof(unsigned), b->stream) != hipSuccess ||
       hipMemsetAsync(b->d_theta, 0, sizeof(double) * chains * D, b->stream) != hipSuccess ||
@@ -406,7 +439,7 @@ int sx_nuts_batch_create(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* c
     return SX_EINVAL;
   }
   return sx::nb::create_batch(out, m, cfg, chains, grad_precision, inits, inv_metric,
-                              cfg->num_samples);
+                              cfg->num_samples, 0);
 }
 
 void sx_nuts_batch_destroy(sx_nuts_batch* b) { release(b); }

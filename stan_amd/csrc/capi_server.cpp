@@ -202,8 +202,10 @@ int start_job(sx_server* srv, job* j) {
   sx_glm_num_params(j->m, &D);
   j->D = D;
   j->samples = s.cfg.num_samples;
+  // one workgroup per chain: jobs share the device, so a chain spread
+  // over several workgroups could not count on their co-residency
   rc = sx::nb::create_batch(&j->b, j->m, &s.cfg, s.chains, prec, nullptr, nullptr,
-                            std::max<int64_t>(s.cfg.num_samples, s.max_samples));
+                            std::max<int64_t>(s.cfg.num_samples, s.max_samples), 1);
   if (rc) return rc;
   j->rows_cap = sx_nuts_batch_rows_cap(j->b);
   return SX_OK;

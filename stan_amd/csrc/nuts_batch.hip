@@ -76,6 +76,10 @@ struct ctx {
   int nt;       // threads per chain
   double* red;  // LDS scratch for block reductions (one slot per wave)
   uint32_t chain_id;
+  const int* sl;  // LDS: the part's slice [sl[0], sl[1]) of the D-vectors, sl[2] = part
+  __device__ int d0() const { return sl[0]; }
+  __device__ int d1() const { return sl[1]; }
+  __device__ int part() const { return sl[2]; }
   __device__ double* slot(int i) const { return v + static_cast<int64_t>(i) * a->Dpad; }
 };
 
@@ -84,20 +88,68 @@ __device__ __forceinline__ double wave_sum(double x) {
   for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
   return x;
 }
-// Sum over the chain's block (fixed order: waves, then wave 0..nw-1), the
-// result in every thread.  Called in block-uniform control flow.
+// Sum of NS block totals over the chain's parts, in part order, so every
+// part gets the same values.  Lane 0 publishes this part's totals (slot
+// epoch % 2), counts itself in, waits for the other parts and reads all
+// totals; a wait longer than spin_limit marks the exchange failed (the
+// step loop then ends the chain) rather than hanging.  Called in
+// block-uniform control flow by every part in the same sequence.
+template <int NS>
+__device__ void part_sum(const ctx& k, double (&x)[NS]) {
+  const batch_args* a = k.a;
+  const int P = a->parts;
+  chain_state* s = k.s;
+  __syncthreads();
+  if (k.lane == 0) {
+    const uint64_t ep = s->xepoch;
+    double* base = a->xch + static_cast<int64_t>(k.c) * 16 * P + (ep & 1) * 8 * P;
+    unsigned long long* cnt = a->xcnt + k.c;
+    for (int j = 0; j < NS; ++j)
+      __hip_atomic_store(base + 8 * k.part() + j, x[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(cnt, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long target = (ep + 1) * static_cast<unsigned long long>(P);
+    int spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > a->spin_limit) {
+        s->xfail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    for (int j = 0; j < NS; ++j) {
+      double t = 0.0;
+      for (int p = 0; p < P; ++p)
+        t += __hip_atomic_load(base + 8 * p + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      k.red[j] = t;
+    }
+    s->xepoch = ep + 1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NS; ++j) x[j] = k.red[j];
+  __syncthreads();
+}
 
+// Sum over the chain (fixed order: waves, then wave 0..nw-1, then parts in
+// order), the result in every thread.  Called in block-uniform control flow.
 __device__ __forceinline__ double block_sum(const ctx& k, double x) {
   x = wave_sum(x);
-  if (k.nt == 64) return x;
-  const int w = k.lane >> 6;
-  __syncthreads();
-  if ((k.lane & 63) == 0) k.red[w] = x;
-  __syncthreads();
-  double t = 0.0;
-  for (int i = 0; i < (k.nt >> 6); ++i) t += k.red[i];
-  __syncthreads();
-  return t;
+  if (k.nt > 64) {
+    const int w = k.lane >> 6;
+    __syncthreads();
+    if ((k.lane & 63) == 0) k.red[w] = x;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < (k.nt >> 6); ++i) t += k.red[i];
+    __syncthreads();
+    x = t;
+  }
+  if (k.a->parts > 1) {
+    double v[1] = {x};
+    part_sum<1>(k, v);
+    x = v[0];
+  }
+  return x;
 }
 
 // ---- vector helpers (threads stride over D) ----
@@ -107,8 +159,8 @@ __device__ __forceinline__ double block_sum(const ctx& k, double x) {
 constexpr int kU = 4;
 template <class T, class Ld, class St>
 __device__ __forceinline__ void vpass(const ctx& k, Ld ld, St st) {
-  const int D = k.a->D, nt = k.nt;
-  for (int i0 = k.lane; i0 < D; i0 += kU * nt) {
+  const int D = k.d1(), nt = k.nt;
+  for (int i0 = k.d0() + k.lane; i0 < D; i0 += kU * nt) {
     T v[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -126,9 +178,9 @@ __device__ __forceinline__ void vpass(const ctx& k, Ld ld, St st) {
 // (the loads of kU indices issued together).
 template <class Ld>
 __device__ __forceinline__ double vsum(const ctx& k, Ld term) {
-  const int D = k.a->D, nt = k.nt;
+  const int D = k.d1(), nt = k.nt;
   double acc = 0.0;
-  for (int i0 = k.lane; i0 < D; i0 += kU * nt) {
+  for (int i0 = k.d0() + k.lane; i0 < D; i0 += kU * nt) {
     double t[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -147,8 +199,8 @@ __device__ __forceinline__ double vsum(const ctx& k, Ld term) {
 // bitwise)
 template <int NS, class T, class Ld, class St>
 __device__ __forceinline__ void vpass_acc(const ctx& k, double (&acc)[NS], Ld ld, St st) {
-  const int D = k.a->D, nt = k.nt;
-  for (int i0 = k.lane; i0 < D; i0 += kU * nt) {
+  const int D = k.d1(), nt = k.nt;
+  for (int i0 = k.d0() + k.lane; i0 < D; i0 += kU * nt) {
     T v[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -167,20 +219,22 @@ template <int NS>
 __device__ __forceinline__ void block_sum_n(const ctx& k, double (&x)[NS]) {
 #pragma unroll
   for (int j = 0; j < NS; ++j) x[j] = wave_sum(x[j]);
-  if (k.nt == 64) return;
-  const int w = k.lane >> 6;
-  __syncthreads();
-  if ((k.lane & 63) == 0)
+  if (k.nt > 64) {
+    const int w = k.lane >> 6;
+    __syncthreads();
+    if ((k.lane & 63) == 0)
 #pragma unroll
-    for (int j = 0; j < NS; ++j) k.red[8 * j + w] = x[j];
-  __syncthreads();
+      for (int j = 0; j < NS; ++j) k.red[8 * j + w] = x[j];
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    double t = 0.0;
-    for (int i = 0; i < (k.nt >> 6); ++i) t += k.red[8 * j + i];
-    x[j] = t;
+    for (int j = 0; j < NS; ++j) {
+      double t = 0.0;
+      for (int i = 0; i < (k.nt >> 6); ++i) t += k.red[8 * j + i];
+      x[j] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  if (k.a->parts > 1) part_sum<NS>(k, x);
 }
 
 __device__ __forceinline__ void vcopy(const ctx& k, int dst, int src) {
@@ -190,7 +244,7 @@ __device__ __forceinline__ void vcopy(const ctx& k, int dst, int src) {
 }
 __device__ __forceinline__ void vzero(const ctx& k, int dst) {
   double* d = k.slot(dst);
-  for (int i = k.lane; i < k.a->D; i += k.nt) d[i] = 0.0;
+  for (int i = k.d0() + k.lane; i < k.d1(); i += k.nt) d[i] = 0.0;
 }
 // sum_i p_sharp_i * rho_i with p_sharp = minv * p
 __device__ __forceinline__ double dot_sharp(const ctx& k, int p, int rho) {
@@ -233,7 +287,7 @@ __device__ void sample_p(const ctx& k) {
   double* p = k.slot(ZP);
   const double* m = k.slot(MINV);
   const uint64_t base = k.s->rng;
-  for (int i = k.lane; i < k.a->D; i += k.nt)
+  for (int i = k.d0() + k.lane; i < k.d1(); i += k.nt)
     p[i] = stanx::philox_normal(k.a->seed, k.chain_id, base + i) / sqrt(m[i]);
   __syncthreads();
   if (k.lane == 0) k.s->rng = base + k.a->D;
@@ -565,8 +619,13 @@ __device__ int learn_variance(const ctx& k) {
               m[i] = 0.0;
               m2[i] = 0.0;
             });
-  const int all_finite = __syncthreads_and(finite);
+  int all_finite = __syncthreads_and(finite);
   __syncthreads();
+  if (k.a->parts > 1) {
+    double nbad[1] = {all_finite ? 0.0 : 1.0};
+    part_sum<1>(k, nbad);
+    all_finite = nbad[0] == 0.0;
+  }
   if (k.lane == 0) {
     s->w_next = next;
     s->w_size = size;
@@ -581,7 +640,7 @@ __device__ void write_draw(const ctx& k) {
   const batch_args* a = k.a;
   if (s->rows >= a->rows_cap) return;
   double* row = a->draws + (static_cast<int64_t>(k.c) * a->rows_cap + s->rows) * a->ncol;
-  if (k.lane == 0) {
+  if (k.lane == 0 && k.part() == 0) {
     row[0] = -s->V_samp;
     row[1] = s->accept;
     row[2] = s->eps;
@@ -601,7 +660,7 @@ __device__ void finish(const ctx& k, int status) {
     k.s->status = status;
     k.s->phase = PH_DONE;
     k.s->pending = 0;
-    atomicAdd(k.a->done_count, 1u);
+    if (k.part() == 0) atomicAdd(k.a->done_count, 1u);
   }
 }
 
@@ -628,7 +687,22 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   __shared__ chain_state s;
   __shared__ batch_args sa;  // helpers read the arguments through ctx::a
   __shared__ double red[48];  // block_sum_n: 8 waves x up to 6 values
-  const int c = blockIdx.x;
+  __shared__ int slice[3];    // d0, d1, part
+  // chain and part of this workgroup; with C a multiple of 8 the parts of
+  // a chain are 8 block ids apart, i.e. on one XCD (shared L2)
+  const int P = a.parts;
+  const int bid = blockIdx.x;
+  int c, part;
+  if (P > 1 && a.C % 8 == 0) {
+    part = (bid >> 3) % P;
+    c = (bid & 7) + 8 * ((bid >> 3) / P);
+  } else {
+    part = bid % P;
+    c = bid / P;
+  }
+  // block-uniform: keep them (and every address derived from them) scalar
+  c = __builtin_amdgcn_readfirstlane(c);
+  part = __builtin_amdgcn_readfirstlane(part);
   const int lane = threadIdx.x;
   const int nt = blockDim.x;
   {
@@ -636,10 +710,18 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
     int* dst = reinterpret_cast<int*>(&s);
     for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += nt) dst[i] = src[i];
   }
-  if (lane == 0) sa = a;
+  if (lane == 0) {
+    sa = a;
+    // slice of the part: whole multiples of 64 elements, the last ends at D
+    const int span = ((a.D + P - 1) / P + 63) & ~63;
+    const int d0 = part * span < a.D ? part * span : a.D;
+    slice[0] = d0;
+    slice[1] = d0 + span < a.D ? d0 + span : a.D;
+    slice[2] = part;
+  }
   __syncthreads();
   ctx k{&sa, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane, nt, red,
-        a.chain0 + static_cast<uint32_t>(c)};
+        a.chain0 + static_cast<uint32_t>(c), slice};
   if (s.phase == PH_DONE) return;
   if (s.pending) {
     // PH_LEAF and PH_SS_B begin with the leapfrog's second half
@@ -651,7 +733,7 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   }
   __syncthreads();
   // advance until a gradient is requested or the chain finishes
-  for (int guard = 0; guard < 4096 && !s.pending && s.phase != PH_DONE; ++guard) {
+  for (int guard = 0; guard < 4096 && !s.pending && s.phase != PH_DONE && !s.xfail; ++guard) {
     const int ph = s.phase;
     __syncthreads();
     switch (ph) {
@@ -660,10 +742,10 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         double* q = k.slot(ZQ);
         if (a.inits) {
           const double* src = a.inits + static_cast<int64_t>(c) * a.D;
-          for (int i = lane; i < a.D; i += k.nt) q[i] = src[i];
+          for (int i = k.d0() + lane; i < k.d1(); i += k.nt) q[i] = src[i];
         } else {
           const uint64_t base = s.rng;
-          for (int i = lane; i < a.D; i += k.nt)
+          for (int i = k.d0() + lane; i < k.d1(); i += k.nt)
             q[i] = a.init_radius == 0.0
                        ? 0.0
                        : -a.init_radius + stanx::philox_uniform(a.seed, k.chain_id, base + i) *
@@ -998,8 +1080,10 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
     }
     __syncthreads();
   }
+  // a part did not answer an exchange: end the chain
+  if (s.xfail && s.phase != PH_DONE) finish(k, 70);
   __syncthreads();
-  {
+  if (part == 0) {  // every part holds the same state; one writes it back
     const int* src = reinterpret_cast<const int*>(&s);
     int* dst = reinterpret_cast<int*>(a.states + c);
     for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += nt) dst[i] = src[i];
@@ -1007,9 +1091,10 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
 }
 
 hipError_t launch_nuts_step(const batch_args& a, hipStream_t s) {
-  // one wave per chain for small D, up to 8 waves for large D
+  // one wave per chain for small D, up to 8 waves for large D; parts
+  // workgroups per chain
   const int nt = a.D <= 2048 ? 64 : (a.D <= 8192 ? 256 : 512);
-  hipLaunchKernelGGL(nuts_step, dim3(a.C), dim3(nt), 0, s, a);
+  hipLaunchKernelGGL(nuts_step, dim3(a.C * a.parts), dim3(nt), 0, s, a);
   return hipGetLastError();
 }
 

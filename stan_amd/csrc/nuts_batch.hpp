@@ -35,7 +35,9 @@ struct chain_state {
   int stage[kMaxDepth + 1];
   unsigned w_counter, w_next, w_size;
   int iter_end;  // warm-up + sampling iterations; raised to extend a run
+  int xfail;     // a cross-part exchange timed out (parts > 1)
   uint64_t rng;
+  uint64_t xepoch;  // cross-part exchanges done (parts > 1)
   int64_t n_grad;
   double nom_eps, eps, H0, lsw, lsw_sub, smp, accept;
   double V_z, V_fwd, V_bck, V_samp, T_samp, V_prop, T_prop, V_init, ss_H0, last_lp;
@@ -58,6 +60,16 @@ struct batch_args {
   int num_warmup, num_samples, num_thin, save_warmup, max_depth, max_init_tries;
   double init_radius, jitter, delta, gamma, kappa, t0, max_deltaH;
   unsigned w_num_warmup, w_init_buffer, w_term_buffer;
+  // Parts per chain: with parts > 1 each chain runs on `parts` workgroups,
+  // each on a slice of the D-vectors; block totals of every reduction are
+  // combined across the parts through xch (C x 2 x parts x 8 doubles) and
+  // the per-chain counters xcnt.  All C x parts workgroups must be resident
+  // together (the launcher bounds C x parts by the CU count); a wait that
+  // exceeds spin_limit fails the chain instead of hanging.
+  int parts;
+  int spin_limit;
+  double* xch;
+  unsigned long long* xcnt;
 };
 
 hipError_t launch_nuts_step(const batch_args& a, hipStream_t s);
@@ -72,9 +84,12 @@ struct sx_nuts_config;
 namespace sx {
 namespace nb {
 // create with room for up to max_samples sampling iterations per chain
+// max_parts: 1 keeps every chain on one workgroup (required where other
+// work shares the device, as in the serving loop); 0 lets large-D batches
+// spread each chain over up to 4 workgroups.
 int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int64_t chains,
                  int grad_precision, const double* inits, const double* inv_metric,
-                 int64_t max_samples);
+                 int64_t max_samples, int max_parts);
 int launch(sx_nuts_batch* b);                  // non-blocking graph launch
 int poll(sx_nuts_batch* b, bool* finished);    // retire finished launches
 int inflight(const sx_nuts_batch* b);

@@ -224,3 +224,33 @@ def test_hier_batch_large_d_follows_host_service(J, K):
         _say(f"chain {c}: discrete columns equal in {discrete.mean():.3f} of rows")
         assert discrete[:20].all()
     m.close()
+
+
+def test_multi_part_chains_match_single_part():
+    """D > 8192: each chain spreads over several workgroups whose block
+    totals are combined across parts.  Only the summation order of the
+    reductions changes, so the first draws agree with the one-workgroup
+    run to rounding, and every chain ends normally."""
+    import os
+    d = W.hier_logistic(J=500, K=20, seed=31, mean_log_size=np.log(80))
+    m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                   precision=S.FP32)
+    assert m.D > 8192
+    runs = {}
+    for parts in ("4", "1"):
+        os.environ["STANX_NUTS_PARTS"] = parts
+        try:
+            b = S.NutsBatch(m, 16, grad_precision=S.FP32, random_seed=5, chain=1,
+                            num_warmup=20, num_samples=10, max_depth=5, save_warmup=1)
+        finally:
+            del os.environ["STANX_NUTS_PARTS"]
+        while True:
+            steps, done = b.run(max_steps=512)
+            if done >= 16 or steps == 0:
+                break
+        runs[parts] = b.results()
+        b.close()
+    a, s1 = runs["4"], runs["1"]
+    assert np.all(a["status"] == 0) and np.all(s1["status"] == 0)
+    np.testing.assert_allclose(a["draws"][:, :3], s1["draws"][:, :3], rtol=1e-6, atol=1e-9)
+    m.close()
