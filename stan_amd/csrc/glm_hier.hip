@@ -254,28 +254,33 @@ __global__ __launch_bounds__(256) void hier_pass_f32(hier_args a) {
 
 // Per group and chain: sum the group's chunks in order; z gradients;
 // per-group contributions [S_0..S_K, z.S terms (K+1), lp, bad, sum z^2].
-__global__ __launch_bounds__(64) void hier_groups(hier_args a) {
+// A block takes one group and 8 chains: a chunk's partials of those chains
+// are contiguous (chain-major, E entries each), so the block reads them
+// coalesced, 32 threads per chain.
+__global__ __launch_bounds__(256) void hier_groups(hier_args a) {
   const int g = blockIdx.x;
-  const int c = blockIdx.y;
+  const int c = blockIdx.y * 8 + (threadIdx.x >> 5);
+  const int e = threadIdx.x & 31;
+  if (c >= a.chains) return;
   const int K = a.K;
   const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
   double* out = a.out + static_cast<int64_t>(c) * a.out_stride;
   const int64_t ch0 = a.group_chunk0[g], ch1 = a.group_chunk0[g + 1];
   const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
   double* gs = a.gpart + (static_cast<int64_t>(c) * a.J + g) * a.G;
-  for (int e = threadIdx.x; e < K + 3; e += blockDim.x) {
+  for (int ee = e; ee < K + 3; ee += 32) {  // K <= 32: at most two rounds
     double v = 0.0;
-    for (int64_t ch = ch0; ch < ch1; ++ch) v += a.cpart[(ch * a.chains + c) * a.E + e];
-    if (e <= K) {
-      const double tau = exp(th[1 + K + e]);
-      out[1 + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1) + e] = tau * v - z[e];
-      gs[e] = v;
-      gs[K + 1 + e] = tau * z[e] * v;
+    for (int64_t ch = ch0; ch < ch1; ++ch) v += a.cpart[(ch * a.chains + c) * a.E + ee];
+    if (ee <= K) {
+      const double tau = exp(th[1 + K + ee]);
+      out[1 + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1) + ee] = tau * v - z[ee];
+      gs[ee] = v;
+      gs[K + 1 + ee] = tau * z[ee] * v;
     } else {
-      gs[K + 1 + e] = v;  // lp (e = K+1) at 2K+2, bad (e = K+2) at 2K+3
+      gs[K + 1 + ee] = v;  // lp (ee = K+1) at 2K+2, bad (ee = K+2) at 2K+3
     }
   }
-  if (threadIdx.x == 0) {
+  if (e == 0) {
     double zz = 0.0;
     for (int k = 0; k <= K; ++k) zz += z[k] * z[k];
     gs[2 * K + 4] = zz;
@@ -393,7 +398,7 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(hier_groups, dim3(a.J, a.chains), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(hier_groups, dim3(a.J, (a.chains + 7) / 8), dim3(256), 0, s, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(hier_final, dim3(a.chains), dim3(1024), 0, s, a);
