@@ -70,9 +70,28 @@ __device__ __forceinline__ double load_y(const pass_args& a, int64_t t, int R, i
 template <int SMAX, int FAM, int PF>
 __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
   extern __shared__ double lds[];
-  // blockIdx.y selects the chain of a batched evaluation
-  a.theta += static_cast<int64_t>(blockIdx.y) * a.theta_stride;
-  a.partials += static_cast<int64_t>(blockIdx.y) * a.E * gridDim.x;
+  // 1-D grid of chains x nb blocks.  With nb a multiple of 8 the blocks of
+  // every chain that stream the same tiles sit on one XCD (block ids 8
+  // apart), so a batched evaluation reads each tile of X from HBM about
+  // once per XCD and the other chains find it in that XCD's L2; for one
+  // chain the mapping is the identity.
+  const int nb = static_cast<int>(gridDim.x) / a.chains;
+  int blk, chain;
+  {
+    const int L = blockIdx.x;
+    if ((nb & 7) == 0) {
+      const int q = L >> 3;
+      blk = (L & 7) + 8 * (q / a.chains);
+      chain = q % a.chains;
+    } else {
+      blk = L % nb;
+      chain = L / nb;
+    }
+    blk = __builtin_amdgcn_readfirstlane(blk);
+    chain = __builtin_amdgcn_readfirstlane(chain);
+  }
+  a.theta += static_cast<int64_t>(chain) * a.theta_stride;
+  a.partials += static_cast<int64_t>(chain) * a.E * nb;
   const int lane = threadIdx.x & 63;
   // the wave index is uniform across the wave: say so, so that tile
   // indices and buffer descriptors live in scalar registers
@@ -95,8 +114,8 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
   for (int j = 0; j < SMAX; ++j) acc[j] = 0.0;
   double s0 = 0.0, s1 = 0.0, bad = 0.0;
 
-  const int64_t W = static_cast<int64_t>(gridDim.x) * nw;
-  int64_t t = static_cast<int64_t>(blockIdx.x) * nw + wib;
+  const int64_t W = static_cast<int64_t>(nb) * nw;
+  int64_t t = static_cast<int64_t>(blk) * nw + wib;
   double xv[SMAX];
   double yv = 0.0;
   if (PF && t < a.T) {
@@ -179,12 +198,11 @@ __global__ __launch_bounds__(256) void glm_pass_f64(pass_args a) {
     lds[wib * E + a.K + 2] = bad;
   }
   __syncthreads();
-  // block partial, stored entry-major: partials[i * gridDim.x + block]
-  const int nb = gridDim.x;
+  // block partial, stored entry-major: partials[i * nb + block]
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
     double v = 0.0;
     for (int w = 0; w < nw; ++w) v += lds[w * E + i];
-    a.partials[static_cast<int64_t>(i) * nb + blockIdx.x] = v;
+    a.partials[static_cast<int64_t>(i) * nb + blk] = v;
   }
 }
 
@@ -255,8 +273,8 @@ hipError_t launch_one(const pass_args& a, int blocks, int threads, hipStream_t s
                       bool reduce) {
   const size_t lds =
       (static_cast<size_t>(threads / 64) * a.E + static_cast<size_t>(SMAX) * a.L) * sizeof(double);
-  hipLaunchKernelGGL((glm_pass_f64<SMAX, FAM, PF>), dim3(blocks, a.chains), dim3(threads), lds,
-                     s, a);
+  hipLaunchKernelGGL((glm_pass_f64<SMAX, FAM, PF>), dim3(blocks * a.chains), dim3(threads),
+                     lds, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !reduce) return e;
   hipLaunchKernelGGL((glm_reduce_f64<FAM>), dim3(a.chains), dim3(1024), a.E * sizeof(double),

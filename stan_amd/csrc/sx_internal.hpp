@@ -50,7 +50,7 @@ struct pass_args {
   int64_t T;
   int K, L, lgL, S, D, E;
   int propto, jacobian;
-  int chains = 1;              // batched evaluation: chains on grid.y
+  int chains = 1;              // batched evaluation: chains in the grid
   int64_t theta_stride = 0;    // doubles between chains' theta
   int64_t out_stride = 0;      // doubles between chains' outputs (>= D+2)
   double alpha_sd, beta_sd, sigma_sd, sigma_mu;
