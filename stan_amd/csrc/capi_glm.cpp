@@ -519,11 +519,13 @@ void free_workspace(hier_workspace& w) {
   w = hier_workspace();
 }
 
-// chains padded to the chain tile of the MFMA kernel: 128 for K <= 256
-// (32x32 tiles, four waves of 32 chains), else 64
+// chains padded to the chain tile of the MFMA kernel (32x32 tiles, waves
+// of 32 chains): 64 for batches of at most 64 chains (two waves per
+// workgroup), else a multiple of 128 (four waves)
 int mx_chain_pad(const sx_glm* m, int C) {
-  const int tile = m->Kp <= 256 ? 128 : 64;
-  return (C + tile - 1) / tile * tile;
+  (void)m;
+  if (C <= 64) return 64;
+  return (C + 127) / 128 * 128;
 }
 
 int mx_prepare(sx_glm* m, mx_workspace& w, int C) {

@@ -80,14 +80,19 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // A job that allows BF16 hi+lo gradients gets them when they cost less
 // than the exact FP64 passes.  The MFMA pass computes whole tiles of 128
 // chains at about 0.09 ns + 2.5 ps x Kp per row and tile (measured at
-// N = 1e6, K = 100 and N = 2e5, K = 256); the FP64 passes stream 8 K
-// bytes per row and chain, 1.25 ps x K per row and chain at 6.4 TB/s.
-// So the MFMA pass wins when chains x K > (72 + 2 Kp) per tile.
+// N = 1e6, K = 100 and N = 2e5, K = 256), and a batch of at most 64 chains
+// one 64-chain tile at about 0.6 of that (0.257 vs 0.433 ms at N = 1e6,
+// K = 100); the FP64 passes stream 8 K bytes per row and chain, 1.25 ps x K
+// per row and chain at 6.4 TB/s.  So the MFMA pass wins when
+// chains x K > (72 + 2 Kp) per 128-chain tile, or 0.6 (72 + 2 Kp) for one
+// 64-chain tile.
 int gradient_precision(const sx_job_spec& s) {
   if (s.family == SX_HIER_LOGIT) return s.grad_precision == SX_FP32 ? SX_FP32 : SX_FP64;
   if (s.grad_precision != SX_BF16X2 || s.K > 256) return SX_FP64;
   const int64_t Kp = std::max<int64_t>(32, ceil_div(s.K, 32) * 32);
-  return s.chains * s.K > ceil_div(s.chains, 128) * (72 + 2 * Kp) ? SX_BF16X2 : SX_FP64;
+  const int64_t ck = s.chains * s.K, tile = 72 + 2 * Kp;
+  const bool mfma = s.chains <= 64 ? 5 * ck > 3 * tile : ck > ceil_div(s.chains, 128) * tile;
+  return mfma ? SX_BF16X2 : SX_FP64;
 }
 
 // Device bytes a job holds while resident: data layouts, the sampler's

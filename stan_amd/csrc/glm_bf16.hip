@@ -91,8 +91,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of global_load_lds
 
-// K <= 256 (KQ = Kp / 32 <= 8): workgroup = 4 waves x 32 chains = 128
-// chains.  Each 32-row tile of X (both layouts, hi and lo) is staged once
+// K <= 256 (KQ = Kp / 32 <= 8): workgroup = NW waves x 32 chains (NW = 4
+// for 128-chain tiles, 2 for batches of at most 64 chains).  Each 32-row tile of X (both layouts, hi and lo) is staged once
 // per workgroup in LDS, straight from global memory (global_load_lds), and
 // read by all four waves, so X crosses L2 once per 128 chains.  A software
 // pipeline over tiles: while the link and adjoint of tile t run, the
@@ -108,9 +108,10 @@ typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of glob
 //            uses the lane's own r[8st .. 8st+7]), and the X operand takes
 //            the same rows from the 8-row-packed tile: rows 16st + 4 (l/32)
 //            + 0..3 of groups 2st and 2st+1 (two 8-byte LDS reads).
-template <int KQ, int FAM>
-__global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
+template <int KQ, int FAM, int NW>
+__global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   static_assert(KQ >= 1 && KQ <= 8, "glm_mfma32 handles 32 <= Kp <= 256");
+  static_assert(NW == 2 || NW == 4, "2 or 4 waves per workgroup");
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;          // forward k-steps of 16
   constexpr int CPR = Kp / 8;          // 16-byte chunks per X1 row
@@ -128,12 +129,12 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
   const int li = lane & 31, h = lane >> 5;
   // XCD-aware 1-D grid: the splits of one chain block are 8 block ids
   // apart, so blocks sharing a row range run on the same XCD and L2
-  const int nCB = a.Cp / 128;
+  const int nCB = a.Cp / (32 * NW);
   const int bid = blockIdx.x;
   const int xcd = bid & 7, widx = bid >> 3;
   const int spl = (widx / nCB) * 8 + xcd;
   const int cblk = widx % nCB;
-  const int cbase = cblk * 128 + 32 * w;  // first chain of this wave
+  const int cbase = cblk * 32 * NW + 32 * w;  // first chain of this wave
   const int chain = cbase + li;
   bf16x8 bh[S16], bl[S16];
 #pragma unroll
@@ -162,11 +163,12 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
   // (hi, then lo); wave w moves blocks w, w + 4, ...  X1 rows are stored
   // with chunk c of row r at c ^ (r & SWZ), so the forward's row-per-lane
   // reads spread over the banks; X2 is copied as is, with the tile's y.
+  // Wave w moves blocks w, w + NW, ... (4 KQ / NW of them).
 #define SX_STAGE_X1(T, BUF)                                                      \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < KQ; ++i) {                             \
-      const int b_ = w + 4 * i;                                                  \
+    _Pragma("unroll") for (int i = 0; i < 4 * KQ / NW; ++i) {                    \
+      const int b_ = w + NW * i;                                                 \
       const int p_ = (b_ % BPA) * 64 + lane;                                     \
       const int r_ = p_ / CPR;                                                   \
       const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
@@ -177,8 +179,8 @@ __global__ __launch_bounds__(256) void glm_mfma32(mx_args a) {
 #define SX_STAGE_X2(T, BUF)                                                      \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < KQ; ++i) {                             \
-      const int b_ = w + 4 * i;                                                  \
+    _Pragma("unroll") for (int i = 0; i < 4 * KQ / NW; ++i) {                    \
+      const int b_ = w + NW * i;                                                 \
       const int p_ = (b_ % BPA) * 64 + lane;                                     \
       __builtin_amdgcn_global_load_lds((b_ < BPA ? X2h : X2l) + base_ + 8 * p_,  \
           (lds_ptr)(&x2[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
@@ -395,15 +397,25 @@ __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   if (tid < 64) finalize<FAM>(pa, st);
 }
 
+template <int KQ, int NW>
+hipError_t launch_kq32_nw(int family, const mx_args& a, hipStream_t s) {
+  if (a.splits % 8 != 0 || a.Cp % (32 * NW) != 0) return hipErrorInvalidValue;
+  dim3 grid(a.splits * (a.Cp / (32 * NW)));
+  if (family == SX_LOGIT)
+    hipLaunchKernelGGL((glm_mfma32<KQ, SX_LOGIT, NW>), grid, dim3(64 * NW), 0, s, a);
+  else
+    hipLaunchKernelGGL((glm_mfma32<KQ, SX_NORMAL, NW>), grid, dim3(64 * NW), 0, s, a);
+  return hipGetLastError();
+}
+
+// chain tile: a batch of at most 64 chains (Cp = 64) gets two waves per
+// workgroup, larger batches four.  (One-wave workgroups of 32 chains were
+// measured slower than two-wave ones of 64: 0.274 vs 0.257 ms at N = 1e6,
+// K = 100.)
 template <int KQ>
 hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
-  if (a.splits % 8 != 0 || a.Cp % 128 != 0) return hipErrorInvalidValue;
-  dim3 grid(a.splits * (a.Cp / 128));
-  if (family == SX_LOGIT)
-    hipLaunchKernelGGL((glm_mfma32<KQ, SX_LOGIT>), grid, dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((glm_mfma32<KQ, SX_NORMAL>), grid, dim3(256), 0, s, a);
-  return hipGetLastError();
+  if (a.Cp == 64) return launch_kq32_nw<KQ, 2>(family, a, s);
+  return launch_kq32_nw<KQ, 4>(family, a, s);
 }
 
 }  // namespace
@@ -428,8 +440,8 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
                      const_cast<float*>(a.alpha));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // chains come padded to 128 (four waves of 32) and K to a multiple of 32
-  if (a.Cp % 128 != 0) return hipErrorInvalidValue;
+  // chains come padded to 64 or a multiple of 128, K to a multiple of 32
+  if (!(a.Cp == 64 || a.Cp % 128 == 0)) return hipErrorInvalidValue;
   switch (a.Kp / 32) {
     case 1: e = launch_kq32<1>(family, a, s); break;
     case 2: e = launch_kq32<2>(family, a, s); break;

@@ -74,6 +74,12 @@ def test_job_same_alone_and_in_mix():
     for name in ("b", "c"):
         r = srv.results(ids[name])
         assert np.all(r["status"] == 0) and np.all(r["rows"] == 100)
+    # gradient precision the server chose: BF16 hi+lo where the job allows
+    # it, K <= 256 and chains x K above the MFMA tile's cost (b: 16 x 64 =
+    # 1024 > 0.6 x (72 + 2 x 64) for its one 64-chain tile); FP64 for K > 256 (c) and for jobs that ask
+    # for FP64 (a); FP32 as asked for the hierarchical job (d)
+    prec = {name: infos[name]["grad_precision"] for name in infos}
+    assert prec == {"a": S.FP64, "b": S.BF16X2, "c": S.FP64, "d": S.FP32}, prec
     srv.close()
 
 
