@@ -5,8 +5,10 @@ Bars:
   FP64 (single chain and batched): log density and gradient within 1e-12
   of the compensated oracle, relative (normwise for the gradient); a chain's
   batched result is bitwise its single evaluation.
-  FP32 batched (X in FP32, FP32 products and sums over 64-row pieces and chunk sums of at most 256 rows,
-  FP64 sums over chunks and groups), random test points theta ~ N(0, 0.5^2):
+  FP32 batched (X in FP32; FP32 matrix-core products and FP32 sums within
+  a chunk of at most 256 rows, FP32 row terms per 32-row tile added in
+  FP64; FP64 sums over chunks and groups), random test points
+  theta ~ N(0, 0.5^2):
   relative error of the log density <= LP_TOL_FP32 and normwise relative
   error of the gradient <= GRAD_TOL_FP32; the measured values are printed.
 """
