@@ -98,7 +98,6 @@ void release(sx_nuts_batch* b) {
   (void)hipFree(b->a.draws);
   (void)hipFree(b->a.done_count);
   (void)hipFree(b->a.xch);
-  (void)hipFree(b->a.xcnt);
   (void)hipFree(b->d_partials);
   (void)hipFree(b->d_out);
   (void)hipFree(b->d_theta);
@@ -196,13 +195,11 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
     b->a.parts = parts;
     b->a.spin_limit = 1 << 22;
     if (parts > 1) {
-      const size_t nx = static_cast<size_t>(chains) * 16 * parts;
-      if (hipMalloc(&b->a.xch, sizeof(double) * nx) != hipSuccess ||
-          hipMalloc(&b->a.xcnt, sizeof(unsigned long long) * chains) != hipSuccess)
+      // tag 0 (the cleared words) never matches: tags are epoch + 1
+      const size_t nx = static_cast<size_t>(chains) * 2 * 16 * parts;
+      if (hipMalloc(&b->a.xch, sizeof(unsigned long long) * nx) != hipSuccess)
         return fail(SX_ENOMEM, "exchange buffers");
-      if (hipMemsetAsync(b->a.xch, 0, sizeof(double) * nx, b->stream) != hipSuccess ||
-          hipMemsetAsync(b->a.xcnt, 0, sizeof(unsigned long long) * chains, b->stream) !=
-              hipSuccess)
+      if (hipMemsetAsync(b->a.xch, 0, sizeof(unsigned long long) * nx, b->stream) != hipSuccess)
         return fail(SX_EDEVICE, "clear");
     }
   }

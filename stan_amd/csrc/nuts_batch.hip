@@ -89,37 +89,51 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 // Sum of NS block totals over the chain's parts, in part order, so every
-// part gets the same values.  Lane 0 publishes this part's totals (slot
-// epoch % 2), counts itself in, waits for the other parts and reads all
-// totals; a wait longer than spin_limit marks the exchange failed (the
-// step loop then ends the chain) rather than hanging.  Called in
-// block-uniform control flow by every part in the same sequence.
+// part gets the same values.  Lane 0 publishes this part's totals and reads
+// every part's.  Each total travels as two 64-bit words, (tag << 32) | one
+// 32-bit half of the double, written and read with relaxed agent-scope
+// atomics: a word that carries this exchange's tag (epoch + 1, mod 2^32)
+// also carries its half (single-copy atomicity), so no release/acquire
+// ordering, and with it no L2 writeback or invalidation, is needed.  The
+// words of consecutive exchanges alternate between two buffers: a part
+// rewrites a buffer only after every part has published the exchange in
+// between, by which time all of them have read the buffer.  A wait longer
+// than spin_limit marks the exchange failed (the step loop then ends the
+// chain) rather than hanging.  Called in block-uniform control flow by every
+// part in the same sequence.
 template <int NS>
 __device__ void part_sum(const ctx& k, double (&x)[NS]) {
+  static_assert(NS <= 8, "at most 8 totals per exchange");
   const batch_args* a = k.a;
   const int P = a->parts;
   chain_state* s = k.s;
   __syncthreads();
   if (k.lane == 0) {
     const uint64_t ep = s->xepoch;
-    double* base = a->xch + static_cast<int64_t>(k.c) * 16 * P + (ep & 1) * 8 * P;
-    unsigned long long* cnt = a->xcnt + k.c;
-    for (int j = 0; j < NS; ++j)
-      __hip_atomic_store(base + 8 * k.part() + j, x[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(cnt, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long target = (ep + 1) * static_cast<unsigned long long>(P);
-    int spins = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++spins > a->spin_limit) {
-        s->xfail = 1;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+    const unsigned long long tag = ((ep + 1) & 0xffffffffull) << 32;
+    unsigned long long* base = a->xch + (static_cast<int64_t>(k.c) * 2 + (ep & 1)) * 16 * P;
+    for (int j = 0; j < NS; ++j) {
+      const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(x[j]));
+      unsigned long long* w = base + 16 * k.part() + 2 * j;
+      __hip_atomic_store(w, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(w + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    int spins = 0;
     for (int j = 0; j < NS; ++j) {
       double t = 0.0;
-      for (int p = 0; p < P; ++p)
-        t += __hip_atomic_load(base + 8 * p + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int p = 0; p < P; ++p) {
+        unsigned long long* w = base + 16 * p + 2 * j;
+        unsigned long long h[2];
+        for (int u = 0; u < 2; ++u) {
+          h[u] = __hip_atomic_load(w + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          while ((h[u] & ~0xffffffffull) != tag && !s->xfail) {
+            if (++spins > a->spin_limit) s->xfail = 1;
+            __builtin_amdgcn_s_sleep(1);
+            h[u] = __hip_atomic_load(w + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        t += __longlong_as_double(static_cast<long long>((h[0] & 0xffffffffull) | (h[1] << 32)));
+      }
       k.red[j] = t;
     }
     s->xepoch = ep + 1;
@@ -439,31 +453,50 @@ __device__ void start_frame(const ctx& k, int l) {
   vzero(k, RI(l));
 }
 
-// base case of build_tree, after the leapfrog's gradient
-__device__ bool leaf(const ctx& k, const dest& d) {
+// base case of build_tree, fused with the leapfrog's second half.  One
+// pass: g = -grad (or the flipped stale g after a domain error), p -= e/2 g,
+// kinetic energy, proposal position, rho += p, subtree ends, and, for the
+// first level ls whose initial half this leaf completes if the merges
+// below it hold (ls = 0: none), that level's final-half start: ZPF(ls) = q,
+// RF(ls) = 0 (no merge below ls reads or writes either slot, and they are
+// unused if the subtree ends there).  The kinetic energy, which the final
+// half's start also needs, is returned in T.
+__device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls, double& T) {
   chain_state* s = k.s;
-  // one pass: kinetic energy, proposal position, rho += p, subtree ends
   double acc[1] = {0.0};
   {
-    const double* p = k.slot(ZP);
+    const double* o = k.a->out + static_cast<int64_t>(k.c) * k.a->out_stride;
+    double* g = k.slot(ZG);
+    double* p = k.slot(ZP);
     const double* m = k.slot(MINV);
     const double* q = k.slot(ZQ);
     double* prop = k.slot(prop_slot(d.zprop));
     double* r = k.slot(d.rho);
     double* pb = k.slot(d.pbeg);
     double* pe = k.slot(d.pend);
-    struct lf { double p, m, q, r; };
-    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{p[i], m[i], q[i], r[i]}; },
+    double* zpf = k.slot(ls > 0 ? ZPF(ls) : TMP);
+    double* rf = k.slot(ls > 0 ? RF(ls) : TMP);
+    const double he = 0.5 * lf_eps;
+    struct lf { double g, p, m, q, r; };
+    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{ok ? o[1 + i] : g[i], p[i], m[i], q[i], r[i]}; },
                      [&](int i, const lf& v, double (&ac)[1]) {
-                       ac[0] += v.p * (v.m * v.p);
+                       const double gn = -v.g;
+                       const double pn = v.p - he * gn;
+                       g[i] = gn;
+                       p[i] = pn;
+                       ac[0] += pn * (v.m * pn);
                        prop[i] = v.q;
-                       r[i] = v.r + v.p;
-                       pb[i] = v.p;
-                       pe[i] = v.p;
+                       r[i] = v.r + pn;
+                       pb[i] = pn;
+                       pe[i] = pn;
+                       if (ls > 0) {
+                         zpf[i] = v.q;
+                         rf[i] = 0.0;
+                       }
                      });
   }
   block_sum_n<1>(k, acc);
-  const double T = 0.5 * acc[0];
+  T = 0.5 * acc[0];
   double h = T + s->V_z;
   if (isnan(h)) h = kInf;
   const double H0 = s->H0;
@@ -723,11 +756,23 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   ctx k{&sa, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane, nt, red,
         a.chain0 + static_cast<uint32_t>(c), slice};
   if (s.phase == PH_DONE) return;
+  // a PH_LEAF gradient is consumed inside leaf(), fused with its pass
+  bool leaf_ok = false;
   if (s.pending) {
-    // PH_LEAF and PH_SS_B begin with the leapfrog's second half
-    const bool lf = s.phase == PH_LEAF || s.phase == PH_SS_B;
-    const double lf_eps = s.phase == PH_LEAF ? (s.fwd ? 1.0 : -1.0) * s.eps : s.nom_eps;
-    consume_gradient(k, lf, lf_eps);
+    if (s.phase == PH_LEAF) {
+      const double* o = a.out + static_cast<int64_t>(c) * a.out_stride;
+      leaf_ok = o[a.D + 1] == 0.0 && isfinite(o[0]);
+      __syncthreads();
+      if (lane == 0) {
+        s.V_z = leaf_ok ? -o[0] : kInf;
+        s.last_lp = o[0];
+        s.last_ok = leaf_ok ? 1 : 0;
+      }
+    } else {
+      // PH_SS_B begins with the leapfrog's second half
+      const bool lf = s.phase == PH_SS_B;
+      consume_gradient(k, lf, s.nom_eps);
+    }
     __syncthreads();
     if (lane == 0) s.pending = 0;
   }
@@ -918,31 +963,20 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
       }
       case PH_LEAF: {  // the leapfrog's second half ran with the gradient
         const double eps = (s.fwd ? 1.0 : -1.0) * s.eps;
-        bool valid = leaf(k, dest_of(k, 0));
+        int ls = 0;  // first level whose initial half is still open
+        for (int l = 1; l <= s.tree_depth && ls == 0; ++l)
+          if (s.stage[l] == 0) ls = l;
+        double T_leaf = 0.0;
+        bool valid = leaf(k, dest_of(k, 0), eps, leaf_ok, ls, T_leaf);
         bool descended = false;
         int l = 1;
         while (l <= s.tree_depth) {
           if (!valid) break;
           if (s.stage[l] == 0) {
-            // initial half done: start the final half from the current point
-            // (one pass: kinetic energy, proposal position, rho_final = 0)
-            double kacc[1] = {0.0};
-            {
-              const double* p = k.slot(ZP);
-              const double* m = k.slot(MINV);
-              const double* q = k.slot(ZQ);
-              double* zpf = k.slot(ZPF(l));
-              double* rf = k.slot(RF(l));
-              struct pmq { double p, m, q; };
-              vpass_acc<1, pmq>(k, kacc, [&](int i) { return pmq{p[i], m[i], q[i]}; },
-                                [&](int i, const pmq& v, double (&ac)[1]) {
-                                  ac[0] += v.p * (v.m * v.p);
-                                  zpf[i] = v.q;
-                                  rf[i] = 0.0;
-                                });
-            }
-            block_sum_n<1>(k, kacc);
-            const double T = 0.5 * kacc[0];
+            // initial half done (l == ls): the final half starts from the
+            // current point, whose proposal position (ZPF(l)), rho_final = 0
+            // and kinetic energy the leaf's pass already produced
+            const double T = T_leaf;
             __syncthreads();
             if (lane == 0) {
               s.stage[l] = 1;

@@ -62,14 +62,13 @@ struct batch_args {
   unsigned w_num_warmup, w_init_buffer, w_term_buffer;
   // Parts per chain: with parts > 1 each chain runs on `parts` workgroups,
   // each on a slice of the D-vectors; block totals of every reduction are
-  // combined across the parts through xch (C x 2 x parts x 8 doubles) and
-  // the per-chain counters xcnt.  All C x parts workgroups must be resident
-  // together (the launcher bounds C x parts by the CU count); a wait that
-  // exceeds spin_limit fails the chain instead of hanging.
+  // combined across the parts through xch (C x 2 x parts x 16 tagged
+  // words).  All C x parts workgroups must be resident together (the
+  // launcher bounds C x parts by the CU count); a wait that exceeds
+  // spin_limit fails the chain instead of hanging.
   int parts;
   int spin_limit;
-  double* xch;
-  unsigned long long* xcnt;
+  unsigned long long* xch;
 };
 
 hipError_t launch_nuts_step(const batch_args& a, hipStream_t s);
