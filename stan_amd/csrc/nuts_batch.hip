@@ -901,22 +901,70 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         if (a.jitter) eps *= 1.0 + a.jitter * (2.0 * uniform(k) - 1.0);
         __syncthreads();
         if (lane == 0) s.eps = eps;
-        sample_p(k);
+        // one pass: sample_p and the gradient request at z
+        {
+          double* p = k.slot(ZP);
+          const double* m = k.slot(MINV);
+          const double* q = k.slot(ZQ);
+          double* th = a.theta + static_cast<int64_t>(c) * a.D;
+          const uint64_t base = s.rng;
+          for (int i = k.d0() + lane; i < k.d1(); i += nt) {
+            p[i] = stanx::philox_normal(a.seed, k.chain_id, base + i) / sqrt(m[i]);
+            th[i] = q[i];
+          }
+        }
         __syncthreads();
-        request_gradient(k, PH_TRANS_INIT);
+        if (lane == 0) {
+          s.rng += a.D;
+          s.phase = PH_TRANS_INIT;
+          s.pending = 1;
+          s.n_grad++;
+        }
         break;
       }
       case PH_TRANS_INIT: {
-        const double T = kinetic(k, ZP);
-        copy_point(k, FQ, ZQ, s.V_fwd, s.V_z);
-        copy_point(k, BQ, ZQ, s.V_bck, s.V_z);
-        vcopy(k, SQ, ZQ);
-        vcopy(k, PQ, ZQ);
-        vcopy(k, PFB, ZP);
-        vcopy(k, PBF, ZP);
-        vcopy(k, RHO, ZP);
+        // one pass: kinetic energy at z; both trajectory ends, the sample
+        // and proposal positions, the two subtree-end momenta and rho from z
+        double kacc[1] = {0.0};
+        {
+          const double* q = k.slot(ZQ);
+          const double* p = k.slot(ZP);
+          const double* g = k.slot(ZG);
+          const double* m = k.slot(MINV);
+          double* fq = k.slot(FQ);
+          double* fp = k.slot(FP);
+          double* fg = k.slot(FG);
+          double* bq = k.slot(BQ);
+          double* bp = k.slot(BP);
+          double* bg = k.slot(BG);
+          double* sq = k.slot(SQ);
+          double* pq = k.slot(PQ);
+          double* pfb = k.slot(PFB);
+          double* pbf = k.slot(PBF);
+          double* rho = k.slot(RHO);
+          struct zm { double q, p, g, m; };
+          vpass_acc<1, zm>(k, kacc, [&](int i) { return zm{q[i], p[i], g[i], m[i]}; },
+                           [&](int i, const zm& v, double (&ac)[1]) {
+                             ac[0] += v.p * (v.m * v.p);
+                             fq[i] = v.q;
+                             fp[i] = v.p;
+                             fg[i] = v.g;
+                             bq[i] = v.q;
+                             bp[i] = v.p;
+                             bg[i] = v.g;
+                             sq[i] = v.q;
+                             pq[i] = v.q;
+                             pfb[i] = v.p;
+                             pbf[i] = v.p;
+                             rho[i] = v.p;
+                           });
+        }
+        block_sum_n<1>(k, kacc);
+        const double T = 0.5 * kacc[0];
         __syncthreads();
         if (lane == 0) {
+          s.V_fwd = s.V_z;
+          s.V_bck = s.V_z;
           s.V_samp = s.V_z;
           s.T_samp = T;
           s.V_prop = s.V_z;
@@ -936,29 +984,59 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
           if (lane == 0) s.phase = PH_TRANS_END;
           break;
         }
-        vzero(k, RHOF);
-        vzero(k, RHOB);
         const bool fwd = uniform(k) > 0.5;
         __syncthreads();
-        if (fwd) {
-          copy_point(k, ZQ, FQ, s.V_z, s.V_fwd);
-          vcopy(k, RHOB, RHO);
-          vcopy(k, PBF, FP);
-        } else {
-          copy_point(k, ZQ, BQ, s.V_z, s.V_bck);
-          vcopy(k, RHOF, RHO);
-          vcopy(k, PFB, BP);
+        // one pass: z = the chosen end; rho of the new subtree's side = 0,
+        // of the other side = rho; the other side's end momentum; every
+        // level's rho_init = 0 (start_frame); then the first leapfrog's
+        // begin half and its gradient request (leapfrog_request)
+        const int td = s.depth;
+        const double eps = (fwd ? 1.0 : -1.0) * s.eps;
+        {
+          const double* eq = k.slot(fwd ? FQ : BQ);
+          const double* ep = k.slot(fwd ? FP : BP);
+          const double* eg = k.slot(fwd ? FG : BG);
+          const double* m = k.slot(MINV);
+          const double* rho = k.slot(RHO);
+          double* zq = k.slot(ZQ);
+          double* zp = k.slot(ZP);
+          double* zg = k.slot(ZG);
+          double* rsame = k.slot(fwd ? RHOF : RHOB);
+          double* rother = k.slot(fwd ? RHOB : RHOF);
+          double* pother = k.slot(fwd ? PBF : PFB);
+          double* ri1 = k.slot(RI(1));  // RI(l) = RI(1) + 5 (l - 1) slots
+          const int64_t rstep = 5 * static_cast<int64_t>(a.Dpad);
+          double* th = a.theta + static_cast<int64_t>(c) * a.D;
+          const double he = 0.5 * eps;
+          struct ez { double q, p, g, m, r; };
+          vpass<ez>(k, [&](int i) { return ez{eq[i], ep[i], eg[i], m[i], rho[i]}; },
+                    [&](int i, const ez& v) {
+                      const double pn = v.p - he * v.g;
+                      const double qn = v.q + eps * (v.m * pn);
+                      zq[i] = qn;
+                      zp[i] = pn;
+                      zg[i] = v.g;
+                      th[i] = qn;
+                      rsame[i] = 0.0;
+                      rother[i] = v.r;
+                      pother[i] = v.p;
+                      for (int l = 0; l < td; ++l) ri1[i + l * rstep] = 0.0;
+                    });
         }
         __syncthreads();
         if (lane == 0) {
+          s.V_z = fwd ? s.V_fwd : s.V_bck;
           s.fwd = fwd ? 1 : 0;
           s.lsw_sub = -kInf;
-          s.tree_depth = s.depth;
+          s.tree_depth = td;
+          for (int l = 1; l <= td; ++l) {
+            s.stage[l] = 0;
+            s.lsw_init[l] = -kInf;
+          }
+          s.phase = PH_LEAF;
+          s.pending = 1;
+          s.n_grad++;
         }
-        __syncthreads();
-        for (int l = s.tree_depth; l >= 1; --l) start_frame(k, l);
-        __syncthreads();
-        leapfrog_request(k, (s.fwd ? 1.0 : -1.0) * s.eps, PH_LEAF);
         break;
       }
       case PH_LEAF: {  // the leapfrog's second half ran with the gradient
