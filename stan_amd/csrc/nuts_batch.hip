@@ -598,24 +598,34 @@ __device__ void learn_stepsize(const ctx& k, double stat) {
 }
 
 // windowed variance adaptation; returns 1 update, 0 none, -1 non-finite
-__device__ int learn_variance(const ctx& k) {
-  chain_state* s = k.s;
+// windowed_adaptation: the draw enters the variance estimate (Welford)
+__device__ bool in_adapt_window(const ctx& k) {
+  const chain_state* s = k.s;
   const batch_args* a = k.a;
   const unsigned nw = a->w_num_warmup, ib = a->w_init_buffer, tb = a->w_term_buffer;
-  const bool in_window = (s->w_counter >= ib) && (s->w_counter < nw - tb) && (s->w_counter != nw);
-  if (in_window) {
+  return (s->w_counter >= ib) && (s->w_counter < nw - tb) && (s->w_counter != nw);
+}
+// Welford update of the variance estimate with the draw (at ZQ); the
+// end-of-transition pass may already have applied it (welford_done)
+__device__ int learn_variance(const ctx& k, bool welford_done) {
+  chain_state* s = k.s;
+  const batch_args* a = k.a;
+  const unsigned nw = a->w_num_warmup, tb = a->w_term_buffer;
+  if (in_adapt_window(k)) {
     const double n = s->w_n + 1.0;
-    double* m = k.slot(WM);
-    double* m2 = k.slot(WM2);
-    const double* q = k.slot(ZQ);
-    struct qmm { double q, m, m2; };
-    vpass<qmm>(k, [&](int i) { return qmm{q[i], m[i], m2[i]}; },
-               [&](int i, const qmm& v) {
-                 const double delta = v.q - v.m;
-                 const double mn = v.m + delta / n;
-                 m[i] = mn;
-                 m2[i] = v.m2 + (v.q - mn) * delta;
-               });
+    if (!welford_done) {
+      double* m = k.slot(WM);
+      double* m2 = k.slot(WM2);
+      const double* q = k.slot(ZQ);
+      struct qmm { double q, m, m2; };
+      vpass<qmm>(k, [&](int i) { return qmm{q[i], m[i], m2[i]}; },
+                 [&](int i, const qmm& v) {
+                   const double delta = v.q - v.m;
+                   const double mn = v.m + delta / n;
+                   m[i] = mn;
+                   m2[i] = v.m2 + (v.q - mn) * delta;
+                 });
+    }
     __syncthreads();
     if (k.lane == 0) s->w_n = n;
   }
@@ -666,26 +676,6 @@ __device__ int learn_variance(const ctx& k) {
     s->w_counter++;
   }
   return all_finite ? 1 : -1;
-}
-
-__device__ void write_draw(const ctx& k) {
-  chain_state* s = k.s;
-  const batch_args* a = k.a;
-  if (s->rows >= a->rows_cap) return;
-  double* row = a->draws + (static_cast<int64_t>(k.c) * a->rows_cap + s->rows) * a->ncol;
-  if (k.lane == 0 && k.part() == 0) {
-    row[0] = -s->V_samp;
-    row[1] = s->accept;
-    row[2] = s->eps;
-    row[3] = s->depth;
-    row[4] = s->n_leapfrog;
-    row[5] = s->divergent;
-    row[6] = s->T_samp + s->V_samp;
-  }
-  const double* q = k.slot(SQ);
-  vpass<double>(k, [&](int i) { return q[i]; }, [&](int i, double v) { row[7 + i] = v; });
-  __syncthreads();
-  if (k.lane == 0) s->rows++;
 }
 
 __device__ void finish(const ctx& k, int status) {
@@ -1075,10 +1065,27 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         if (descended) break;
         // the subtree of depth tree_depth is complete (or invalid)
         if (!valid) {
-          if (s.fwd)
-            copy_point(k, FQ, ZQ, s.V_fwd, s.V_z);
-          else
-            copy_point(k, BQ, ZQ, s.V_bck, s.V_z);
+          // the trajectory's end on this side = z (one pass)
+          {
+            const double* zq = k.slot(ZQ);
+            const double* zp = k.slot(ZP);
+            const double* zg = k.slot(ZG);
+            double* eq = k.slot(s.fwd ? FQ : BQ);
+            struct z3 { double q, p, g; };
+            vpass<z3>(k, [&](int i) { return z3{zq[i], zp[i], zg[i]}; },
+                      [&](int i, const z3& v) {
+                        eq[i] = v.q;
+                        eq[i + a.Dpad] = v.p;
+                        eq[i + 2 * a.Dpad] = v.g;
+                      });
+          }
+          __syncthreads();
+          if (lane == 0) {
+            if (s.fwd)
+              s.V_fwd = s.V_z;
+            else
+              s.V_bck = s.V_z;
+          }
           __syncthreads();
           if (lane == 0) s.phase = PH_TRANS_END;
           break;
@@ -1155,22 +1162,58 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         break;
       }
       case PH_TRANS_END: {
-        // z = z_sample; record; adapt
-        vcopy(k, ZQ, SQ);
+        // z = z_sample; record; adapt.  One pass: z.q = sample, the draw
+        // row (write_draw) and, in an adaptation window, the Welford update
+        // with the sample (learn_variance)
+        const bool warm = s.iter < a.num_warmup;
+        const int mi = warm ? s.iter : s.iter - a.num_warmup;
+        const bool draw = (warm ? a.save_warmup != 0 : true) && (mi % a.num_thin) == 0 &&
+                          s.rows < a.rows_cap;
+        const bool welford = s.adapt_flag && in_adapt_window(k);
+        {
+          const double n = s.w_n + 1.0;
+          const double* sq = k.slot(SQ);
+          double* zq = k.slot(ZQ);
+          double* wm = k.slot(WM);
+          double* wm2 = k.slot(WM2);
+          double* row = a.draws + (static_cast<int64_t>(c) * a.rows_cap + s.rows) * a.ncol;
+          struct sw { double q, m, m2; };
+          vpass<sw>(k,
+                    [&](int i) { return sw{sq[i], welford ? wm[i] : 0.0, welford ? wm2[i] : 0.0}; },
+                    [&](int i, const sw& v) {
+                      zq[i] = v.q;
+                      if (draw) row[7 + i] = v.q;
+                      if (welford) {
+                        const double delta = v.q - v.m;
+                        const double mn = v.m + delta / n;
+                        wm[i] = mn;
+                        wm2[i] = v.m2 + (v.q - mn) * delta;
+                      }
+                    });
+        }
         __syncthreads();
         if (lane == 0) {
           s.accept = s.smp / static_cast<double>(s.n_leapfrog);
           s.V_z = s.V_samp;
+          if (draw) {
+            if (k.part() == 0) {
+              double* row = a.draws + (static_cast<int64_t>(c) * a.rows_cap + s.rows) * a.ncol;
+              row[0] = -s.V_samp;
+              row[1] = s.accept;
+              row[2] = s.eps;
+              row[3] = s.depth;
+              row[4] = s.n_leapfrog;
+              row[5] = s.divergent;
+              row[6] = s.T_samp + s.V_samp;
+            }
+            s.rows++;
+          }
         }
-        __syncthreads();
-        const bool warm = s.iter < a.num_warmup;
-        const int m = warm ? s.iter : s.iter - a.num_warmup;
-        if ((warm ? a.save_warmup != 0 : true) && (m % a.num_thin) == 0) write_draw(k);
         __syncthreads();
         if (s.adapt_flag) {
           if (lane == 0) learn_stepsize(k, s.accept);
           __syncthreads();
-          const int upd = learn_variance(k);
+          const int upd = learn_variance(k, welford);
           __syncthreads();
           if (upd < 0) {
             finish(k, 70);
