@@ -194,6 +194,18 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
     }
     b->a.parts = parts;
     b->a.spin_limit = 1 << 22;
+    // Phase turns per chain and launch: above 8192 coordinates a chain's
+    // vector passes cost about as much as its gradient, and one chain
+    // closing a deep subtree (a merge pass per level, then the transition)
+    // would hold the whole step; such a chain yields after 4 turns and
+    // continues in the next step.  Below, a yield's extra step (a batched
+    // gradient) costs more than the wait.  Measured totals (ms/step x
+    // steps): hierarchical D = 21,042, 64 chains: 1008 unbounded, 954
+    // with 6, 908 with 4, 950 with 3; logistic D = 101, 256 chains: 2508
+    // unbounded, 2807 with 4.  STANX_NUTS_PHASE_BUDGET=n sets it (n >= 1).
+    b->a.phase_budget = D > 8192 ? 4 : 4096;
+    if (const char* e = std::getenv("STANX_NUTS_PHASE_BUDGET"))
+      b->a.phase_budget = std::max(1, std::atoi(e));
     if (parts > 1) {
       // tag 0 (the cleared words) never matches: tags are epoch + 1
       const size_t nx = static_cast<size_t>(chains) * 2 * 16 * parts;

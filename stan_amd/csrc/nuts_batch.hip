@@ -767,8 +767,14 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
     if (lane == 0) s.pending = 0;
   }
   __syncthreads();
-  // advance until a gradient is requested or the chain finishes
-  for (int guard = 0; guard < 4096 && !s.pending && s.phase != PH_DONE && !s.xfail; ++guard) {
+  // advance until a gradient is requested, the chain finishes, or it has
+  // taken phase_budget phase turns in this launch: it then continues where
+  // it stopped in the next launch (the step between evaluates the same
+  // point again, unread).  The bound keeps the launch from waiting on the
+  // one chain whose leaf closes a deep subtree or a transition while the
+  // others take one leapfrog each; the arithmetic and the random stream
+  // are those of an unbounded run.
+  for (int turn = 0; turn < a.phase_budget && !s.pending && s.phase != PH_DONE && !s.xfail; ++turn) {
     const int ph = s.phase;
     __syncthreads();
     switch (ph) {
@@ -1030,39 +1036,48 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         break;
       }
       case PH_LEAF: {  // the leapfrog's second half ran with the gradient
-        const double eps = (s.fwd ? 1.0 : -1.0) * s.eps;
         int ls = 0;  // first level whose initial half is still open
         for (int l = 1; l <= s.tree_depth && ls == 0; ++l)
           if (s.stage[l] == 0) ls = l;
         double T_leaf = 0.0;
-        bool valid = leaf(k, dest_of(k, 0), eps, leaf_ok, ls, T_leaf);
-        bool descended = false;
-        int l = 1;
-        while (l <= s.tree_depth) {
-          if (!valid) break;
+        const bool valid = leaf(k, dest_of(k, 0), (s.fwd ? 1.0 : -1.0) * s.eps, leaf_ok, ls, T_leaf);
+        if (lane == 0) {
+          s.T_leaf = T_leaf;
+          s.mlevel = 1;
+          s.mvalid = valid ? 1 : 0;
+          s.phase = PH_MERGE;
+        }
+        break;
+      }
+      case PH_MERGE: {  // the ladder above a leaf, one level per loop turn
+        const int l = s.mlevel;
+        const bool valid = s.mvalid != 0;
+        if (valid && l <= s.tree_depth) {
           if (s.stage[l] == 0) {
-            // initial half done (l == ls): the final half starts from the
-            // current point, whose proposal position (ZPF(l)), rho_final = 0
-            // and kinetic energy the leaf's pass already produced
-            const double T = T_leaf;
+            // initial half done (l is the leaf's ls): the final half starts
+            // from the current point, whose proposal position (ZPF(l)),
+            // rho_final = 0 and kinetic energy the leaf's pass produced
             __syncthreads();
             if (lane == 0) {
               s.stage[l] = 1;
               s.V_zpf[l] = s.V_z;
-              s.T_zpf[l] = T;
+              s.T_zpf[l] = s.T_leaf;
               s.lsw_final[l] = -kInf;
             }
             __syncthreads();
             for (int m = l - 1; m >= 1; --m) start_frame(k, m);
             __syncthreads();
-            leapfrog_request(k, eps, PH_LEAF);
-            descended = true;
+            leapfrog_request(k, (s.fwd ? 1.0 : -1.0) * s.eps, PH_LEAF);
             break;
           }
-          valid = merge(k, l, dest_of(k, l));
-          ++l;
+          const bool v = merge(k, l, dest_of(k, l));
+          __syncthreads();
+          if (lane == 0) {
+            s.mvalid = v ? 1 : 0;
+            s.mlevel = l + 1;
+          }
+          break;
         }
-        if (descended) break;
         // the subtree of depth tree_depth is complete (or invalid)
         if (!valid) {
           // the trajectory's end on this side = z (one pass)

@@ -23,6 +23,7 @@ enum phase : int {
   PH_TRANS_INIT,
   PH_DOUBLING,
   PH_LEAF,
+  PH_MERGE,
   PH_TRANS_END,
   PH_DONE
 };
@@ -32,6 +33,7 @@ struct chain_state {
   int phase, pending, status, iter, rows, init_tries, adapt_flag;
   int depth, tree_depth, n_leapfrog, divergent, fwd;
   int ss_ret, ss_iter, ss_dir, last_ok;
+  int mlevel, mvalid;  // PH_MERGE: next level of the ladder above a leaf, still valid
   int stage[kMaxDepth + 1];
   unsigned w_counter, w_next, w_size;
   int iter_end;  // warm-up + sampling iterations; raised to extend a run
@@ -42,6 +44,7 @@ struct chain_state {
   double nom_eps, eps, H0, lsw, lsw_sub, smp, accept;
   double V_z, V_fwd, V_bck, V_samp, T_samp, V_prop, T_prop, V_init, ss_H0, last_lp;
   double sa_counter, s_bar, x_bar, mu, w_n;
+  double T_leaf;  // kinetic energy at the last leaf (the final half's start)
   double lsw_init[kMaxDepth + 1], lsw_final[kMaxDepth + 1];
   double V_zpf[kMaxDepth + 1], T_zpf[kMaxDepth + 1];
 };
@@ -69,6 +72,9 @@ struct batch_args {
   int parts;
   int spin_limit;
   unsigned long long* xch;
+  // phase turns a chain may take per launch before it yields (large D:
+  // the vector work of a deep subtree's merges would set the step time)
+  int phase_budget;
 };
 
 hipError_t launch_nuts_step(const batch_args& a, hipStream_t s);

@@ -256,3 +256,37 @@ def test_multi_part_chains_match_single_part():
     assert np.all(a["status"] == 0) and np.all(s1["status"] == 0)
     np.testing.assert_allclose(a["draws"][:, :3], s1["draws"][:, :3], rtol=1e-6, atol=1e-9)
     m.close()
+
+
+def test_phase_budget_yields_leave_draws_unchanged():
+    """A chain that reaches its per-launch phase budget stops without a
+    gradient request and continues in the next step.  The arithmetic and
+    the random stream are those of an unbounded run, so the draws (and
+    every diagnostic column) are bitwise the same with a budget of 2 turns
+    as with none, while the run takes more steps."""
+    import os
+    d = W.hier_logistic(J=500, K=20, seed=31, mean_log_size=np.log(80))
+    m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                   precision=S.FP32)
+    runs, nsteps = {}, {}
+    for budget in ("2", "4096"):
+        os.environ["STANX_NUTS_PHASE_BUDGET"] = budget
+        try:
+            b = S.NutsBatch(m, 16, grad_precision=S.FP32, random_seed=5, chain=1,
+                            num_warmup=20, num_samples=10, max_depth=5, save_warmup=1)
+        finally:
+            del os.environ["STANX_NUTS_PHASE_BUDGET"]
+        n = 0
+        while True:
+            steps, done = b.run(max_steps=512)
+            n += steps
+            if done >= 16 or steps == 0:
+                break
+        runs[budget], nsteps[budget] = b.results(), n
+        b.close()
+    _say(f"steps with budget 2: {nsteps['2']}, unbounded: {nsteps['4096']}")
+    a, u = runs["2"], runs["4096"]
+    assert np.all(a["status"] == 0) and np.all(u["status"] == 0)
+    assert np.array_equal(a["draws"], u["draws"])
+    assert nsteps["2"] > nsteps["4096"]
+    m.close()
