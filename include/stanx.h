@@ -79,7 +79,7 @@ extern "C" {
                         in FP32 for batched evaluations (FP32 products and
                         chunk sums, FP64 sums over chunks and groups)   */
 #define SX_BF16X2 2 /* FP64 X for single evaluations, plus X as BF16 hi+lo
-                        for batched MFMA evaluations (K <= 256)        */
+                        for batched MFMA evaluations (K <= 512)        */
 
 /* service error codes (src/stan/services/error_codes.hpp:7-18) */
 #define SX_CODE_OK 0
@@ -384,9 +384,10 @@ typedef struct sx_job_spec {
   int64_t J;
   sx_prior prior;
   int64_t chains;
-  int grad_precision;         /* SX_BF16X2 (used when K <= 256 and chains >=
-                                 64; FP64 otherwise), SX_FP32 (SX_HIER_LOGIT)
-                                 or SX_FP64                                    */
+  int grad_precision;         /* SX_BF16X2 (used when K <= 512 and chains x K
+                                 exceeds the MFMA tile's cost; FP64
+                                 otherwise: sx_job_info.grad_precision tells),
+                                 SX_FP32 (SX_HIER_LOGIT) or SX_FP64          */
   sx_nuts_config cfg;         /* chain ids cfg.chain, cfg.chain + 1, ...;
                                  cfg.num_samples: first sampling phase       */
   double target_min_ess;      /* <= 0: run cfg.num_samples only              */

@@ -220,8 +220,8 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
         return SX_EINVAL;
       }
   }
-  if (precision == SX_BF16X2 && K > 256) {
-    set_error("sx_glm_create: BF16 hi+lo batched evaluation supports K <= 256");
+  if (precision == SX_BF16X2 && K > 512) {
+    set_error("sx_glm_create: BF16 hi+lo batched evaluation supports K <= 512");
     return SX_EINVAL;
   }
   if (N < 0 || K < 0 || K > 1024 || (N > 0 && (!X || !y))) {
@@ -357,7 +357,9 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
   }
   if (rc == SX_OK && precision == SX_BF16X2) {
     m->Np = (N + 31) / 32 * 32;
-    m->Kp = std::max<int64_t>(32, (K + 31) / 32 * 32);
+    // a multiple of 32 up to 256; above, of 64 (the adjoint's column
+    // blocks of 256 + 64 j)
+    m->Kp = K <= 256 ? std::max<int64_t>(32, (K + 31) / 32 * 32) : (K + 63) / 64 * 64;
     const size_t elems = static_cast<size_t>(m->Np) * m->Kp;
     if (hipMalloc(&m->d_X1h, elems * 2) != hipSuccess || hipMalloc(&m->d_X1l, elems * 2) != hipSuccess ||
         hipMalloc(&m->d_X2h, elems * 2) != hipSuccess || hipMalloc(&m->d_X2l, elems * 2) != hipSuccess) {
@@ -507,6 +509,8 @@ namespace sx {
 void free_workspace(mx_workspace& w) {
   (void)hipFree(w.Bh);
   (void)hipFree(w.Bl);
+  (void)hipFree(w.Rh);
+  (void)hipFree(w.Rl);
   (void)hipFree(w.alpha);
   (void)hipFree(w.gpart);
   (void)hipFree(w.spart);
@@ -544,6 +548,11 @@ int mx_prepare(sx_glm* m, mx_workspace& w, int C) {
   SX_HIP(hipMalloc(&w.alpha, sizeof(float) * Cp));
   SX_HIP(hipMalloc(&w.gpart, sizeof(float) * ck * splits));
   SX_HIP(hipMalloc(&w.spart, sizeof(double) * 3 * static_cast<size_t>(Cp) * splits));
+  if (m->Kp > 256) {  // residuals between the forward and the adjoint launch
+    const size_t nr = static_cast<size_t>(m->Np) * Cp;
+    SX_HIP(hipMalloc(&w.Rh, nr * 2));
+    SX_HIP(hipMalloc(&w.Rl, nr * 2));
+  }
   w.cap = Cp;
   w.Cp = Cp;
   w.splits = splits;
@@ -565,6 +574,8 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   a.alpha = w.alpha;
   a.gpart = w.gpart;
   a.spart = w.spart;
+  a.Rh = w.Rh;
+  a.Rl = w.Rl;
   a.N = m->N;
   a.n_tiles = m->Np / 32;
   a.Kp = static_cast<int>(m->Kp);

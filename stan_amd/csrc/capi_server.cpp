@@ -82,16 +82,19 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // chains at about 0.09 ns + 2.5 ps x Kp per row and tile (measured at
 // N = 1e6, K = 100 and N = 2e5, K = 256), and a batch of at most 64 chains
 // one 64-chain tile at about 0.6 of that (0.257 vs 0.433 ms at N = 1e6,
-// K = 100); the FP64 passes stream 8 K bytes per row and chain, 1.25 ps x K
-// per row and chain at 6.4 TB/s.  So the MFMA pass wins when
-// chains x K > (72 + 2 Kp) per 128-chain tile, or 0.6 (72 + 2 Kp) for one
-// 64-chain tile.
+// K = 100); above K = 256 the two-launch path costs up to about 1.5 times
+// that (0.46 ms at N = 2e5, K = 320, 256 chains; 0.88 ms at N = 1e6,
+// K = 300, 64 chains).  The FP64 passes stream 8 K bytes per row and
+// chain, 1.25 ps x K per row and chain at 6.4 TB/s.  So the MFMA pass wins
+// when chains x K > (72 + 2 Kp) per 128-chain tile, or 0.6 (72 + 2 Kp) for
+// one 64-chain tile, times 1.5 above K = 256.
 int gradient_precision(const sx_job_spec& s) {
   if (s.family == SX_HIER_LOGIT) return s.grad_precision == SX_FP32 ? SX_FP32 : SX_FP64;
-  if (s.grad_precision != SX_BF16X2 || s.K > 256) return SX_FP64;
-  const int64_t Kp = std::max<int64_t>(32, ceil_div(s.K, 32) * 32);
-  const int64_t ck = s.chains * s.K, tile = 72 + 2 * Kp;
-  const bool mfma = s.chains <= 64 ? 5 * ck > 3 * tile : ck > ceil_div(s.chains, 128) * tile;
+  if (s.grad_precision != SX_BF16X2 || s.K > 512) return SX_FP64;
+  const int64_t Kp = s.K <= 256 ? std::max<int64_t>(32, ceil_div(s.K, 32) * 32) : ceil_div(s.K, 64) * 64;
+  const int64_t ck = s.chains * s.K, tile = 72 + 2 * Kp, wide = Kp > 256 ? 3 : 2;
+  const bool mfma = s.chains <= 64 ? 10 * ck > 3 * tile * wide
+                                   : 2 * ck > ceil_div(s.chains, 128) * tile * wide;
   return mfma ? SX_BF16X2 : SX_FP64;
 }
 
@@ -118,10 +121,12 @@ int64_t footprint_of(const sx_job_spec& s) {
     bytes += ceil_div(N, R) * S * 64 * 8;                         // tiled FP64 X
     bytes += 2048 * (K + 3) * 8;                                  // context partials
     if (prec == SX_BF16X2) {
-      const int64_t Np = ceil_div(N, 32) * 32, Kp = std::max<int64_t>(32, ceil_div(K, 32) * 32);
+      const int64_t Np = ceil_div(N, 32) * 32;
+      const int64_t Kp = K <= 256 ? std::max<int64_t>(32, ceil_div(K, 32) * 32) : ceil_div(K, 64) * 64;
       const int64_t Cp = ceil_div(C, 128) * 128;
       bytes += 4 * Np * Kp * 2;                                   // X1/X2 hi/lo
       bytes += Cp * Kp * 4 + Cp * 4 + Cp * Kp * 4 * 512 + 3 * Cp * 512 * 8;  // workspace
+      if (Kp > 256) bytes += Np * Cp * 4;                         // residuals hi/lo
     } else {
       bytes += 2048 * (K + 3) * 8 * C;                            // per-chain partials
     }

@@ -325,6 +325,290 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   }
 }
 
+// 256 < K <= 512 (KQ = Kp / 32 in {10, 12, 14, 16}): B hi/lo of 32 chains
+// over the full K fill 256 registers, so the gradient accumulators cannot
+// stay beside them; the evaluation is split in two launches.  This one is
+// glm_mfma32's forward and link (same workgroup, grid, X1 staging and
+// software pipeline): it writes the residual r of every row and chain as
+// the BF16 hi/lo MFMA operands the adjoint will use, lane by lane (tile t,
+// 32-chain group g, half st: the lane's bf16x8 at
+// R[((t * Cp / 32 + g) * 2 + st) * 64 + lane]), and the row terms per
+// split.  glm_mfma32_adj then forms G^T = R^T X per block of 256 k.
+template <int KQ, int FAM, int NW>
+__global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
+  static_assert(KQ > 8 && KQ <= 16 && KQ % 2 == 0, "glm_mfma32_fwd handles 256 < Kp <= 512, Kp % 64 == 0");
+  static_assert(NW == 2 || NW == 4, "2 or 4 waves per workgroup");
+  constexpr int Kp = 32 * KQ;
+  constexpr int S16 = 2 * KQ;
+  constexpr int CPR = Kp / 8;
+  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int TE = 32 * Kp;
+  constexpr int BPA = Kp / 16;
+  __shared__ __attribute__((aligned(16))) __bf16 x1[2][2][TE];  // [buffer][hi, lo]
+  __shared__ float ys[2][32];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 31, h = lane >> 5;
+  const int nCB = a.Cp / (32 * NW);
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7, widx = bid >> 3;
+  const int spl = (widx / nCB) * 8 + xcd;
+  const int cblk = widx % nCB;
+  const int cbase = cblk * 32 * NW + 32 * w;
+  const int chain = cbase + li;
+  const int64_t g32 = cbase / 32, nG = a.Cp / 32;
+  bf16x8 bh[S16], bl[S16];
+#pragma unroll
+  for (int s = 0; s < S16; ++s) {
+    const int64_t off = static_cast<int64_t>(chain) * Kp + 16 * s + 8 * h;
+    bh[s] = *reinterpret_cast<const bf16x8*>(a.Bh + off);
+    bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
+  }
+  const float alpha = a.alpha[chain];
+  double lp = 0.0, sr = 0.0, bad = 0.0;
+  const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
+  const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
+  const __bf16* X1h = reinterpret_cast<const __bf16*>(a.X1h);
+  const __bf16* X1l = reinterpret_cast<const __bf16*>(a.X1l);
+  bf16x8* Rh = reinterpret_cast<bf16x8*>(a.Rh);
+  bf16x8* Rl = reinterpret_cast<bf16x8*>(a.Rl);
+  float ystage = 0.f;
+#define SX_STAGE_X1(T, BUF)                                                      \
+  {                                                                              \
+    const int64_t base_ = (T) * 32 * Kp;                                         \
+    _Pragma("unroll") for (int i = 0; i < 4 * KQ / NW; ++i) {                    \
+      const int b_ = w + NW * i;                                                 \
+      const int p_ = (b_ % BPA) * 64 + lane;                                     \
+      const int r_ = p_ / CPR;                                                   \
+      const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? X1h : X1l) + base_ + 8 * src_, \
+          (lds_ptr)(&x1[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
+    }                                                                            \
+  }
+#define SX_STAGE_Y(T)                                                            \
+  {                                                                              \
+    if (tid < 32) {                                                              \
+      const int64_t n_ = (T) * 32 + tid;                                         \
+      float v_ = 0.f;                                                            \
+      if (n_ < a.N) {                                                            \
+        if (FAM == SX_LOGIT)                                                     \
+          v_ = static_cast<const int8_t*>(a.y)[n_] ? 1.0f : -1.0f;               \
+        else                                                                     \
+          v_ = static_cast<float>(static_cast<const double*>(a.y)[n_]);          \
+      }                                                                          \
+      ystage = v_;                                                               \
+    }                                                                            \
+  }
+#define SX_STAGE_DONE(YBUF, WITH_Y)                                              \
+  {                                                                              \
+    __builtin_amdgcn_s_waitcnt(0);                                               \
+    if ((WITH_Y) && tid < 32) ys[YBUF][tid] = ystage;                            \
+  }
+#define SX_FORWARD(BUF, ACC)                                                     \
+  {                                                                              \
+    _Pragma("unroll") for (int j = 0; j < 16; ++j) ACC[j] = 0.f;                 \
+    _Pragma("unroll") for (int s = 0; s < S16; ++s) {                            \
+      const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));               \
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1[BUF][0][off]);      \
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1[BUF][1][off]);      \
+      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], ACC, 0, 0, 0);    \
+      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], ACC, 0, 0, 0);    \
+      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], ACC, 0, 0, 0);    \
+    }                                                                            \
+  }
+  f32x16 hcur, hnext;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
+  if (t0 < t1) {
+    SX_STAGE_X1(t0, 0);
+    if (t0 + 1 < t1) SX_STAGE_X1(t0 + 1, 1);
+    SX_STAGE_Y(t0);
+    SX_STAGE_DONE(0, true);
+  }
+  __syncthreads();
+  if (t0 < t1) SX_FORWARD(0, hcur);
+  for (int64_t t = t0; t < t1; ++t) {
+    const int cur = static_cast<int>((t - t0) & 1), nxt = cur ^ 1;
+    if (t + 2 < t1) SX_STAGE_X1(t + 2, cur);
+    if (t + 1 < t1) SX_STAGE_Y(t + 1);
+    SX_FORWARD(nxt, hnext);
+    const int64_t n0 = t * 32;
+    float lpt = 0.f, srt = 0.f;
+    int badt = 0;
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      bf16x8 rh, rl;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int j = 8 * st + jj;
+        const int row = 8 * (j >> 2) + 4 * h + (j & 3);
+        const bool valid = n0 + row < a.N;
+        const float yv = ys[cur][row];
+        const float eta = hcur[j] + alpha;
+        float term, rr;
+        if (FAM == SX_LOGIT) {
+          logit_row_fast(eta, yv, &term, &rr);
+        } else {
+          rr = yv - eta;
+          term = rr * rr;
+        }
+        term = valid ? term : 0.f;
+        rr = valid ? rr : 0.f;
+        badt += (valid && !isfinite(eta)) ? 1 : 0;
+        lpt += term;
+        srt += rr;
+        __bf16 hh, ll;
+        split(rr, &hh, &ll);
+        rh[jj] = hh;
+        rl[jj] = ll;
+      }
+      const int64_t ri = ((t * nG + g32) * 2 + st) * 64 + lane;
+      Rh[ri] = rh;
+      Rl[ri] = rl;
+    }
+    lp += static_cast<double>(lpt);
+    sr += static_cast<double>(srt);
+    bad += static_cast<double>(badt);
+    SX_STAGE_DONE(nxt, t + 1 < t1);
+    __syncthreads();
+    hcur = hnext;
+  }
+#undef SX_FORWARD
+#undef SX_STAGE_X1
+#undef SX_STAGE_Y
+#undef SX_STAGE_DONE
+  const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
+  lp += __shfl_xor(lp, 32, 64);
+  sr += __shfl_xor(sr, 32, 64);
+  bad += __shfl_xor(bad, 32, 64);
+  if (h == 0) {
+    double* sp = a.spart + (pbase + chain) * 3;
+    sp[0] = lp;
+    sp[1] = sr;
+    sp[2] = bad;
+  }
+}
+
+// The adjoint of a 256 < K <= 512 evaluation for one block of k (KQA x 32
+// columns from column 32 kq0): G^T (32 chains x 32 k) += R^T X per tile,
+// as in glm_mfma32 (step st takes the lane's residuals of rows 8 (j/4) +
+// 4 h + j%4, j = 8 st .. 8 st + 7, and the same rows of the 8-row-packed
+// tile), with R read from the forward launch's output and the tile's
+// column block of X2 staged through LDS one tile ahead.
+template <int KQA, int NW>
+__global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
+  static_assert(KQA >= 1 && KQA <= 8, "blocks of at most 256 columns");
+  static_assert((4 * KQA) % NW == 0, "staging blocks per wave");
+  constexpr int KR = 32 * KQA;         // columns of the block
+  constexpr int TE = 32 * KR;          // elements of one tile array
+  constexpr int NB = 2 * KQA;          // 1 KB staging blocks per tile array
+  __shared__ __attribute__((aligned(16))) __bf16 x2[2][2][TE];  // [buffer][hi, lo]
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 31, h = lane >> 5;
+  const int nCB = a.Cp / (32 * NW);
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7, widx = bid >> 3;
+  const int spl = (widx / nCB) * 8 + xcd;
+  const int cblk = widx % nCB;
+  const int cbase = cblk * 32 * NW + 32 * w;
+  const int64_t g32 = cbase / 32, nG = a.Cp / 32;
+  const int Kp = a.Kp;
+  const int k0 = 32 * kq0;
+  f32x16 g[KQA];
+#pragma unroll
+  for (int i = 0; i < KQA; ++i)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) g[i][j] = 0.f;
+  const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
+  const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
+  const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
+  const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
+  const bf16x8* Rh = reinterpret_cast<const bf16x8*>(a.Rh);
+  const bf16x8* Rl = reinterpret_cast<const bf16x8*>(a.Rl);
+  // staging: chunk p (16 bytes: the 8 rows of one column) of the tile's
+  // column block is group p / KR, column k0 + p % KR; wave w moves blocks
+  // w, w + NW, ... of 64 chunks per array (hi, then lo)
+#define SX_STAGE_X2B(T, BUF)                                                     \
+  {                                                                              \
+    _Pragma("unroll") for (int i = 0; i < 2 * NB / NW; ++i) {                    \
+      const int b_ = w + NW * i;                                                 \
+      const int bb_ = b_ % NB;                                                   \
+      const int p_ = bb_ * 64 + lane;                                            \
+      const int64_t src_ = ((4 * (T) + p_ / KR) * Kp + k0 + p_ % KR) * 8;        \
+      __builtin_amdgcn_global_load_lds((b_ < NB ? X2h : X2l) + src_,             \
+          (lds_ptr)(&x2[BUF][b_ / NB][bb_ * 512]), 16, 0, 0);                    \
+    }                                                                            \
+  }
+  bf16x8 rh[2], rl[2], nh[2], nl[2];
+  if (t0 < t1) {
+    SX_STAGE_X2B(t0, 0);
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      const int64_t ri = ((t0 * nG + g32) * 2 + st) * 64 + lane;
+      rh[st] = Rh[ri];
+      rl[st] = Rl[ri];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+  __syncthreads();
+  for (int64_t t = t0; t < t1; ++t) {
+    const int cur = static_cast<int>((t - t0) & 1), nxt = cur ^ 1;
+    // x2[nxt] held tile t - 1, read before the last barrier
+    if (t + 1 < t1) {
+      SX_STAGE_X2B(t + 1, nxt);
+#pragma unroll
+      for (int st = 0; st < 2; ++st) {
+        const int64_t ri = (((t + 1) * nG + g32) * 2 + st) * 64 + lane;
+        nh[st] = Rh[ri];
+        nl[st] = Rl[ri];
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+#pragma unroll
+      for (int kb = 0; kb < KQA; ++kb) {
+        const int k = 32 * kb + li;
+        const int o0 = ((2 * st) * KR + k) * 8 + 4 * h;
+        const int o1 = ((2 * st + 1) * KR + k) * 8 + 4 * h;
+        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o0]);
+        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o1]);
+        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o0]);
+        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o1]);
+        bf16x8 xh, xl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          xh[j] = ah0[j];
+          xh[4 + j] = ah1[j];
+          xl[j] = al0[j];
+          xl[4 + j] = al1[j];
+        }
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[st], xh, g[kb], 0, 0, 0);
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[st], xl, g[kb], 0, 0, 0);
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[st], xh, g[kb], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      rh[st] = nh[st];
+      rl[st] = nl[st];
+    }
+  }
+#undef SX_STAGE_X2B
+  const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
+#pragma unroll
+  for (int kb = 0; kb < KQA; ++kb)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int ci = cbase + 8 * (j >> 2) + 4 * h + (j & 3);
+      a.gpart[(pbase + ci) * Kp + k0 + 32 * kb + li] = g[kb][j];
+    }
+}
+
 // Per chain: FP64 sums of the split partials, then priors and output.  A
 // split's gradient partials are a contiguous row of Kp floats; the block's
 // 1024 threads cover the rows as Kp/4 float4 columns x 4096/Kp slices of
@@ -418,6 +702,29 @@ hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
   return launch_kq32_nw<KQ, 4>(family, a, s);
 }
 
+// 256 < Kp <= 512: forward + link, then the adjoint per block of 256 k
+template <int KQ, int NW>
+hipError_t launch_wide_nw(int family, const mx_args& a, hipStream_t s) {
+  if (a.splits % 8 != 0 || a.Cp % (32 * NW) != 0 || !a.Rh || !a.Rl) return hipErrorInvalidValue;
+  dim3 grid(a.splits * (a.Cp / (32 * NW)));
+  if (family == SX_LOGIT)
+    hipLaunchKernelGGL((glm_mfma32_fwd<KQ, SX_LOGIT, NW>), grid, dim3(64 * NW), 0, s, a);
+  else
+    hipLaunchKernelGGL((glm_mfma32_fwd<KQ, SX_NORMAL, NW>), grid, dim3(64 * NW), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((glm_mfma32_adj<8, NW>), grid, dim3(64 * NW), 0, s, a, 0);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((glm_mfma32_adj<KQ - 8, NW>), grid, dim3(64 * NW), 0, s, a, 8);
+  return hipGetLastError();
+}
+template <int KQ>
+hipError_t launch_wide(int family, const mx_args& a, hipStream_t s) {
+  if (a.Cp == 64) return launch_wide_nw<KQ, 2>(family, a, s);
+  return launch_wide_nw<KQ, 4>(family, a, s);
+}
+
 }  // namespace
 
 hipError_t launch_split_x(const double* Xrm, uint16_t* X1h, uint16_t* X1l, uint16_t* X2h,
@@ -441,6 +748,7 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // chains come padded to 64 or a multiple of 128, K to a multiple of 32
+  // (of 64 above 256)
   if (!(a.Cp == 64 || a.Cp % 128 == 0)) return hipErrorInvalidValue;
   switch (a.Kp / 32) {
     case 1: e = launch_kq32<1>(family, a, s); break;
@@ -451,6 +759,10 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
     case 6: e = launch_kq32<6>(family, a, s); break;
     case 7: e = launch_kq32<7>(family, a, s); break;
     case 8: e = launch_kq32<8>(family, a, s); break;
+    case 10: e = launch_wide<10>(family, a, s); break;
+    case 12: e = launch_wide<12>(family, a, s); break;
+    case 14: e = launch_wide<14>(family, a, s); break;
+    case 16: e = launch_wide<16>(family, a, s); break;
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;

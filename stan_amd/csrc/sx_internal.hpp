@@ -74,6 +74,7 @@ struct mx_args {
   const float* alpha;       // Cp intercepts
   float* gpart;             // splits x Cp x Kp
   double* spart;            // splits x Cp x 3
+  uint16_t *Rh, *Rl;        // Kp > 256: BF16 hi/lo residuals, Np x Cp each
   int64_t N, n_tiles, tiles_per_split;
   int Kp, Cp, splits;
 };
@@ -110,6 +111,7 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s);
 struct mx_workspace {
   int cap = 0, Cp = 0, splits = 0;
   uint16_t *Bh = nullptr, *Bl = nullptr;
+  uint16_t *Rh = nullptr, *Rl = nullptr;  // Kp > 256 only
   float* alpha = nullptr;
   float* gpart = nullptr;
   double* spart = nullptr;

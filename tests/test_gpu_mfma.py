@@ -29,7 +29,11 @@ def _say(m):
 
 @pytest.mark.parametrize("family", [S.LOGIT, S.NORMAL])
 @pytest.mark.parametrize("N,K,C", [(100, 5, 1), (4097, 37, 7), (3000, 100, 64), (2000, 256, 130),
-                                   (777, 32, 65)])
+                                   (777, 32, 65),
+                                   # 256 < K <= 512: forward + link, then the adjoint
+                                   # per block of 256 columns
+                                   (1500, 257, 9), (2000, 300, 130), (1000, 468, 64),
+                                   (600, 512, 3)])
 def test_mfma_batch_against_oracle(family, N, K, C):
     d = W.logistic(N, K, 1.0 / np.sqrt(K), 0.5, seed=N + K) if family == S.LOGIT else \
         W.normal(N, K, seed=N + K)
@@ -63,6 +67,26 @@ def test_mfma_chain_independent_of_batch():
         assert lp1[0] == lp_all[c]
         np.testing.assert_array_equal(g1[0], g_all[c])
     m.close()
+
+
+def test_mfma_wide_chain_independent_of_batch():
+    """K = 468: the two-launch path gives a chain the same result alone as
+    in a batch of 130 (64- and 256-chain padding, two and four waves)."""
+    d = W.logistic(3000, 468, 1.0 / np.sqrt(468), 0.5, seed=8)
+    m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"], precision=S.BF16X2)
+    th = W.random_thetas(m.D, 130, seed=9) * 0.5
+    lp_all, g_all = m.log_prob_grad_batch(th)
+    for c in (0, 63, 64, 129):
+        lp1, g1 = m.log_prob_grad_batch(th[c:c + 1])
+        assert lp1[0] == lp_all[c]
+        np.testing.assert_array_equal(g1[0], g_all[c])
+    m.close()
+
+
+def test_mfma_rejects_k_above_512():
+    d = W.logistic(64, 513, 0.05, 0.5, seed=1)
+    with pytest.raises(Exception, match="K <= 512"):
+        S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"], precision=S.BF16X2)
 
 
 def test_mfma_domain_error_is_per_chain():

@@ -37,7 +37,7 @@ def _jobs():
         "b": dict(family=S.NORMAL, X=b["X"], y=b["y"], chains=16, prior=b["prior"],
                   grad_precision=S.BF16X2, chain=1, **common),
         "c": dict(family=S.LOGIT, X=c["X"], y=c["y"], chains=4, prior=c["prior"],
-                  grad_precision=S.BF16X2, chain=1, **common),  # K > 256: FP64 gradients
+                  grad_precision=S.BF16X2, chain=1, **common),  # K = 300: two-launch MFMA
         "d": dict(family=S.HIER_LOGIT, X=d["X"], y=d["y"], offsets=d["offsets"], chains=8,
                   prior=d["prior"], grad_precision=S.FP32, chain=1, **common),
     }
@@ -75,11 +75,12 @@ def test_job_same_alone_and_in_mix():
         r = srv.results(ids[name])
         assert np.all(r["status"] == 0) and np.all(r["rows"] == 100)
     # gradient precision the server chose: BF16 hi+lo where the job allows
-    # it, K <= 256 and chains x K above the MFMA tile's cost (b: 16 x 64 =
-    # 1024 > 0.6 x (72 + 2 x 64) for its one 64-chain tile); FP64 for K > 256 (c) and for jobs that ask
-    # for FP64 (a); FP32 as asked for the hierarchical job (d)
+    # it, K <= 512 and chains x K above the MFMA tile's cost (b: 16 x 64 =
+    # 1024 > 0.6 x (72 + 2 x 64) for its one 64-chain tile; c: 4 x 300 =
+    # 1200 > 0.6 x (72 + 2 x 320)); FP64 for jobs that ask for FP64 (a);
+    # FP32 as asked for the hierarchical job (d)
     prec = {name: infos[name]["grad_precision"] for name in infos}
-    assert prec == {"a": S.FP64, "b": S.BF16X2, "c": S.FP64, "d": S.FP32}, prec
+    assert prec == {"a": S.FP64, "b": S.BF16X2, "c": S.BF16X2, "d": S.FP32}, prec
     srv.close()
 
 
