@@ -42,17 +42,22 @@ __device__ __forceinline__ void split(float x, __bf16* hi, __bf16* lo) {
   *lo = to_bf16(x - static_cast<float>(h));
 }
 
-// FP32 Bernoulli-logit row on the hardware transcendentals (v_exp_f32,
-// v_log_f32, v_rcp_f32): term = -(max(-s eta, 0) + log(1 + t)), t =
-// exp(-|s eta|), r = s inv_logit(-s eta).  Absolute error of term ~1e-7,
-// relative error of r a few ulp: below the BF16 hi+lo representation
-// error of the products that feed eta.
+// FP32 Bernoulli-logit row on the raw hardware transcendentals: term =
+// -(max(-s eta, 0) + log(1 + t)), t = exp(-|s eta|), r = s inv_logit(-s eta).
+// The exp2 argument is <= 0 and the log2 argument lies in [1, 2], where
+// v_exp_f32 and v_log_f32 need none of the range handling of the library
+// calls (an exp2 result below 2^-126 may flush to zero; it enters u = 1 + t
+// and r = s t / u only at that size).  Absolute error of term ~1e-7,
+// relative error of r a few ulp: below the BF16 hi+lo representation error
+// of the products that feed eta.  s = 0 (a row past N) gives term = r = 0
+// through the factor |s|, without a select.
 __device__ __forceinline__ void logit_row_fast(float eta, float s, float* term, float* r) {
+  constexpr float kLog2e = 1.44269504088896341f, kLn2 = 0.693147180559945309f;
   const float se = s * eta;
-  const float t = __expf(-fabsf(se));
+  const float t = __builtin_amdgcn_exp2f(-fabsf(se) * kLog2e);
   const float u = 1.0f + t;
   const float inv = __builtin_amdgcn_rcpf(u);
-  *term = -(fmaxf(-se, 0.0f) + __logf(u));
+  *term = -fabsf(s) * fmaf(__builtin_amdgcn_logf(u), kLn2, fmaxf(-se, 0.0f));
   *r = s * (se >= 0.0f ? t * inv : inv);
 }
 
@@ -149,7 +154,8 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   for (int i = 0; i < KQ; ++i)
 #pragma unroll
     for (int j = 0; j < 16; ++j) g[i][j] = 0.f;
-  double lp = 0.0, sr = 0.0, bad = 0.0;
+  double lp = 0.0, sr = 0.0;
+  float nf = 0.f;  // sum of 0 * eta over the lane's rows
 
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
@@ -243,7 +249,6 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
     // The tile's sums are FP32 and are added to the FP64 totals per tile.
     const int64_t n0 = t * 32;
     float lpt = 0.f, srt = 0.f;
-    int badt = 0;
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
       bf16x8 rh, rl;
@@ -251,19 +256,19 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
       for (int jj = 0; jj < 8; ++jj) {
         const int j = 8 * st + jj;
         const int row = 8 * (j >> 2) + 4 * h + (j & 3);
-        const bool valid = n0 + row < a.N;
         const float yv = ys[cur][row];
         const float eta = hcur[j] + alpha;
         float term, rr;
         if (FAM == SX_LOGIT) {
+          // rows past N carry sign 0: term = r = 0
           logit_row_fast(eta, yv, &term, &rr);
         } else {
-          rr = yv - eta;
+          rr = n0 + row < a.N ? yv - eta : 0.f;
           term = rr * rr;
         }
-        term = valid ? term : 0.f;
-        rr = valid ? rr : 0.f;
-        badt += (valid && !isfinite(eta)) ? 1 : 0;
+        // NaN once any eta is infinite or NaN; a row past N has eta =
+        // alpha, non-finite only if every row's is
+        nf = fmaf(eta, 0.f, nf);
         lpt += term;
         srt += rr;
         __bf16 hh, ll;
@@ -295,7 +300,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
     }
     lp += static_cast<double>(lpt);
     sr += static_cast<double>(srt);
-    bad += static_cast<double>(badt);
+
     SX_STAGE_DONE(nxt, t + 1 < t1);
     __syncthreads();
     hcur = hnext;
@@ -316,7 +321,8 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   // row terms of this lane's chain: combine the two halves of the rows
   lp += __shfl_xor(lp, 32, 64);
   sr += __shfl_xor(sr, 32, 64);
-  bad += __shfl_xor(bad, 32, 64);
+  nf += __shfl_xor(nf, 32, 64);
+  const double bad = nf == nf ? 0.0 : 1.0;
   if (h == 0) {
     double* sp = a.spart + (pbase + chain) * 3;
     sp[0] = lp;
@@ -366,7 +372,8 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
     bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
   }
   const float alpha = a.alpha[chain];
-  double lp = 0.0, sr = 0.0, bad = 0.0;
+  double lp = 0.0, sr = 0.0;
+  float nf = 0.f;  // sum of 0 * eta over the lane's rows
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
   const __bf16* X1h = reinterpret_cast<const __bf16*>(a.X1h);
@@ -435,7 +442,6 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
     SX_FORWARD(nxt, hnext);
     const int64_t n0 = t * 32;
     float lpt = 0.f, srt = 0.f;
-    int badt = 0;
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
       bf16x8 rh, rl;
@@ -443,19 +449,19 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
       for (int jj = 0; jj < 8; ++jj) {
         const int j = 8 * st + jj;
         const int row = 8 * (j >> 2) + 4 * h + (j & 3);
-        const bool valid = n0 + row < a.N;
         const float yv = ys[cur][row];
         const float eta = hcur[j] + alpha;
         float term, rr;
         if (FAM == SX_LOGIT) {
+          // rows past N carry sign 0: term = r = 0
           logit_row_fast(eta, yv, &term, &rr);
         } else {
-          rr = yv - eta;
+          rr = n0 + row < a.N ? yv - eta : 0.f;
           term = rr * rr;
         }
-        term = valid ? term : 0.f;
-        rr = valid ? rr : 0.f;
-        badt += (valid && !isfinite(eta)) ? 1 : 0;
+        // NaN once any eta is infinite or NaN; a row past N has eta =
+        // alpha, non-finite only if every row's is
+        nf = fmaf(eta, 0.f, nf);
         lpt += term;
         srt += rr;
         __bf16 hh, ll;
@@ -469,7 +475,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
     }
     lp += static_cast<double>(lpt);
     sr += static_cast<double>(srt);
-    bad += static_cast<double>(badt);
+
     SX_STAGE_DONE(nxt, t + 1 < t1);
     __syncthreads();
     hcur = hnext;
@@ -481,7 +487,8 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
   lp += __shfl_xor(lp, 32, 64);
   sr += __shfl_xor(sr, 32, 64);
-  bad += __shfl_xor(bad, 32, 64);
+  nf += __shfl_xor(nf, 32, 64);
+  const double bad = nf == nf ? 0.0 : 1.0;
   if (h == 0) {
     double* sp = a.spart + (pbase + chain) * 3;
     sp[0] = lp;
