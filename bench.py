@@ -26,6 +26,11 @@ PEAK_HBM_GBS = 8000.0
 PMC_TRAFFIC_BYTES = {(4, 25): 783350 * 1024.0 + 3298 * 1024.0}
 
 
+def _nanmax(a, b):
+    """max that keeps a NaN (the builtin max(a, nan) returns a)."""
+    return a if a != a else (b if b != b else max(a, b))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -202,8 +207,8 @@ def mixed_error(model, ref_data, family, thetas):
     e_lp = e_g = 0.0
     for c in range(len(thetas)):
         st, lp_r, g_r = ref.lpgrad(thetas[c], mode=1)
-        e_lp = max(e_lp, abs(lp[c] - lp_r) / abs(lp_r))
-        e_g = max(e_g, float(np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r)))
+        e_lp = _nanmax(e_lp, abs(lp[c] - lp_r) / abs(lp_r))
+        e_g = _nanmax(e_g, float(np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r)))
     return e_lp, e_g
 
 
