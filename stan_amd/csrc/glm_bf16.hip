@@ -94,6 +94,7 @@ __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, 
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));  // a 16-byte staging block
 typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of global_load_lds
 
 // K <= 256 (KQ = Kp / 32 <= 8): workgroup = NW waves x 32 chains (NW = 4
@@ -163,52 +164,97 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   const __bf16* X1l = reinterpret_cast<const __bf16*>(a.X1l);
   const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
   const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
-  float ystage = 0.f;
-  // Staging (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at
-  // the wave's LDS base + 16 l).  One layout's tile is 2 BPA blocks of 1 KB
-  // (hi, then lo); wave w moves blocks w, w + 4, ...  X1 rows are stored
-  // with chunk c of row r at c ^ (r & SWZ), so the forward's row-per-lane
-  // reads spread over the banks; X2 is copied as is, with the tile's y.
-  // Wave w moves blocks w, w + NW, ... (4 KQ / NW of them).
-#define SX_STAGE_X1(T, BUF)                                                      \
+  double yraw = 0.0;  // y of a staged row (tid < 32): converted when committed
+  int yraw_i = 0;
+  bool yvalid = false;
+  float ystage = 0.f;  // (direct staging: converted at once)
+  // Staging.  Up to K = 192 (KQ <= 6) through registers: a tile's X1 (X2,
+  // with its y) is loaded with ordinary 16-byte global loads at the top of
+  // an iteration and written to its LDS buffer at the end, so the loads
+  // have the iteration's compute to land and the only wait for them is at
+  // those writes.  Above, where the loaded blocks would not fit beside B
+  // hi/lo and the gradient accumulators, straight from global memory to LDS
+  // (global_load_lds), waited for before the barrier (the compiler also
+  // waits for them before the next LDS read, so these do not overlap the
+  // tile's compute).  One layout's tile is 2 BPA blocks of 1 KB (hi, then
+  // lo); wave w moves blocks w, w + NW, ... (4 KQ / NW of them), lane l 16
+  // bytes at block offset 16 l.  X1 rows are stored with chunk c of row r at
+  // c ^ (r & SWZ), so the forward's row-per-lane reads spread over the
+  // banks; X2 is copied as is.  y is loaded raw and converted when
+  // committed, so that no early use of its register forces a wait for it
+  // (and for every load issued before it).
+  constexpr bool REG = KQ <= 6;
+  constexpr int NBW = 4 * KQ / NW;     // staging blocks per wave and layout
+  u32x4 r1[REG ? NBW : 1], r2[REG ? NBW : 1];
+#define SX_LOAD_X1(T, BUF)                                                       \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < 4 * KQ / NW; ++i) {                    \
+    _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {           \
       const int b_ = w + NW * i;                                                 \
       const int p_ = (b_ % BPA) * 64 + lane;                                     \
       const int r_ = p_ / CPR;                                                   \
       const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
-      __builtin_amdgcn_global_load_lds((b_ < BPA ? X1h : X1l) + base_ + 8 * src_, \
-          (lds_ptr)(&x1[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
+      const __bf16* g_ = (b_ < BPA ? X1h : X1l) + base_ + 8 * src_;              \
+      if constexpr (REG)                                                         \
+        r1[i] = *reinterpret_cast<const u32x4*>(g_);                             \
+      else                                                                       \
+        __builtin_amdgcn_global_load_lds(g_,                                     \
+            (lds_ptr)(&x1[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);          \
     }                                                                            \
   }
-#define SX_STAGE_X2(T, BUF)                                                      \
+#define SX_LOAD_X2(T, BUF)                                                       \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < 4 * KQ / NW; ++i) {                    \
+    _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {           \
       const int b_ = w + NW * i;                                                 \
       const int p_ = (b_ % BPA) * 64 + lane;                                     \
-      __builtin_amdgcn_global_load_lds((b_ < BPA ? X2h : X2l) + base_ + 8 * p_,  \
-          (lds_ptr)(&x2[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
+      const __bf16* g_ = (b_ < BPA ? X2h : X2l) + base_ + 8 * p_;                \
+      if constexpr (REG)                                                         \
+        r2[i] = *reinterpret_cast<const u32x4*>(g_);                             \
+      else                                                                       \
+        __builtin_amdgcn_global_load_lds(g_,                                     \
+            (lds_ptr)(&x2[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);          \
     }                                                                            \
     if (tid < 32) {                                                              \
       const int64_t n_ = (T) * 32 + tid;                                         \
-      float v_ = 0.f;                                                            \
-      if (n_ < a.N) {                                                            \
-        if (FAM == SX_LOGIT)                                                     \
-          v_ = static_cast<const int8_t*>(a.y)[n_] ? 1.0f : -1.0f;               \
-        else                                                                     \
-          v_ = static_cast<float>(static_cast<const double*>(a.y)[n_]);          \
-      }                                                                          \
-      ystage = v_;                                                               \
+      yvalid = n_ < a.N;                                                         \
+      const int64_t m_ = yvalid ? n_ : 0;                                        \
+      if (FAM == SX_LOGIT)                                                       \
+        yraw_i = static_cast<const int8_t*>(a.y)[m_];                            \
+      else                                                                       \
+        yraw = static_cast<const double*>(a.y)[m_];                              \
+      if constexpr (!REG)                                                        \
+        ystage = !yvalid ? 0.f : FAM == SX_LOGIT ? (yraw_i ? 1.0f : -1.0f)      \
+                                                 : static_cast<float>(yraw);     \
     }                                                                            \
   }
-  // the wave's staged loads have landed (before the barrier that publishes
-  // them) and the staged y goes to LDS
-#define SX_STAGE_DONE(YBUF, WITH_Y)                                              \
+#define SX_COMMIT_X1(BUF)                                                        \
   {                                                                              \
-    __builtin_amdgcn_s_waitcnt(0);                                               \
-    if ((WITH_Y) && tid < 32) ys[YBUF][tid] = ystage;                            \
+    if constexpr (REG) {                                                         \
+      _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {         \
+        const int b_ = w + NW * i;                                               \
+        *reinterpret_cast<u32x4*>(&x1[BUF][b_ / BPA][(b_ % BPA) * 512 + 8 * lane]) = r1[i]; \
+      }                                                                          \
+    }                                                                            \
+  }
+  // X2 and the tile's y; the direct loads of both layouts are waited for
+#define SX_COMMIT_X2(BUF)                                                        \
+  {                                                                              \
+    if constexpr (REG) {                                                         \
+      _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {         \
+        const int b_ = w + NW * i;                                               \
+        *reinterpret_cast<u32x4*>(&x2[BUF][b_ / BPA][(b_ % BPA) * 512 + 8 * lane]) = r2[i]; \
+      }                                                                          \
+    } else {                                                                     \
+      __builtin_amdgcn_s_waitcnt(0);                                             \
+    }                                                                            \
+    if (tid < 32) {                                                              \
+      float v_ = ystage;                                                         \
+      if constexpr (REG)                                                         \
+        v_ = !yvalid ? 0.f : FAM == SX_LOGIT ? (yraw_i ? 1.0f : -1.0f)          \
+                                             : static_cast<float>(yraw);         \
+      ys[BUF][tid] = v_;                                                         \
+    }                                                                            \
   }
 #define SX_FORWARD(BUF, ACC)                                                     \
   {                                                                              \
@@ -227,10 +273,14 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
   // prologue: X1 and X2 of tile t0, X1 of tile t0 + 1; forward of tile t0
   if (t0 < t1) {
-    SX_STAGE_X1(t0, 0);
-    SX_STAGE_X2(t0, 0);
-    if (t0 + 1 < t1) SX_STAGE_X1(t0 + 1, 1);
-    SX_STAGE_DONE(0, true);
+    SX_LOAD_X1(t0, 0);
+    SX_COMMIT_X1(0);
+    if (t0 + 1 < t1) {
+      SX_LOAD_X1(t0 + 1, 1);
+      SX_COMMIT_X1(1);
+    }
+    SX_LOAD_X2(t0, 0);
+    SX_COMMIT_X2(0);
   }
   __syncthreads();
   if (t0 < t1) SX_FORWARD(0, hcur);
@@ -240,8 +290,8 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
     // x2[nxt] X2 of tile t - 1: both free since the last barrier.  The
     // forward of the tile after the last reads a stale buffer; its result
     // is discarded.
-    if (t + 2 < t1) SX_STAGE_X1(t + 2, cur);
-    if (t + 1 < t1) SX_STAGE_X2(t + 1, nxt);
+    if (t + 2 < t1) SX_LOAD_X1(t + 2, cur);
+    if (t + 1 < t1) SX_LOAD_X2(t + 1, nxt);
     SX_FORWARD(nxt, hnext);
     // link and adjoint of tile t in two halves of the lane's 16 rows: half
     // st holds elements j = 8 st .. 8 st + 7 (rows 8 (j/4) + 4 h + j%4),
@@ -301,14 +351,16 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
     lp += static_cast<double>(lpt);
     sr += static_cast<double>(srt);
 
-    SX_STAGE_DONE(nxt, t + 1 < t1);
+    if (t + 2 < t1) SX_COMMIT_X1(cur);
+    if (t + 1 < t1) SX_COMMIT_X2(nxt);
     __syncthreads();
     hcur = hnext;
   }
 #undef SX_FORWARD
-#undef SX_STAGE_X1
-#undef SX_STAGE_X2
-#undef SX_STAGE_DONE
+#undef SX_LOAD_X1
+#undef SX_LOAD_X2
+#undef SX_COMMIT_X1
+#undef SX_COMMIT_X2
   // g[kb][j] = G^T[chain cbase + 8 (j/4) + 4 h + j%4][k = 32 kb + li]
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll
