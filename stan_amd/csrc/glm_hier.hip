@@ -190,8 +190,10 @@ __global__ __launch_bounds__(256) void hier_mfma_f32(hier_args a) {
   float stg[NJ];
   float ystg = 0.f;
   int lim = 0;
-  const int dr = 64 / K, dk = 64 % K;
-  const int row0 = lane / K, col0 = lane % K;
+  // (K = 0, intercepts only: nothing is staged; no division by zero)
+  const int Kd = K > 0 ? K : 1;
+  const int dr = 64 / Kd, dk = 64 % Kd;
+  const int row0 = lane / Kd, col0 = lane % Kd;
   // branch-free staging: element indices past the tile's last row are
   // clamped to it (an in-bounds load) and the value replaced by zero
 #define SX_LOAD_TILE(T0)                                                         \
