@@ -383,22 +383,8 @@ __device__ void consume_gradient(const ctx& k, bool lf, double lf_eps) {
   }
 }
 
-// leapfrog halves (expl_leapfrog): begin = p -= e/2 g; q += e minv p
-__device__ void leapfrog_begin(const ctx& k, double eps) {
-  double* p = k.slot(ZP);
-  double* q = k.slot(ZQ);
-  const double* g = k.slot(ZG);
-  const double* m = k.slot(MINV);
-  const double he = 0.5 * eps;
-  struct pgqm { double p, g, q, m; };
-  vpass<pgqm>(k, [&](int i) { return pgqm{p[i], g[i], q[i], m[i]}; },
-              [&](int i, const pgqm& v) {
-                const double pn = v.p - he * v.g;
-                p[i] = pn;
-                q[i] = v.q + eps * (v.m * pn);
-              });
-}
-// leapfrog_begin then request_gradient, in one pass
+// the leapfrog's begin half (expl_leapfrog: p -= e/2 g; q += e minv p)
+// and the gradient request at the new q, in one pass
 __device__ void leapfrog_request(const ctx& k, double eps, int next_phase) {
   double* p = k.slot(ZP);
   double* q = k.slot(ZQ);
@@ -444,13 +430,6 @@ __device__ dest dest_of(const ctx& k, int level) {
   for (int l = k.s->tree_depth; l > level; --l)
     d = k.s->stage[l] == 0 ? init_dest(d, l) : final_dest(d, l);
   return d;
-}
-__device__ void start_frame(const ctx& k, int l) {
-  if (k.lane == 0) {
-    k.s->stage[l] = 0;
-    k.s->lsw_init[l] = -kInf;
-  }
-  vzero(k, RI(l));
 }
 
 // base case of build_tree, fused with the leapfrog's second half.  One
@@ -984,7 +963,7 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         __syncthreads();
         // one pass: z = the chosen end; rho of the new subtree's side = 0,
         // of the other side = rho; the other side's end momentum; every
-        // level's rho_init = 0 (start_frame); then the first leapfrog's
+        // level's rho_init = 0 (and its frame reset); then the first leapfrog's
         // begin half and its gradient request (leapfrog_request)
         const int td = s.depth;
         const double eps = (fwd ? 1.0 : -1.0) * s.eps;
@@ -1065,9 +1044,40 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
               s.lsw_final[l] = -kInf;
             }
             __syncthreads();
-            for (int m = l - 1; m >= 1; --m) start_frame(k, m);
+            // one pass: every lower level's rho_init = 0 (its frame reset) and
+            // the leapfrog's begin half with its gradient request
+            // (leapfrog_request)
+            {
+              const double eps = (s.fwd ? 1.0 : -1.0) * s.eps;
+              double* p = k.slot(ZP);
+              double* q = k.slot(ZQ);
+              const double* g = k.slot(ZG);
+              const double* m = k.slot(MINV);
+              double* th = a.theta + static_cast<int64_t>(c) * a.D;
+              double* ri1 = k.slot(RI(1));  // RI(j) = RI(1) + 5 (j - 1) slots
+              const int64_t rstep = 5 * static_cast<int64_t>(a.Dpad);
+              const double he = 0.5 * eps;
+              struct pgqm { double p, g, q, m; };
+              vpass<pgqm>(k, [&](int i) { return pgqm{p[i], g[i], q[i], m[i]}; },
+                          [&](int i, const pgqm& v) {
+                            const double pn = v.p - he * v.g;
+                            const double qn = v.q + eps * (v.m * pn);
+                            p[i] = pn;
+                            q[i] = qn;
+                            th[i] = qn;
+                            for (int j = 0; j < l - 1; ++j) ri1[i + j * rstep] = 0.0;
+                          });
+            }
             __syncthreads();
-            leapfrog_request(k, (s.fwd ? 1.0 : -1.0) * s.eps, PH_LEAF);
+            if (lane == 0) {
+              for (int mm = 1; mm < l; ++mm) {
+                s.stage[mm] = 0;
+                s.lsw_init[mm] = -kInf;
+              }
+              s.phase = PH_LEAF;
+              s.pending = 1;
+              s.n_grad++;
+            }
             break;
           }
           const bool v = merge(k, l, dest_of(k, l));
