@@ -303,12 +303,14 @@ __global__ __launch_bounds__(256) void hier_mfma_f32(hier_args a) {
 }
 
 // Per group and chain: sum the group's chunks in order; z gradients;
-// per-group contributions [S_0..S_K, z.S terms (K+1), lp, bad, sum z^2].
-// A block takes kGroupsPerBlock consecutive groups and 8 chains (32
-// threads per chain; tau = exp(v) computed once per chain and block): a
-// chunk's partials of those chains are contiguous (chain-major, E entries
-// each), so the block reads them coalesced.  sum z^2 is a fixed-order
-// shuffle reduction over the chain's 32 threads.
+// per-group contributions [S_0..S_K, z.S terms (K+1), lp, bad, sum z^2],
+// summed over the block's groups in group order: the block's row of
+// partials, [(chain x blocks + block) x G].  A block takes kGroupsPerBlock
+// consecutive groups and 8 chains (32 threads per chain; tau = exp(v)
+// computed once per chain and block): a chunk's partials of those chains
+// are contiguous (chain-major, E entries each), so the block reads them
+// coalesced.  sum z^2 is a fixed-order shuffle reduction over the chain's
+// 32 threads.
 constexpr int kGroupsPerBlock = 8;
 __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
   const int c = blockIdx.y * 8 + (threadIdx.x >> 5);
@@ -322,10 +324,10 @@ __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
   const double tau1 = e + 32 <= K ? exp(th[1 + K + e + 32]) : 0.0;
   const int g0 = blockIdx.x * kGroupsPerBlock;
   const int g1 = g0 + kGroupsPerBlock < a.J ? g0 + kGroupsPerBlock : a.J;
+  double sv[2] = {0.0, 0.0}, szs[2] = {0.0, 0.0}, szz = 0.0;  // sums over the block's groups
   for (int g = g0; g < g1; ++g) {
     const int64_t ch0 = a.group_chunk0[g], ch1 = a.group_chunk0[g + 1];
     const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
-    double* gs = a.gpart + (static_cast<int64_t>(c) * a.J + g) * a.G;
     double zz = 0.0;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -342,28 +344,40 @@ __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
         for (int u = 0; u < 8; ++u) v += x[u];
       }
       for (; ch < ch1; ++ch) v += a.cpart[(ch * a.chains + c) * a.E + ee];
+      sv[r] += v;
       if (ee <= K) {
         const double tau = r == 0 ? tau0 : tau1;
         const double zv = z[ee];
         out[1 + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1) + ee] = tau * v - zv;
-        gs[ee] = v;
-        gs[K + 1 + ee] = tau * zv * v;
+        szs[r] += tau * zv * v;
         zz += zv * zv;
-      } else {
-        gs[K + 1 + ee] = v;  // lp (ee = K+1) at 2K+2, bad (ee = K+2) at 2K+3
       }
     }
 #pragma unroll
     for (int off = 16; off > 0; off >>= 1) zz += __shfl_xor(zz, off, 32);
-    if (e == 0) gs[2 * K + 4] = zz;
+    szz += zz;
   }
+  const int nblk = (a.J + kGroupsPerBlock - 1) / kGroupsPerBlock;
+  double* gs = a.gpart + (static_cast<int64_t>(c) * nblk + blockIdx.x) * a.G;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int ee = e + 32 * r;
+    if (ee >= K + 3) continue;
+    if (ee <= K) {
+      gs[ee] = sv[r];
+      gs[K + 1 + ee] = szs[r];
+    } else {
+      gs[K + 1 + ee] = sv[r];  // lp (ee = K+1) at 2K+2, bad (ee = K+2) at 2K+3
+    }
+  }
+  if (e == 0) gs[2 * K + 4] = szz;
 }
 
-// Per chain: sums over groups (sixteen fixed interleaved slices of groups,
-// each summed with four accumulators, then combined in order), priors,
-// Jacobian, output.  Every load of a slice is independent and sixteen are
-// issued before they are added, so the sums are bandwidth- rather than
-// latency-bound.
+// Per chain: sums over the group blocks' rows (sixteen fixed interleaved
+// slices of rows, each summed with four accumulators, then combined in
+// order), priors, Jacobian, output.  Every load of a slice is independent
+// and sixteen are issued before they are added, so the sums are bandwidth-
+// rather than latency-bound.
 __global__ __launch_bounds__(1024) void hier_final(hier_args a) {
   __shared__ double red[16][64];
   __shared__ double st[128];
@@ -372,25 +386,26 @@ __global__ __launch_bounds__(1024) void hier_final(hier_args a) {
   const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
   double* out = a.out + static_cast<int64_t>(c) * a.out_stride;
   const int sl = threadIdx.x >> 6;
+  const int R = (a.J + kGroupsPerBlock - 1) / kGroupsPerBlock;  // rows per chain
   for (int e0 = 0; e0 < a.G; e0 += 64) {
     const int e = e0 + (threadIdx.x & 63);
     if (e < a.G) {
-      const double* gp = a.gpart + static_cast<int64_t>(c) * a.J * a.G + e;
+      const double* gp = a.gpart + static_cast<int64_t>(c) * R * a.G + e;
       double v[4] = {0.0, 0.0, 0.0, 0.0};
       int g = sl;
       // sixteen loads in flight, added in the order of the 4-load steps
-      for (; g + 240 < a.J; g += 256) {
+      for (; g + 240 < R; g += 256) {
         double x[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) x[u] = gp[static_cast<int64_t>(g + 16 * u) * a.G];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u & 3] += x[u];
       }
-      for (; g + 48 < a.J; g += 64) {
+      for (; g + 48 < R; g += 64) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] += gp[static_cast<int64_t>(g + 16 * u) * a.G];
       }
-      for (int u = 0; g < a.J; g += 16, ++u) v[u] += gp[static_cast<int64_t>(g) * a.G];
+      for (int u = 0; g < R; g += 16, ++u) v[u] += gp[static_cast<int64_t>(g) * a.G];
       red[sl][threadIdx.x & 63] = (v[0] + v[1]) + (v[2] + v[3]);
     }
     __syncthreads();
