@@ -89,8 +89,10 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 // Sum of NS block totals over the chain's parts, in part order, so every
-// part gets the same values.  Lane 0 publishes this part's totals and reads
-// every part's.  Each total travels as two 64-bit words, (tag << 32) | one
+// part gets the same values.  Wave 0 publishes this part's totals and
+// waits for every part's, one word per lane, so that the waits overlap
+// instead of following each other.  Each total travels as two 64-bit
+// words, (tag << 32) | one
 // 32-bit half of the double, written and read with relaxed agent-scope
 // atomics: a word that carries this exchange's tag (epoch + 1, mod 2^32)
 // also carries its half (single-copy atomicity), so no release/acquire
@@ -105,42 +107,58 @@ template <int NS>
 __device__ void part_sum(const ctx& k, double (&x)[NS]) {
   static_assert(NS <= 8, "at most 8 totals per exchange");
   const batch_args* a = k.a;
-  const int P = a->parts;
+  const int P = a->parts;  // <= 4: 2 NS P <= 64 words, one per lane of wave 0
   chain_state* s = k.s;
   __syncthreads();
-  if (k.lane == 0) {
+  if (k.lane < 64) {
+    const int ln = k.lane;
     const uint64_t ep = s->xepoch;
     const unsigned long long tag = ((ep + 1) & 0xffffffffull) << 32;
     unsigned long long* base = a->xch + (static_cast<int64_t>(k.c) * 2 + (ep & 1)) * 16 * P;
-    for (int j = 0; j < NS; ++j) {
-      const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(x[j]));
-      unsigned long long* w = base + 16 * k.part() + 2 * j;
-      __hip_atomic_store(w, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(w + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // publish: lane q < 2 NS writes word q of this part (half q % 2 of total q / 2)
+    if (ln < 2 * NS) {
+      double v = x[0];
+#pragma unroll
+      for (int j = 1; j < NS; ++j)
+        if ((ln >> 1) == j) v = x[j];
+      const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
+      const unsigned long long half = (ln & 1) ? (bits >> 32) : (bits & 0xffffffffull);
+      __hip_atomic_store(base + 16 * k.part() + ln, tag | half, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
-    int spins = 0;
-    for (int j = 0; j < NS; ++j) {
+    // collect: lane q < 2 NS P waits for word q % (2 NS) of part q / (2 NS),
+    // all of them at once, and leaves it in LDS
+    unsigned long long* xw = reinterpret_cast<unsigned long long*>(k.red);
+    if (ln < 2 * NS * P) {
+      unsigned long long* w = base + 16 * (ln / (2 * NS)) + ln % (2 * NS);
+      unsigned long long h = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int spins = 0;
+      while ((h & ~0xffffffffull) != tag) {
+        if (++spins > a->spin_limit) {
+          s->xfail = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        h = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      xw[ln] = h;
+    }
+    // the wave's LDS writes land before its reads of them
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (ln < NS) {
       double t = 0.0;
       for (int p = 0; p < P; ++p) {
-        unsigned long long* w = base + 16 * p + 2 * j;
-        unsigned long long h[2];
-        for (int u = 0; u < 2; ++u) {
-          h[u] = __hip_atomic_load(w + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          while ((h[u] & ~0xffffffffull) != tag && !s->xfail) {
-            if (++spins > a->spin_limit) s->xfail = 1;
-            __builtin_amdgcn_s_sleep(1);
-            h[u] = __hip_atomic_load(w + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-        }
-        t += __longlong_as_double(static_cast<long long>((h[0] & 0xffffffffull) | (h[1] << 32)));
+        const unsigned long long lo = xw[p * 2 * NS + 2 * ln], hi = xw[p * 2 * NS + 2 * ln + 1];
+        t += __longlong_as_double(static_cast<long long>((lo & 0xffffffffull) | (hi << 32)));
       }
-      k.red[j] = t;
+      k.red[64 + ln] = t;
     }
-    s->xepoch = ep + 1;
+    if (ln == 0) s->xepoch = ep + 1;
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < NS; ++j) x[j] = k.red[j];
+  for (int j = 0; j < NS; ++j) x[j] = k.red[64 + j];
   __syncthreads();
 }
 
@@ -688,7 +706,7 @@ __device__ void begin_stepsize_heuristic(const ctx& k, int ret_phase) {
 __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   __shared__ chain_state s;
   __shared__ batch_args sa;  // helpers read the arguments through ctx::a
-  __shared__ double red[48];  // block_sum_n: 8 waves x up to 6 values
+  __shared__ double red[72];  // block_sum_n: 8 waves x up to 6 values; part_sum: 64 words + 8 sums
   __shared__ int slice[3];    // d0, d1, part
   // chain and part of this workgroup; with C a multiple of 8 the parts of
   // a chain are 8 block ids apart, i.e. on one XCD (shared L2)
