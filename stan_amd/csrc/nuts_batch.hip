@@ -115,12 +115,17 @@ __device__ void part_sum(const ctx& k, double (&x)[NS]) {
     const uint64_t ep = s->xepoch;
     const unsigned long long tag = ((ep + 1) & 0xffffffffull) << 32;
     unsigned long long* base = a->xch + (static_cast<int64_t>(k.c) * 2 + (ep & 1)) * 16 * P;
-    // publish: lane q < 2 NS writes word q of this part (half q % 2 of total q / 2)
-    if (ln < 2 * NS) {
-      double v = x[0];
+    // publish: lane q < 2 NS writes word q of this part (half q % 2 of total
+    // q / 2); the totals reach the lanes through LDS (every thread holds
+    // them, lane 0 stores them with fixed indices)
+    if (ln == 0) {
 #pragma unroll
-      for (int j = 1; j < NS; ++j)
-        if ((ln >> 1) == j) v = x[j];
+      for (int j = 0; j < NS; ++j) k.red[64 + j] = x[j];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (ln < 2 * NS) {
+      const double v = k.red[64 + (ln >> 1)];
       const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
       const unsigned long long half = (ln & 1) ? (bits >> 32) : (bits & 0xffffffffull);
       __hip_atomic_store(base + 16 * k.part() + ln, tag | half, __ATOMIC_RELAXED,
