@@ -335,7 +335,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(the multi-GPU design shards rows or chains, see DESIGN.md)"}))
+                          "(across GPUs, the plain path would shard rows and the batched "
+                          "sampler whole chains)"}))
         sys.exit(2)
     if args.kernels:
         print(json.dumps({"kernels": kernels_section(args.steps)}))
