@@ -678,7 +678,7 @@ template <int FAM>
 __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   extern __shared__ double st[];
   __shared__ double red[4096];  // [slice][k], Kp x slices = 4096
-  __shared__ double rs[3][16];
+  __shared__ double rs[3][64];
   const int c = blockIdx.x;
   pa.theta += static_cast<int64_t>(c) * pa.theta_stride;
   pa.out += static_cast<int64_t>(c) * pa.out_stride;
@@ -715,14 +715,22 @@ __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   if (sl < nsl)
 #pragma unroll
     for (int i = 0; i < 4; ++i) red[sl * Kp + 4 * kq + i] = v[i];
-  // row terms (lp, sum r, bad): wave 0, entry e on lanes 16 e .. 16 e + 15,
-  // each lane over splits lane, lane + 16, ... then lanes in order
-  if (tid < 48) {
-    const int e = tid >> 4, l = tid & 15;
+  // row terms (lp, sum r, bad): entry e on threads 64 e .. 64 e + 63, each
+  // over splits l, l + 64, ... (eight loads in flight), then the 64 in order
+  if (tid < 192) {
+    const int e = tid >> 6, l = tid & 63;
     const double* sp = a.spart + static_cast<int64_t>(c) * 3 + e;
     const int64_t sstride = 3 * static_cast<int64_t>(a.Cp);
     double t = 0.0;
-    for (int s2 = l; s2 < a.splits; s2 += 16) t += sp[s2 * sstride];
+    int s2 = l;
+    for (; s2 + 7 * 64 < a.splits; s2 += 8 * 64) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = sp[static_cast<int64_t>(s2 + 64 * u) * sstride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t += x[u];
+    }
+    for (; s2 < a.splits; s2 += 64) t += sp[static_cast<int64_t>(s2) * sstride];
     rs[e][l] = t;
   }
   __syncthreads();
@@ -733,7 +741,7 @@ __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   } else if (tid >= 1021) {
     const int e = tid - 1021;
     double t = 0.0;
-    for (int l = 0; l < 16; ++l) t += rs[e][l];
+    for (int l = 0; l < 64; ++l) t += rs[e][l];
     st[e == 0 ? 0 : (e == 1 ? 1 : K + 2)] = t;
   }
   __syncthreads();
