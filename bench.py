@@ -289,7 +289,7 @@ def c1_section(with_cpu=True):
     return out
 
 
-def kernels_section(steps, sampler_iters=30):
+def kernels_section(steps, sampler_iters=30, only=None):
     """Short runs of the batched kernels alone, sized for a kernel trace:
     `steps` batched evaluations of each batched workload (logistic N=1e6
     K=100 with 256 chains and normal N=2e5 K=256 with 256 chains on BF16
@@ -304,15 +304,18 @@ def kernels_section(steps, sampler_iters=30):
              ("normal_n2e5_k256_c256", S.NORMAL, W.normal_batched, 256, S.BF16X2),
              ("hier_logistic_j1000_k20_c64", S.HIER_LOGIT, W.hier_logistic, 64, S.FP32)]
     for name, fam, gen, C, prec in cases:
+        if only and name not in only:
+            continue
         log(f"kernels: {name}")
         d = gen()
         m = S.GlmModel(fam, d["X"], d["y"], offsets=d.get("offsets"), prior=d["prior"],
                        precision=prec)
         th = W.random_thetas(m.D, C, seed=7) * 0.5
         ms, _ = m.time_log_prob_grad(th, iters=steps, warmup=3, chains=C)
-        r = batched_run(m, C, sampler_iters, sampler_iters, prec, max_depth=6)
-        out[name] = dict(ms_per_batched_eval=ms, chains=C, sampler_steps=r["steps"],
-                         sampler_ms_per_step=r["ms_per_step"])
+        out[name] = dict(ms_per_batched_eval=ms, chains=C)
+        if sampler_iters > 0:
+            r = batched_run(m, C, sampler_iters, sampler_iters, prec, max_depth=6)
+            out[name].update(sampler_steps=r["steps"], sampler_ms_per_step=r["ms_per_step"])
         m.close()
     return out
 
@@ -332,6 +335,11 @@ def main():
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--kernels", action="store_true",
                     help="only short runs of the batched kernels (for kernel traces)")
+    ap.add_argument("--case", action="append",
+                    help="with --kernels: run only the named workload (repeatable)")
+    ap.add_argument("--sampler-iters", type=int, default=30,
+                    help="with --kernels: warm-up and draws of the device sampler per case "
+                         "(0: evaluations only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -339,7 +347,8 @@ def main():
                           "sampler whole chains)"}))
         sys.exit(2)
     if args.kernels:
-        print(json.dumps({"kernels": kernels_section(args.steps)}))
+        print(json.dumps({"kernels": kernels_section(args.steps, args.sampler_iters,
+                                                     args.case)}))
         return
 
     import stan_amd as S

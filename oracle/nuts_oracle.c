@@ -556,6 +556,30 @@ static int learn_variance(sampler* s, double* var, const double* q) {
   return 0;
 }
 
+/* var_adaptation_test.cpp:5-27: a variance window fed only zero draws.
+ * Constructs the window state as the reference's constructor and
+ * set_window_params do, calls learn_variance n_learn times with q = 0 and
+ * returns the metric (started at 0) in var. */
+int orc_var_window_zero(int D, int n_learn, unsigned num_warmup, unsigned init_buffer,
+                        unsigned term_buffer, unsigned base_window, double* var) {
+  sampler s;
+  memset(&s, 0, sizeof(s));
+  s.D = D;
+  s.wm = vec(D);
+  s.wm2 = vec(D);
+  double* q = vec(D);
+  wa_restart(&s);
+  wa_set_params(&s, num_warmup, init_buffer, term_buffer, base_window);
+  for (int i = 0; i < D; ++i) var[i] = 0.0;
+  int rc = 0;
+  for (int i = 0; i < n_learn && rc == 0; ++i)
+    if (learn_variance(&s, var, q) < 0) rc = -1;
+  free(q);
+  free(s.wm);
+  free(s.wm2);
+  return rc;
+}
+
 /* covar_adaptation::learn_covariance (covar_adaptation.hpp:17-47) with the
  * Welford running covariance of Stan Math's welford_covar_estimator
  * (absent from the tree; restated from its definition: delta = q - m,

@@ -168,6 +168,9 @@ int orc_build_tree(orc_lpgrad_fn f, void* ctx, orc_rng* rng, double eps,
 /* Dual averaging step (stepsize_adaptation::learn_stepsize). state =
  * {counter, s_bar, x_bar, mu, delta, gamma, kappa, t0}.                */
 void orc_learn_stepsize(double* state, double* epsilon, double adapt_stat);
+/* var_adaptation_test.cpp:5-27: the metric after n_learn zero draws */
+int orc_var_window_zero(int D, int n_learn, unsigned num_warmup, unsigned init_buffer,
+                        unsigned term_buffer, unsigned base_window, double* var);
 
 #ifdef __cplusplus
 }

@@ -49,28 +49,6 @@ struct sx_nuts_batch {
 
 namespace {
 
-// windowed_adaptation::set_window_params (src/stan/mcmc/windowed_adaptation.hpp:29-78)
-void window_params(unsigned num_warmup, unsigned init_buffer, unsigned term_buffer,
-                   unsigned base_window, unsigned* nw, unsigned* ib, unsigned* tb,
-                   unsigned* bw) {
-  *nw = 0;
-  *ib = 0;
-  *tb = 0;
-  *bw = 0;
-  if (num_warmup < 20) return;
-  if (init_buffer + base_window + term_buffer > num_warmup) {
-    *nw = num_warmup;
-    *ib = static_cast<unsigned>(0.15 * num_warmup);
-    *tb = static_cast<unsigned>(0.10 * num_warmup);
-    *bw = num_warmup - (*ib + *tb);
-    return;
-  }
-  *nw = num_warmup;
-  *ib = init_buffer;
-  *tb = term_buffer;
-  *bw = base_window;
-}
-
 int enqueue_gradient(sx_nuts_batch* b) {
   if (b->m->family == SX_HIER_LOGIT)
     return sx::hier_enqueue(b->m, b->hw, b->d_theta, static_cast<int>(b->C), b->d_out, 1, 1,
@@ -239,9 +217,11 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
     }
   }
   // chain scalar state
-  unsigned nw, ib, tb, bw;
-  window_params(static_cast<unsigned>(std::max(0, cfg->num_warmup)), cfg->init_buffer,
-                cfg->term_buffer, cfg->window, &nw, &ib, &tb, &bw);
+  // the metric windows: the schedule the host samplers compute
+  stanx::callbacks::logger quiet;
+  const stanx::mcmc::window_plan wplan = stanx::mcmc::plan_windows(
+      static_cast<unsigned>(std::max(0, cfg->num_warmup)), cfg->init_buffer, cfg->term_buffer,
+      cfg->window, "variance", quiet);
   {
     std::vector<sx::nb::chain_state> st(static_cast<size_t>(chains));
     for (auto& s : st) {
@@ -250,8 +230,6 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
       s.nom_eps = cfg->stepsize > 0 ? cfg->stepsize : 0.1;
       s.eps = s.nom_eps;
       s.mu = std::log(10 * cfg->stepsize);
-      s.w_size = bw;
-      s.w_next = ib + bw - 1;
       s.iter_end = std::max(0, cfg->num_warmup) + std::max(0, cfg->num_samples);
     }
     if (hipMemcpy(b->a.states, st.data(), sizeof(sx::nb::chain_state) * chains,
@@ -284,9 +262,7 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
   a.kappa = cfg->kappa > 0 ? cfg->kappa : 0.75;
   a.t0 = cfg->t0 > 0 ? cfg->t0 : 10;
   a.max_deltaH = 1000;
-  a.w_num_warmup = nw;
-  a.w_init_buffer = ib;
-  a.w_term_buffer = tb;
+  a.wplan = wplan;
   // batched FP64 gradient: every chain's pass on grid.y
   if (m->family != SX_HIER_LOGIT) {
     // each chain's pass uses the launch shape of the single-chain pass: a

@@ -599,20 +599,16 @@ __device__ void learn_stepsize(const ctx& k, double stat) {
   s->nom_eps = exp(x);
 }
 
-// windowed variance adaptation; returns 1 update, 0 none, -1 non-finite
-// windowed_adaptation: the draw enters the variance estimate (Welford)
+// windowed variance adaptation over the run's window plan
+// (stanx/mcmc/warmup_plan.hpp): does this transition's draw feed the
+// running variance?
 __device__ bool in_adapt_window(const ctx& k) {
-  const chain_state* s = k.s;
-  const batch_args* a = k.a;
-  const unsigned nw = a->w_num_warmup, ib = a->w_init_buffer, tb = a->w_term_buffer;
-  return (s->w_counter >= ib) && (s->w_counter < nw - tb) && (s->w_counter != nw);
+  return stanx::mcmc::plan_feeds(k.a->wplan, k.s->w_counter);
 }
 // Welford update of the variance estimate with the draw (at ZQ); the
 // end-of-transition pass may already have applied it (welford_done)
 __device__ int learn_variance(const ctx& k, bool welford_done) {
   chain_state* s = k.s;
-  const batch_args* a = k.a;
-  const unsigned nw = a->w_num_warmup, tb = a->w_term_buffer;
   if (in_adapt_window(k)) {
     const double n = s->w_n + 1.0;
     if (!welford_done) {
@@ -632,21 +628,11 @@ __device__ int learn_variance(const ctx& k, bool welford_done) {
     if (k.lane == 0) s->w_n = n;
   }
   __syncthreads();
-  const bool end = (s->w_counter == s->w_next) && (s->w_counter != nw);
+  const bool end = stanx::mcmc::plan_closes(k.a->wplan, s->w_cursor, s->w_counter);
   if (!end) {
     __syncthreads();
     if (k.lane == 0) s->w_counter++;
     return 0;
-  }
-  // compute_next_window
-  unsigned next = s->w_next, size = s->w_size;
-  if (!(next == nw - tb - 1)) {
-    size *= 2;
-    next = s->w_counter + size;
-    if (!(next == nw - tb - 1)) {
-      const unsigned boundary = next + 2 * size;
-      if (boundary >= nw - tb) next = nw - tb - 1;
-    }
   }
   const double n = s->w_n;
   double* minv = k.slot(MINV);
@@ -672,8 +658,7 @@ __device__ int learn_variance(const ctx& k, bool welford_done) {
     all_finite = nbad[0] == 0.0;
   }
   if (k.lane == 0) {
-    s->w_next = next;
-    s->w_size = size;
+    s->w_cursor++;
     s->w_n = 0;
     s->w_counter++;
   }

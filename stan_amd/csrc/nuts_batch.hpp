@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "../include/stanx/mcmc/warmup_plan.hpp"
+
 namespace sx {
 namespace nb {
 
@@ -35,7 +37,8 @@ struct chain_state {
   int ss_ret, ss_iter, ss_dir, last_ok;
   int mlevel, mvalid;  // PH_MERGE: next level of the ladder above a leaf, still valid
   int stage[kMaxDepth + 1];
-  unsigned w_counter, w_next, w_size;
+  unsigned w_counter;  // warm-up transitions seen by the metric windows
+  int w_cursor;        // next close of the window plan not yet passed
   int iter_end;  // warm-up + sampling iterations; raised to extend a run
   int xfail;     // a cross-part exchange timed out (parts > 1)
   uint64_t rng;
@@ -62,7 +65,7 @@ struct batch_args {
   uint32_t seed, chain0;
   int num_warmup, num_samples, num_thin, save_warmup, max_depth, max_init_tries;
   double init_radius, jitter, delta, gamma, kappa, t0, max_deltaH;
-  unsigned w_num_warmup, w_init_buffer, w_term_buffer;
+  stanx::mcmc::window_plan wplan;  // metric windows (the host samplers' schedule)
   // Parts per chain: with parts > 1 each chain runs on `parts` workgroups,
   // each on a slice of the D-vectors; block totals of every reduction are
   // combined across the parts through xch (C x 2 x parts x 16 tagged

@@ -28,15 +28,15 @@ class adapt_dense_e_nuts : public dense_e_nuts<Model> {
  public:
   adapt_dense_e_nuts(const Model& model, rng_t& rng)
       : dense_e_nuts<Model>(model, rng),
-        covar_adaptation_(static_cast<int>(model.num_params_r())) {}
+        covar_adaptation_(static_cast<int>(model.num_params_r()), "covariance") {}
 
   void engage_adaptation() { adapt_flag_ = true; }
   void disengage_adaptation() {
     adapt_flag_ = false;
-    stepsize_adaptation_.complete_adaptation(this->nom_epsilon_);
+    this->nom_epsilon_ = stepsize_adaptation_.final_step_size();
   }
   bool adapting() const { return adapt_flag_; }
-  stepsize_adaptation& get_stepsize_adaptation() { return stepsize_adaptation_; }
+  step_size_tuner& get_stepsize_adaptation() { return stepsize_adaptation_; }
   covar_adaptation& get_covar_adaptation() { return covar_adaptation_; }
   void set_window_params(unsigned int num_warmup, unsigned int init_buffer,
                          unsigned int term_buffer, unsigned int base_window,
@@ -50,9 +50,9 @@ class adapt_dense_e_nuts : public dense_e_nuts<Model> {
   sample transition(const sample& init_sample, callbacks::logger& logger) {
     sample s = base_nuts<Model, dense_e_metric<Model>>::transition(init_sample, logger);
     if (adapt_flag_) {
-      stepsize_adaptation_.learn_stepsize(this->nom_epsilon_, s.accept_stat());
+      this->nom_epsilon_ = stepsize_adaptation_.update(s.accept_stat());
       const bool update =
-          covar_adaptation_.learn_covariance(this->z_.inv_e_metric_, this->z_.q);
+          covar_adaptation_.observe(this->z_.inv_e_metric_, this->z_.q);
       if (update) {
         this->z_.factorize();
         this->init_stepsize(logger);
@@ -65,7 +65,7 @@ class adapt_dense_e_nuts : public dense_e_nuts<Model> {
 
  protected:
   bool adapt_flag_ = false;
-  stepsize_adaptation stepsize_adaptation_;
+  step_size_tuner stepsize_adaptation_;
   covar_adaptation covar_adaptation_;
 };
 

@@ -30,15 +30,15 @@ class adapt_diag_e_nuts : public diag_e_nuts<Model> {
  public:
   adapt_diag_e_nuts(const Model& model, rng_t& rng)
       : diag_e_nuts<Model>(model, rng),
-        var_adaptation_(static_cast<int>(model.num_params_r())) {}
+        var_adaptation_(static_cast<int>(model.num_params_r()), "variance") {}
 
   void engage_adaptation() { adapt_flag_ = true; }
   void disengage_adaptation() {
     adapt_flag_ = false;
-    stepsize_adaptation_.complete_adaptation(this->nom_epsilon_);
+    this->nom_epsilon_ = stepsize_adaptation_.final_step_size();
   }
   bool adapting() const { return adapt_flag_; }
-  stepsize_adaptation& get_stepsize_adaptation() { return stepsize_adaptation_; }
+  step_size_tuner& get_stepsize_adaptation() { return stepsize_adaptation_; }
   var_adaptation& get_var_adaptation() { return var_adaptation_; }
   void set_window_params(unsigned int num_warmup, unsigned int init_buffer,
                          unsigned int term_buffer, unsigned int base_window,
@@ -50,9 +50,9 @@ class adapt_diag_e_nuts : public diag_e_nuts<Model> {
   sample transition(const sample& init_sample, callbacks::logger& logger) {
     sample s = base_nuts<Model>::transition(init_sample, logger);
     if (adapt_flag_) {
-      stepsize_adaptation_.learn_stepsize(this->nom_epsilon_, s.accept_stat());
+      this->nom_epsilon_ = stepsize_adaptation_.update(s.accept_stat());
       const bool update =
-          var_adaptation_.learn_variance(this->z_.inv_e_metric_, this->z_.q);
+          var_adaptation_.observe(this->z_.inv_e_metric_, this->z_.q);
       if (update) {
         this->init_stepsize(logger);
         stepsize_adaptation_.set_mu(std::log(10 * this->nom_epsilon_));
@@ -64,7 +64,7 @@ class adapt_diag_e_nuts : public diag_e_nuts<Model> {
 
  protected:
   bool adapt_flag_ = false;
-  stepsize_adaptation stepsize_adaptation_;
+  step_size_tuner stepsize_adaptation_;
   var_adaptation var_adaptation_;
 };
 

@@ -1,301 +1,213 @@
-// Warm-up adaptation, mirroring
-//   stepsize_adaptation   src/stan/mcmc/stepsize_adaptation.hpp:11-84
-//   windowed_adaptation   src/stan/mcmc/windowed_adaptation.hpp:12-123
-//   var_adaptation        src/stan/mcmc/var_adaptation.hpp:12-50
-//   covar_adaptation      src/stan/mcmc/covar_adaptation.hpp:12-51
-// The running variance and covariance are Welford's (1962) update, the
-// algorithm of Stan Math's welford_var_estimator and
-// welford_covar_estimator (absent from the tree).
+// Warm-up adaptation of the host samplers: the step size by dual averaging
+// and the inverse metric by running moments over the windows of a
+// window_plan.  The arithmetic follows the reference so that draws match
+// (paths relative to the stan-dev/stan 2.35.0 tree):
+//   dual averaging       src/stan/mcmc/stepsize_adaptation.hpp:55-73
+//   variance window      src/stan/mcmc/var_adaptation.hpp:17-46
+//   covariance window    src/stan/mcmc/covar_adaptation.hpp:17-47
+//   window schedule      src/stan/mcmc/windowed_adaptation.hpp:23-110 (warmup_plan.hpp)
+// The running moments are Welford's (1962) update, the algorithm of Stan
+// Math's welford_var_estimator and welford_covar_estimator (absent from the
+// tree).  The device sampler applies the same updates per chain
+// (csrc/nuts_batch.hip) and reads the same window_plan.
 #ifndef STANX_MCMC_ADAPTATION_HPP
 #define STANX_MCMC_ADAPTATION_HPP
 
+#include <algorithm>
 #include <cmath>
-#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../callbacks.hpp"
+#include "warmup_plan.hpp"
 
 namespace stanx {
 namespace mcmc {
 
-class base_adaptation {
+// Nesterov dual averaging of log(step size) (Hoffman and Gelman 2014,
+// algorithm 5).  Targets mean acceptance `delta`; `mu` is the point the
+// iterates shrink towards, `gamma` the shrinkage, `t0` damps the first
+// iterations and `kappa` sets the decay of the averaging weights.
+class step_size_tuner {
  public:
-  virtual ~base_adaptation() {}
-  virtual void restart() {}
-};
+  struct params {
+    double mu = 0.5, delta = 0.5, gamma = 0.05, kappa = 0.75, t0 = 10;
+  };
 
-class stepsize_adaptation : public base_adaptation {
- public:
-  stepsize_adaptation()
-      : mu_(0.5), delta_(0.5), gamma_(0.05), kappa_(0.75), t0_(10) {
-    restart();
-  }
-  void set_mu(double m) { mu_ = m; }
-  void set_delta(double d) {
-    if (d > 0 && d < 1) delta_ = d;
-  }
-  void set_gamma(double g) {
-    if (g > 0) gamma_ = g;
-  }
-  void set_kappa(double k) {
-    if (k > 0) kappa_ = k;
-  }
-  void set_t0(double t) {
-    if (t > 0) t0_ = t;
-  }
-  double get_mu() const { return mu_; }
-  double get_delta() const { return delta_; }
-  double get_gamma() const { return gamma_; }
-  double get_kappa() const { return kappa_; }
-  double get_t0() const { return t0_; }
-  void restart() override {
-    counter_ = 0;
-    s_bar_ = 0;
-    x_bar_ = 0;
-  }
-  // Nesterov dual averaging of log(epsilon) (Hoffman and Gelman 2014)
-  void learn_stepsize(double& epsilon, double adapt_stat) {
-    ++counter_;
-    adapt_stat = adapt_stat > 1 ? 1 : adapt_stat;
-    const double eta = 1.0 / (counter_ + t0_);
-    s_bar_ = (1.0 - eta) * s_bar_ + eta * (delta_ - adapt_stat);
-    const double x = mu_ - s_bar_ * std::sqrt(counter_) / gamma_;
-    const double x_eta = std::pow(counter_, -kappa_);
-    x_bar_ = (1.0 - x_eta) * x_bar_ + x_eta * x;
-    epsilon = std::exp(x);
-  }
-  void complete_adaptation(double& epsilon) { epsilon = std::exp(x_bar_); }
+  // Out-of-domain values keep the current setting (delta in (0, 1); gamma,
+  // kappa, t0 positive), as the reference's setters do.
+  void set_mu(double m) { p_.mu = m; }
+  void set_delta(double d) { p_.delta = (d > 0 && d < 1) ? d : p_.delta; }
+  void set_gamma(double g) { p_.gamma = g > 0 ? g : p_.gamma; }
+  void set_kappa(double k) { p_.kappa = k > 0 ? k : p_.kappa; }
+  void set_t0(double t) { p_.t0 = t > 0 ? t : p_.t0; }
+  const params& settings() const { return p_; }
 
-  // full state access (tests restate goldens that set it directly)
-  void set_state(double counter, double s_bar, double x_bar) {
-    counter_ = counter;
-    s_bar_ = s_bar;
-    x_bar_ = x_bar;
-  }
-  double counter() const { return counter_; }
-  double s_bar() const { return s_bar_; }
-  double x_bar() const { return x_bar_; }
+  // forget the iterates (a new metric starts a new averaging run)
+  void restart() { n_ = gap_ = avg_ = 0.0; }
 
- protected:
-  double counter_, s_bar_, x_bar_;
-  double mu_, delta_, gamma_, kappa_, t0_;
-};
+  // One iteration with acceptance statistic `accept`: the step size to use
+  // next.  gap_ is the running mean of (delta - accept), avg_ the weighted
+  // mean of the log step sizes.
+  double update(double accept) {
+    n_ += 1.0;
+    const double a = std::min(accept, 1.0);
+    const double w = 1.0 / (n_ + p_.t0);
+    gap_ = (1.0 - w) * gap_ + w * (p_.delta - a);
+    const double log_eps = p_.mu - gap_ * std::sqrt(n_) / p_.gamma;
+    const double v = std::pow(n_, -p_.kappa);
+    avg_ = (1.0 - v) * avg_ + v * log_eps;
+    return std::exp(log_eps);
+  }
+  // the step size after warm-up: the average iterate
+  double final_step_size() const { return std::exp(avg_); }
 
-class windowed_adaptation : public base_adaptation {
- public:
-  explicit windowed_adaptation(std::string name) : estimator_name_(name) {
-    num_warmup_ = 0;
-    adapt_init_buffer_ = 0;
-    adapt_term_buffer_ = 0;
-    adapt_base_window_ = 0;
-    restart();
+  // iteration state (tests set it to restate the reference's goldens)
+  void set_iterates(double n, double gap, double avg) {
+    n_ = n;
+    gap_ = gap;
+    avg_ = avg;
   }
-  void restart() override {
-    adapt_window_counter_ = 0;
-    adapt_window_size_ = adapt_base_window_;
-    adapt_next_window_ = adapt_init_buffer_ + adapt_window_size_ - 1;
-  }
-  void set_window_params(unsigned int num_warmup, unsigned int init_buffer,
-                         unsigned int term_buffer, unsigned int base_window,
-                         callbacks::logger& logger) {
-    if (num_warmup < 20) {
-      logger.info("WARNING: No " + estimator_name_ + " estimation is");
-      logger.info("         performed for num_warmup < 20");
-      logger.info("");
-      return;
-    }
-    if (init_buffer + base_window + term_buffer > num_warmup) {
-      logger.info("WARNING: There aren't enough warmup iterations to fit the");
-      logger.info("         three stages of adaptation as currently configured.");
-      num_warmup_ = num_warmup;
-      adapt_init_buffer_ = 0.15 * num_warmup;
-      adapt_term_buffer_ = 0.10 * num_warmup;
-      adapt_base_window_ = num_warmup - (adapt_init_buffer_ + adapt_term_buffer_);
-      logger.info("         Reducing each adaptation stage to 15%/75%/10% of");
-      logger.info("         the given number of warmup iterations:");
-      std::stringstream m1, m2, m3;
-      m1 << "           init_buffer = " << adapt_init_buffer_;
-      logger.info(m1);
-      m2 << "           adapt_window = " << adapt_base_window_;
-      logger.info(m2);
-      m3 << "           term_buffer = " << adapt_term_buffer_;
-      logger.info(m3);
-      logger.info("");
-      return;
-    }
-    num_warmup_ = num_warmup;
-    adapt_init_buffer_ = init_buffer;
-    adapt_term_buffer_ = term_buffer;
-    adapt_base_window_ = base_window;
-    restart();
-  }
-  bool adaptation_window() const {
-    return (adapt_window_counter_ >= adapt_init_buffer_) &&
-           (adapt_window_counter_ < num_warmup_ - adapt_term_buffer_) &&
-           (adapt_window_counter_ != num_warmup_);
-  }
-  bool end_adaptation_window() const {
-    return (adapt_window_counter_ == adapt_next_window_) &&
-           (adapt_window_counter_ != num_warmup_);
-  }
-  void compute_next_window() {
-    if (adapt_next_window_ == num_warmup_ - adapt_term_buffer_ - 1) return;
-    adapt_window_size_ *= 2;
-    adapt_next_window_ = adapt_window_counter_ + adapt_window_size_;
-    if (adapt_next_window_ == num_warmup_ - adapt_term_buffer_ - 1) return;
-    unsigned int next_window_boundary =
-        adapt_next_window_ + 2 * adapt_window_size_;
-    if (next_window_boundary >= num_warmup_ - adapt_term_buffer_)
-      adapt_next_window_ = num_warmup_ - adapt_term_buffer_ - 1;
-  }
-  unsigned int window_counter() const { return adapt_window_counter_; }
-  unsigned int next_window() const { return adapt_next_window_; }
-  unsigned int init_buffer() const { return adapt_init_buffer_; }
-  unsigned int term_buffer() const { return adapt_term_buffer_; }
-  unsigned int base_window() const { return adapt_base_window_; }
-
- protected:
-  std::string estimator_name_;
-  unsigned int num_warmup_, adapt_init_buffer_, adapt_term_buffer_,
-      adapt_base_window_;
-  unsigned int adapt_window_counter_, adapt_next_window_, adapt_window_size_;
-};
-
-class welford_var_estimator {
- public:
-  explicit welford_var_estimator(int n) : m_(n, 0.0), m2_(n, 0.0) {}
-  void restart() {
-    num_samples_ = 0;
-    std::fill(m_.begin(), m_.end(), 0.0);
-    std::fill(m2_.begin(), m2_.end(), 0.0);
-  }
-  void add_sample(const std::vector<double>& q) {
-    num_samples_ += 1.0;
-    for (size_t i = 0; i < m_.size(); ++i) {
-      const double delta = q[i] - m_[i];
-      m_[i] += delta / num_samples_;
-      m2_[i] += (q[i] - m_[i]) * delta;
-    }
-  }
-  void sample_variance(std::vector<double>& var) const {
-    if (num_samples_ > 1)
-      for (size_t i = 0; i < m_.size(); ++i) var[i] = m2_[i] / (num_samples_ - 1.0);
-  }
-  double num_samples() const { return num_samples_; }
+  double iterations() const { return n_; }
+  double gap() const { return gap_; }
+  double average() const { return avg_; }
 
  private:
-  double num_samples_ = 0;
-  std::vector<double> m_, m2_;
+  params p_;
+  double n_ = 0.0, gap_ = 0.0, avg_ = 0.0;
 };
 
-class var_adaptation : public windowed_adaptation {
+// Running mean and variance per coordinate (Welford).
+class running_variance {
  public:
-  explicit var_adaptation(int n)
-      : windowed_adaptation("variance"), estimator_(n) {}
-  bool learn_variance(std::vector<double>& var, const std::vector<double>& q) {
-    if (adaptation_window()) estimator_.add_sample(q);
-    if (end_adaptation_window()) {
-      compute_next_window();
-      estimator_.sample_variance(var);
-      const double n = estimator_.num_samples();
-      bool finite = true;
-      for (auto& v : var) {
-        v = (n / (n + 5.0)) * v + 1e-3 * (5.0 / (n + 5.0)) * 1.0;
-        if (!std::isfinite(v)) finite = false;
+  explicit running_variance(int n) : mean_(n, 0.0), ss_(n, 0.0) {}
+  void clear() {
+    count_ = 0.0;
+    std::fill(mean_.begin(), mean_.end(), 0.0);
+    std::fill(ss_.begin(), ss_.end(), 0.0);
+  }
+  void push(const std::vector<double>& x) {
+    count_ += 1.0;
+    for (size_t i = 0; i < mean_.size(); ++i) {
+      const double d = x[i] - mean_[i];
+      mean_[i] += d / count_;
+      ss_[i] += (x[i] - mean_[i]) * d;
+    }
+  }
+  double count() const { return count_; }
+  // sample variance into `out` (left as it is with fewer than two draws)
+  void estimate(std::vector<double>& out) const {
+    if (count_ <= 1) return;
+    for (size_t i = 0; i < mean_.size(); ++i) out[i] = ss_[i] / (count_ - 1.0);
+  }
+  // shrink `est` towards 1e-3 (times the identity) by the draw count:
+  // est <- n / (n + 5) est + 1e-3 * 5 / (n + 5) on the diagonal
+  static bool regularise(std::vector<double>& est, double n, int /*dim*/) {
+    const double keep = n / (n + 5.0), floor = 1e-3 * (5.0 / (n + 5.0));
+    bool ok = true;
+    for (double& v : est) {
+      v = keep * v + floor * 1.0;
+      ok = ok && std::isfinite(v);
+    }
+    return ok;
+  }
+
+ private:
+  double count_ = 0.0;
+  std::vector<double> mean_, ss_;
+};
+
+// Running mean and covariance (Welford), row-major dim x dim:
+// d = x - mean; mean += d / n; ss += (x - mean) d^T.
+class running_covariance {
+ public:
+  explicit running_covariance(int n)
+      : dim_(n), mean_(n, 0.0), ss_(static_cast<size_t>(n) * n, 0.0), d_(n, 0.0) {}
+  void clear() {
+    count_ = 0.0;
+    std::fill(mean_.begin(), mean_.end(), 0.0);
+    std::fill(ss_.begin(), ss_.end(), 0.0);
+  }
+  void push(const std::vector<double>& x) {
+    count_ += 1.0;
+    for (int i = 0; i < dim_; ++i) d_[i] = x[i] - mean_[i];
+    for (int i = 0; i < dim_; ++i) mean_[i] += d_[i] / count_;
+    for (int i = 0; i < dim_; ++i) {
+      const double e = x[i] - mean_[i];
+      double* row = ss_.data() + static_cast<size_t>(i) * dim_;
+      for (int j = 0; j < dim_; ++j) row[j] += e * d_[j];
+    }
+  }
+  double count() const { return count_; }
+  void estimate(std::vector<double>& out) const {
+    if (count_ <= 1) return;
+    for (size_t k = 0; k < ss_.size(); ++k) out[k] = ss_[k] / (count_ - 1.0);
+  }
+  static bool regularise(std::vector<double>& est, double n, int dim) {
+    const double keep = n / (n + 5.0), floor = 1e-3 * (5.0 / (n + 5.0));
+    bool ok = true;
+    for (int i = 0; i < dim; ++i)
+      for (int j = 0; j < dim; ++j) {
+        double& v = est[static_cast<size_t>(i) * dim + j];
+        v = keep * v + floor * (i == j ? 1.0 : 0.0);
+        ok = ok && std::isfinite(v);
       }
-      if (!finite)
-        throw std::runtime_error(
-            "Numerical overflow in metric adaptation. "
-            "This occurs when the sampler encounters extreme values on the "
-            "unconstrained space; this may happen when the posterior density "
-            "function is too wide or improper. "
-            "There may be problems with your model specification.");
-      estimator_.restart();
-      ++adapt_window_counter_;
-      return true;
-    }
-    ++adapt_window_counter_;
-    return false;
+    return ok;
   }
-
- protected:
-  welford_var_estimator estimator_;
-};
-
-// Running mean and covariance: delta = q - m, m += delta / n,
-// m2 += (q - m) delta^T (row-major D x D), covariance m2 / (n - 1).
-class welford_covar_estimator {
- public:
-  explicit welford_covar_estimator(int n)
-      : n_(n), m_(n, 0.0), m2_(static_cast<size_t>(n) * n, 0.0), delta_(n, 0.0) {}
-  void restart() {
-    num_samples_ = 0;
-    std::fill(m_.begin(), m_.end(), 0.0);
-    std::fill(m2_.begin(), m2_.end(), 0.0);
-  }
-  void add_sample(const std::vector<double>& q) {
-    num_samples_ += 1.0;
-    for (int i = 0; i < n_; ++i) delta_[i] = q[i] - m_[i];
-    for (int i = 0; i < n_; ++i) m_[i] += delta_[i] / num_samples_;
-    for (int i = 0; i < n_; ++i) {
-      const double d = q[i] - m_[i];
-      double* row = m2_.data() + static_cast<size_t>(i) * n_;
-      for (int j = 0; j < n_; ++j) row[j] += d * delta_[j];
-    }
-  }
-  void sample_covariance(std::vector<double>& covar) const {
-    if (num_samples_ > 1)
-      for (size_t k = 0; k < m2_.size(); ++k) covar[k] = m2_[k] / (num_samples_ - 1.0);
-  }
-  double num_samples() const { return num_samples_; }
 
  private:
-  int n_;
-  double num_samples_ = 0;
-  std::vector<double> m_, m2_, delta_;
+  int dim_;
+  double count_ = 0.0;
+  std::vector<double> mean_, ss_, d_;
 };
 
-// Windowed covariance estimation of a dense inverse metric (row-major):
-// at each window end the regularised estimate n/(n+5) C + 1e-3 5/(n+5) I.
-class covar_adaptation : public windowed_adaptation {
+// The inverse metric estimated over the windows of a plan.  observe() is
+// called once per warm-up transition with its draw; it returns true when a
+// window closed and `metric` holds the new estimate.
+template <class Moments>
+class metric_windows {
  public:
-  explicit covar_adaptation(int n)
-      : windowed_adaptation("covariance"), n_(n), estimator_(n) {}
-  bool learn_covariance(std::vector<double>& covar, const std::vector<double>& q) {
-    if (adaptation_window()) estimator_.add_sample(q);
-    if (end_adaptation_window()) {
-      compute_next_window();
-      estimator_.sample_covariance(covar);
-      const double n = estimator_.num_samples();
-      const double a = n / (n + 5.0), b = 1e-3 * (5.0 / (n + 5.0));
-      bool finite = true;
-      for (int i = 0; i < n_; ++i)
-        for (int j = 0; j < n_; ++j) {
-          double& c = covar[static_cast<size_t>(i) * n_ + j];
-          c = a * c + b * (i == j ? 1.0 : 0.0);
-          if (!std::isfinite(c)) finite = false;
-        }
-      if (!finite)
-        throw std::runtime_error(
-            "Numerical overflow in metric adaptation. "
-            "This occurs when the sampler encounters extreme values on the "
-            "unconstrained space; this may happen when the posterior density "
-            "function is too wide or improper. "
-            "There may be problems with your model specification.");
-      estimator_.restart();
-      ++adapt_window_counter_;
-      return true;
-    }
-    ++adapt_window_counter_;
-    return false;
+  metric_windows(int dim, std::string estimate)
+      : dim_(dim), what_(std::move(estimate)), moments_(dim) {
+    plan_ = window_plan{};
   }
 
- protected:
-  int n_;
-  welford_covar_estimator estimator_;
+  void set_window_params(unsigned num_warmup, unsigned init_buffer, unsigned term_buffer,
+                         unsigned base_window, callbacks::logger& logger) {
+    plan_ = plan_windows(num_warmup, init_buffer, term_buffer, base_window, what_, logger);
+    it_ = 0;
+    cursor_ = 0;
+  }
+  const window_plan& plan() const { return plan_; }
+
+  bool observe(std::vector<double>& metric, const std::vector<double>& q) {
+    const unsigned it = it_++;
+    if (plan_feeds(plan_, it)) moments_.push(q);
+    if (!plan_closes(plan_, cursor_, it)) return false;
+    ++cursor_;
+    moments_.estimate(metric);
+    const double n = moments_.count();
+    if (!Moments::regularise(metric, n, dim_))
+      throw std::runtime_error(
+          "Numerical overflow in metric adaptation: the " + what_ +
+          " estimate is not finite.  The sampler met extreme values on the "
+          "unconstrained space; the posterior may be improper or too wide.");
+    moments_.clear();
+    return true;
+  }
+
+ private:
+  int dim_;
+  std::string what_;
+  Moments moments_;
+  window_plan plan_;
+  unsigned it_ = 0;
+  int cursor_ = 0;
 };
+
+using var_adaptation = metric_windows<running_variance>;
+using covar_adaptation = metric_windows<running_covariance>;
 
 }  // namespace mcmc
 }  // namespace stanx

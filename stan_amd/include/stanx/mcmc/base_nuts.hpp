@@ -122,51 +122,60 @@ class base_nuts {
       epsilon_ *= 1.0 + epsilon_jitter_ * (2.0 * rng_.uniform() - 1.0);
   }
 
-  // step-size heuristic (base_hmc.hpp:93-161)
+  // Step-size heuristic run before sampling and after each metric update
+  // (the behaviour of base_hmc::init_stepsize, base_hmc.hpp:93-161).  A
+  // probe refreshes the momentum at the starting point and takes one
+  // leapfrog step of the nominal size; the first probe fixes the direction
+  // (grow while the energy change stays above log 0.8, otherwise shrink),
+  // and the size doubles or halves until a probe's energy change crosses
+  // log 0.8.  Every probe starts from the same point and draws a fresh
+  // momentum, so the random stream is consumed as the reference consumes it.
   void init_stepsize(callbacks::logger& logger) {
-    ps_point z_init(static_cast<int>(z_.q.size()));
-    z_init.assign(z_);
-    if (z_.q.size() == 0) {
+    if (z_.q.empty()) {  // no step size in a zero-dimensional space
       nom_epsilon_ = std::numeric_limits<double>::quiet_NaN();
       return;
     }
-    if (nom_epsilon_ == 0 || nom_epsilon_ > 1e7 || std::isnan(nom_epsilon_))
-      return;
-    hamiltonian_.sample_p(z_, rng_);
-    hamiltonian_.init(z_, logger);
-    double H0 = hamiltonian_.H(z_);
-    integrator_.evolve(z_, hamiltonian_, nom_epsilon_, logger);
-    double h = hamiltonian_.H(z_);
-    if (std::isnan(h)) h = std::numeric_limits<double>::infinity();
-    double delta_H = H0 - h;
-    const int direction = delta_H > std::log(0.8) ? 1 : -1;
-    while (1) {
-      z_.assign(z_init);
+    if (nom_epsilon_ == 0 || nom_epsilon_ > 1e7 || std::isnan(nom_epsilon_)) return;
+    const double log_accept = std::log(0.8);
+    ps_point start(static_cast<int>(z_.q.size()));
+    start.assign(z_);
+    auto probe = [&]() {  // energy change of one step from `start`
+      z_.assign(start);
       hamiltonian_.sample_p(z_, rng_);
       hamiltonian_.init(z_, logger);
-      double H0b = hamiltonian_.H(z_);
+      const double before = hamiltonian_.H(z_);
       integrator_.evolve(z_, hamiltonian_, nom_epsilon_, logger);
-      double hb = hamiltonian_.H(z_);
-      if (std::isnan(hb)) hb = std::numeric_limits<double>::infinity();
-      double dH = H0b - hb;
-      if ((direction == 1) && !(dH > std::log(0.8)))
-        break;
-      else if ((direction == -1) && !(dH < std::log(0.8)))
-        break;
-      else
-        nom_epsilon_ = direction == 1 ? 2.0 * nom_epsilon_ : 0.5 * nom_epsilon_;
+      const double after = hamiltonian_.H(z_);
+      return before - (std::isnan(after) ? std::numeric_limits<double>::infinity() : after);
+    };
+    const bool grow = probe() > log_accept;
+    for (;;) {
+      const double dH = probe();
+      const bool crossed = grow ? !(dH > log_accept) : !(dH < log_accept);
+      if (crossed) break;
+      nom_epsilon_ *= grow ? 2.0 : 0.5;
       if (nom_epsilon_ > 1e7)
         throw std::runtime_error(
-            "Posterior is improper. Please check your model.");
+            "Step-size search: the step size grew past 1e7 with the energy change still "
+            "small; the posterior is improper.  Check the model.");
       if (nom_epsilon_ == 0)
         throw std::runtime_error(
-            "No acceptably small step size could be found. Perhaps the "
-            "posterior is not continuous?");
+            "Step-size search: the step size shrank to zero without an acceptable energy "
+            "change; the log density may not be continuous.");
     }
-    z_.assign(z_init);
+    z_.assign(start);
   }
 
-  // ---- NUTS transition (base_nuts.hpp:78-204) ----
+  // One multinomial NUTS transition (the behaviour of base_nuts::transition,
+  // base_nuts.hpp:78-204).  The trajectory is kept as its two sides,
+  // side 0 backward and side 1 forward, so that growing either way is one
+  // code path: a doubling picks a side by a uniform draw, hands the whole
+  // trajectory so far to the other side as its subtree, and builds a new
+  // subtree of the current depth from the chosen side's outer point.  The
+  // new subtree's proposal replaces the sample with probability
+  // min(1, w_new / w_old) (biased progressive sampling); three U-turn
+  // checks over the merged trajectory and its two joints decide whether
+  // to double again.
   sample transition(const sample& init_sample, callbacks::logger& logger) {
     const int D = static_cast<int>(z_.q.size());
     sample_stepsize();
@@ -174,96 +183,52 @@ class base_nuts {
     hamiltonian_.sample_p(z_, rng_);
     hamiltonian_.init(z_, logger);
     ensure_frames(D);
+    trajectory& t = traj_;
+    vec_t p_sharp(z_.p.size());
+    hamiltonian_.dtau_dp(z_, p_sharp);
+    for (side& e : t.sides) {
+      e.z.assign(z_);
+      e.p_out = z_.p;
+      e.p_in = z_.p;
+      e.ps_out = p_sharp;
+      e.ps_in = p_sharp;
+    }
+    t.sample.assign(z_);
+    t.propose.assign(z_);
+    t.rho = z_.p;
 
-    ps_point& z_fwd = top_.z_fwd;
-    ps_point& z_bck = top_.z_bck;
-    ps_point& z_sample = top_.z_sample;
-    ps_point& z_propose = top_.z_propose;
-    z_fwd.assign(z_);
-    z_bck.assign(z_fwd);
-    z_sample.assign(z_fwd);
-    z_propose.assign(z_fwd);
-
-    vec_t& p_fwd_fwd = top_.p_fwd_fwd;
-    vec_t& p_sharp_fwd_fwd = top_.p_sharp_fwd_fwd;
-    vec_t& p_fwd_bck = top_.p_fwd_bck;
-    vec_t& p_sharp_fwd_bck = top_.p_sharp_fwd_bck;
-    vec_t& p_bck_fwd = top_.p_bck_fwd;
-    vec_t& p_sharp_bck_fwd = top_.p_sharp_bck_fwd;
-    vec_t& p_bck_bck = top_.p_bck_bck;
-    vec_t& p_sharp_bck_bck = top_.p_sharp_bck_bck;
-    vec_t& rho = top_.rho;
-    vec_t& rho_fwd = top_.rho_fwd;
-    vec_t& rho_bck = top_.rho_bck;
-    vec_t& rho_ext = top_.rho_ext;
-
-    p_fwd_fwd = z_.p;
-    hamiltonian_.dtau_dp(z_, p_sharp_fwd_fwd);
-    p_fwd_bck = z_.p;
-    p_sharp_fwd_bck = p_sharp_fwd_fwd;
-    p_bck_fwd = z_.p;
-    p_sharp_bck_fwd = p_sharp_fwd_fwd;
-    p_bck_bck = z_.p;
-    p_sharp_bck_bck = p_sharp_fwd_fwd;
-    rho = z_.p;
-
-    double log_sum_weight = 0;
-    double H0 = hamiltonian_.H(z_);
+    const double H0 = hamiltonian_.H(z_);
+    double log_w = 0;  // log of the trajectory's total weight
     int n_leapfrog = 0;
-    double sum_metro_prob = 0;
+    double sum_metro = 0;
     depth_ = 0;
     divergent_ = false;
-
     while (depth_ < max_depth_) {
-      std::fill(rho_fwd.begin(), rho_fwd.end(), 0.0);
-      std::fill(rho_bck.begin(), rho_bck.end(), 0.0);
-      bool valid_subtree = false;
-      double log_sum_weight_subtree = -std::numeric_limits<double>::infinity();
-      if (rng_.uniform() > 0.5) {
-        z_.assign(z_fwd);
-        rho_bck = rho;
-        p_bck_fwd = p_fwd_fwd;
-        p_sharp_bck_fwd = p_sharp_fwd_fwd;
-        valid_subtree = build_tree(depth_, z_propose, p_sharp_fwd_bck,
-                                   p_sharp_fwd_fwd, rho_fwd, p_fwd_bck,
-                                   p_fwd_fwd, H0, 1, n_leapfrog,
-                                   log_sum_weight_subtree, sum_metro_prob,
-                                   logger);
-        z_fwd.assign(z_);
-      } else {
-        z_.assign(z_bck);
-        rho_fwd = rho;
-        p_fwd_bck = p_bck_bck;
-        p_sharp_fwd_bck = p_sharp_bck_bck;
-        valid_subtree = build_tree(depth_, z_propose, p_sharp_bck_fwd,
-                                   p_sharp_bck_bck, rho_bck, p_bck_fwd,
-                                   p_bck_bck, H0, -1, n_leapfrog,
-                                   log_sum_weight_subtree, sum_metro_prob,
-                                   logger);
-        z_bck.assign(z_);
-      }
-      if (!valid_subtree) break;
+      const int s = rng_.uniform() > 0.5 ? 1 : 0;
+      side& grow = t.sides[s];
+      side& other = t.sides[1 - s];
+      // the trajectory so far becomes the other side's subtree
+      other.rho = t.rho;
+      other.p_in = grow.p_out;
+      other.ps_in = grow.ps_out;
+      std::fill(grow.rho.begin(), grow.rho.end(), 0.0);
+      z_.assign(grow.z);
+      double log_w_new = -std::numeric_limits<double>::infinity();
+      const bool valid =
+          build_tree(depth_, t.propose, grow.ps_in, grow.ps_out, grow.rho, grow.p_in,
+                     grow.p_out, H0, s ? 1.0 : -1.0, n_leapfrog, log_w_new, sum_metro, logger);
+      grow.z.assign(z_);
+      if (!valid) break;
       ++depth_;
-      if (log_sum_weight_subtree > log_sum_weight) {
-        z_sample.assign(z_propose);
-      } else {
-        const double accept_prob = std::exp(log_sum_weight_subtree - log_sum_weight);
-        if (rng_.uniform() < accept_prob) z_sample.assign(z_propose);
-      }
-      log_sum_weight = log_sum_exp(log_sum_weight, log_sum_weight_subtree);
-      for (int i = 0; i < D; ++i) rho[i] = rho_bck[i] + rho_fwd[i];
-      bool persist = compute_criterion(p_sharp_bck_bck, p_sharp_fwd_fwd, rho);
-      for (int i = 0; i < D; ++i) rho_ext[i] = rho_bck[i] + p_fwd_bck[i];
-      persist &= compute_criterion(p_sharp_bck_bck, p_sharp_fwd_bck, rho_ext);
-      for (int i = 0; i < D; ++i) rho_ext[i] = rho_fwd[i] + p_bck_fwd[i];
-      persist &= compute_criterion(p_sharp_bck_fwd, p_sharp_fwd_fwd, rho_ext);
-      if (!persist) break;
+      if (log_w_new > log_w || rng_.uniform() < std::exp(log_w_new - log_w))
+        t.sample.assign(t.propose);
+      log_w = log_sum_exp(log_w, log_w_new);
+      if (!t.no_u_turn()) break;
     }
     n_leapfrog_ = n_leapfrog;
-    const double accept_prob = sum_metro_prob / static_cast<double>(n_leapfrog);
-    z_.assign(z_sample);
+    z_.assign(t.sample);
     energy_ = hamiltonian_.H(z_);
-    return sample(z_.q, -z_.V, accept_prob);
+    return sample(z_.q, -z_.V, sum_metro / static_cast<double>(n_leapfrog));
   }
 
   // sampler parameters of a draw (base_nuts.hpp:206-220)
@@ -362,20 +327,36 @@ class base_nuts {
     ps_point z_propose_final;
     vec_t p_final_beg, p_sharp_final_beg, rho_final, rho_subtree;
   };
-  struct top_state {
-    explicit top_state(int D)
-        : z_fwd(D), z_bck(D), z_sample(D), z_propose(D), p_fwd_fwd(D),
-          p_sharp_fwd_fwd(D), p_fwd_bck(D), p_sharp_fwd_bck(D), p_bck_fwd(D),
-          p_sharp_bck_fwd(D), p_bck_bck(D), p_sharp_bck_bck(D), rho(D),
-          rho_fwd(D), rho_bck(D), rho_ext(D) {}
-    ps_point z_fwd, z_bck, z_sample, z_propose;
-    vec_t p_fwd_fwd, p_sharp_fwd_fwd, p_fwd_bck, p_sharp_fwd_bck, p_bck_fwd,
-        p_sharp_bck_fwd, p_bck_bck, p_sharp_bck_bck, rho, rho_fwd, rho_bck,
-        rho_ext;
+  // One end of the trajectory: the outermost point, the momentum and its
+  // metric image (p-sharp) there and at the inner end of the subtree last
+  // added on this side, and that subtree's momentum sum.
+  struct side {
+    explicit side(int D) : z(D), p_out(D), ps_out(D), p_in(D), ps_in(D), rho(D) {}
+    ps_point z;
+    vec_t p_out, ps_out, p_in, ps_in, rho;
+  };
+  struct trajectory {
+    explicit trajectory(int D) : sides{side(D), side(D)}, sample(D), propose(D), rho(D), tmp(D) {}
+    side sides[2];         // 0: backward, 1: forward
+    ps_point sample, propose;
+    vec_t rho, tmp;        // momentum sum of the whole trajectory; scratch
+    // No U-turn across the whole trajectory (outer ends) nor across either
+    // joint (each side's subtree extended by the other side's inner point).
+    bool no_u_turn() {
+      const side& b = sides[0];
+      const side& f = sides[1];
+      for (size_t i = 0; i < rho.size(); ++i) rho[i] = b.rho[i] + f.rho[i];
+      bool ok = compute_criterion(b.ps_out, f.ps_out, rho);
+      for (size_t i = 0; i < rho.size(); ++i) tmp[i] = b.rho[i] + f.p_in[i];
+      ok &= compute_criterion(b.ps_out, f.ps_in, tmp);
+      for (size_t i = 0; i < rho.size(); ++i) tmp[i] = f.rho[i] + b.p_in[i];
+      ok &= compute_criterion(b.ps_in, f.ps_out, tmp);
+      return ok;
+    }
   };
 
   void ensure_frames(int D) {
-    if (top_.rho.size() != static_cast<size_t>(D)) top_ = top_state(D);
+    if (traj_.rho.size() != static_cast<size_t>(D)) traj_ = trajectory(D);
     const size_t want = static_cast<size_t>(max_depth_ + 1);
     if (frames_.empty() || frames_[0].rho_init.size() != static_cast<size_t>(D))
       frames_.assign(want, frame(D));
@@ -454,7 +435,7 @@ class base_nuts {
   double energy_;
   std::vector<frame> frames_;
   std::vector<dest> dests_;
-  top_state top_{0};
+  trajectory traj_{0};
 };
 
 }  // namespace mcmc

@@ -37,10 +37,10 @@ int main() {
   // ---- covariance adaptation: 10 samples of q = 0, window of 10 ----
   {
     const int n = 10;
-    stanx::mcmc::covar_adaptation adapter(n);
+    stanx::mcmc::covar_adaptation adapter(n, "covariance");
     adapter.set_window_params(50, 0, 0, 10, logger);
     vec_t covar(n * n, 0.0), q(n, 0.0);
-    for (int i = 0; i < 10; ++i) adapter.learn_covariance(covar, q);
+    for (int i = 0; i < 10; ++i) adapter.observe(covar, q);
     double worst = 0.0;
     const double target = 1e-3 * 5.0 / (10 + 5.0);
     for (int i = 0; i < n; ++i)

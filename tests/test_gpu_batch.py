@@ -52,6 +52,34 @@ def test_batch_chains_follow_host_service(family):
     m.close()
 
 
+@pytest.mark.parametrize("num_warmup", [100, 19, 300])
+def test_batch_metric_windows_follow_host_plan(num_warmup):
+    """The device chains close their metric windows where the host sampler
+    does.  num_warmup = 100 takes the reference's reduced 15/75/10 split, in
+    which no window ever closes (the metric stays at its initial value);
+    19 estimates nothing; 300 closes windows at 99, 149 and 249."""
+    _say(f"metric windows num_warmup={num_warmup}")
+    d = W.logistic(2000, 5, 0.4, 0.5, seed=15)
+    m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"])
+    cfg = dict(random_seed=515, chain=1, num_warmup=num_warmup, num_samples=20, max_depth=8)
+    res = S.hmc_nuts_diag_e_adapt_batch(m, 2, **cfg)
+    assert np.all(res["status"] == 0), res["status"]
+    for c in range(2):
+        host = S.hmc_nuts_diag_e_adapt(m, **{**cfg, "chain": 1 + c})
+        assert host["rc"] == 0
+        if num_warmup < 150:
+            assert np.all(host["inv_metric"] == 1.0)
+            assert np.all(res["inv_metric"][c] == 1.0)
+        else:
+            # both estimated the metric; last-bit differences of the device
+            # reductions can move a chain's path, so the estimates of the
+            # same posterior variances agree only statistically
+            assert not np.any(host["inv_metric"] == 1.0)
+            assert not np.any(res["inv_metric"][c] == 1.0)
+            np.testing.assert_allclose(res["inv_metric"][c], host["inv_metric"], rtol=0.5)
+    m.close()
+
+
 def test_batch_posterior_matches_oracle_sampler():
     d = W.logistic(2000, 5, 1.0 / np.sqrt(5), 0.5, seed=77)
     m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"])
