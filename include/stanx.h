@@ -269,6 +269,35 @@ int sx_hmc_nuts_diag_e_adapt_cb_csv(sx_lpgrad_fn f, void* ctx, int D,
                                     const char* csv_path, int sig_figs,
                                     sx_nuts_result* out);
 
+/* The services with all three of the reference's file outputs
+ * (src/stan/services/sample/hmc_nuts_diag_e_adapt.hpp:57-112: sample_writer,
+ * diagnostic_writer, metric_writer; hmc_nuts_diag_e.hpp:49-91 has no
+ * metric_writer):
+ *   sample_csv      the Stan CSV of the _csv entry points;
+ *   diagnostic_csv  per saved draw lp__ .. energy__, then the unconstrained
+ *                   position, momentum (p_*) and potential gradient (g_*)
+ *                   of the selected point (mcmc_writer.hpp:157-190,
+ *                   ps_point.hpp:25-44), after the same configuration block;
+ *   metric_json     one JSON record {"stepsize", "metric_type", "inv_metric"}
+ *                   written when warm-up ends (base_hmc.hpp:70-76).
+ * Any path may be NULL (that output is not written).                     */
+typedef struct sx_output_files {
+  const char* sample_csv;
+  const char* diagnostic_csv;
+  const char* metric_json;
+  int sig_figs; /* > 0: significant digits of the CSV values */
+} sx_output_files;
+int sx_hmc_nuts_diag_e_adapt_files(sx_glm* m, const sx_nuts_config* cfg,
+                                   const double* init, const double* init_inv_metric,
+                                   const sx_output_files* files, sx_nuts_result* out);
+int sx_hmc_nuts_diag_e_adapt_cb_files(sx_lpgrad_fn f, void* ctx, int D,
+                                      const sx_nuts_config* cfg, const double* init,
+                                      const double* init_inv_metric,
+                                      const sx_output_files* files, sx_nuts_result* out);
+int sx_hmc_nuts_diag_e_files(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                             const double* init_inv_metric, const sx_output_files* files,
+                             sx_nuts_result* out);
+
 /* hmc_nuts_diag_e (src/stan/services/sample/hmc_nuts_diag_e.hpp:49-91)
  * through run_sampler (src/stan/services/util/run_sampler.hpp:39-85): the
  * step size cfg->stepsize (with jitter) and the metric init_inv_metric (or

@@ -31,6 +31,8 @@ EXPORTED = [
     "sx_nuts_batch_rows_cap", "sx_nuts_batch_results", "sx_nuts_batch_destroy",
     "sx_hmc_nuts_dense_e_adapt", "sx_hmc_nuts_dense_e_adapt_cb", "sx_hmc_nuts_dense_e_adapt_csv",
     "sx_hmc_nuts_dense_e", "sx_hmc_nuts_dense_e_cb", "sx_hmc_nuts_dense_e_adapt_cb_csv",
+    "sx_hmc_nuts_diag_e_adapt_files", "sx_hmc_nuts_diag_e_adapt_cb_files",
+    "sx_hmc_nuts_diag_e_files",
 ]
 
 _lib = None
@@ -66,6 +68,11 @@ class NutsConfig(C.Structure):
                 ("max_depth", C.c_int32), ("delta", C.c_double), ("gamma", C.c_double),
                 ("kappa", C.c_double), ("t0", C.c_double), ("init_buffer", C.c_uint32),
                 ("term_buffer", C.c_uint32), ("window", C.c_uint32)]
+
+
+class OutputFiles(C.Structure):
+    _fields_ = [("sample_csv", C.c_char_p), ("diagnostic_csv", C.c_char_p),
+                ("metric_json", C.c_char_p), ("sig_figs", C.c_int)]
 
 
 class JobSpec(C.Structure):
@@ -133,6 +140,13 @@ def lib():
         L.sx_hmc_nuts_diag_e_adapt_cb_csv.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
                                                       vp, vp, C.c_char_p, C.c_int,
                                                       C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_adapt_files.argtypes = [vp, C.POINTER(NutsConfig), vp, vp,
+                                                     C.POINTER(OutputFiles),
+                                                     C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_adapt_cb_files.argtypes = [vp, vp, C.c_int, C.POINTER(NutsConfig),
+                                                        vp, vp, C.POINTER(OutputFiles),
+                                                        C.POINTER(NutsResult)]
+        L.sx_hmc_nuts_diag_e_files.argtypes = L.sx_hmc_nuts_diag_e_adapt_files.argtypes
         L.sx_hmc_nuts_dense_e_adapt.argtypes = L.sx_hmc_nuts_diag_e_adapt.argtypes
         L.sx_hmc_nuts_dense_e_adapt_cb.argtypes = L.sx_hmc_nuts_diag_e_adapt_cb.argtypes
         L.sx_hmc_nuts_dense_e_adapt_csv.argtypes = L.sx_hmc_nuts_diag_e_adapt_csv.argtypes
@@ -350,6 +364,49 @@ def hmc_nuts_diag_e_adapt_cb_csv(fn_ptr, ctx, D, path, sig_figs=-1, init=None,
                                                               str(path).encode(),
                                                               int(sig_figs), C.byref(r)),
         D, cfg, init, inv_metric)
+
+
+def _files(sample_csv, diagnostic_csv, metric_json, sig_figs):
+    enc = lambda p: None if p is None else str(p).encode()  # noqa: E731
+    return OutputFiles(enc(sample_csv), enc(diagnostic_csv), enc(metric_json), int(sig_figs))
+
+
+def hmc_nuts_diag_e_adapt_files(model, sample_csv=None, diagnostic_csv=None, metric_json=None,
+                                sig_figs=-1, init=None, inv_metric=None, **kw):
+    """hmc_nuts_diag_e_adapt with the reference's three outputs as files: the
+    sample CSV, the diagnostic CSV (q, p_*, g_* per saved draw) and the
+    metric JSON (adapted stepsize and inverse metric).  Paths may be None."""
+    cfg = nuts_config(**kw)
+    files = _files(sample_csv, diagnostic_csv, metric_json, sig_figs)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_files(model.handle, C.byref(cfg), _ptr(i),
+                                                             _ptr(m), C.byref(files),
+                                                             C.byref(r)),
+        model.D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e_adapt_cb_files(fn_ptr, ctx, D, sample_csv=None, diagnostic_csv=None,
+                                   metric_json=None, sig_figs=-1, init=None, inv_metric=None,
+                                   **kw):
+    """hmc_nuts_diag_e_adapt_files for a model given as a C callback."""
+    cfg = nuts_config(**kw)
+    files = _files(sample_csv, diagnostic_csv, metric_json, sig_figs)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_adapt_cb_files(fn_ptr, ctx, D, C.byref(cfg),
+                                                                _ptr(i), _ptr(m),
+                                                                C.byref(files), C.byref(r)),
+        D, cfg, init, inv_metric)
+
+
+def hmc_nuts_diag_e_files(model, sample_csv=None, diagnostic_csv=None, sig_figs=-1, init=None,
+                          inv_metric=None, **kw):
+    """hmc_nuts_diag_e (no adaptation) with sample and diagnostic CSV files."""
+    cfg = nuts_config(**kw)
+    files = _files(sample_csv, diagnostic_csv, None, sig_figs)
+    return _run_service(
+        lambda i, m, r: lib().sx_hmc_nuts_diag_e_files(model.handle, C.byref(cfg), _ptr(i),
+                                                       _ptr(m), C.byref(files), C.byref(r)),
+        model.D, cfg, init, inv_metric)
 
 
 def hmc_nuts_diag_e(model, init=None, inv_metric=None, **kw):

@@ -65,7 +65,8 @@ void write_csv_config(std::ostream& o, const std::string& model_name, const sx_n
 template <class Model>
 int run_service(const Model& model, const sx_nuts_config* cfg, const double* init,
                 const double* init_inv_metric, sx_nuts_result* out, bool adapt = true,
-                const char* csv_path = nullptr, int sig_figs = -1, bool dense = false) {
+                const char* csv_path = nullptr, int sig_figs = -1, bool dense = false,
+                const char* diagnostic_path = nullptr, const char* metric_path = nullptr) {
   const char* name = dense ? (adapt ? "sx_hmc_nuts_dense_e_adapt" : "sx_hmc_nuts_dense_e")
                            : (adapt ? "sx_hmc_nuts_diag_e_adapt" : "sx_hmc_nuts_diag_e");
   if (!cfg || !out || !out->draws) {
@@ -79,22 +80,43 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
   if (init_inv_metric) minv_v.assign(init_inv_metric, init_inv_metric + M);
   stanx::callbacks::interrupt interrupt;
   stanx::callbacks::counting_logger logger;
-  stanx::callbacks::writer init_writer, diag_writer;
+  stanx::callbacks::writer init_writer;
   stanx::callbacks::table_writer sample_writer;
-  std::ofstream csv;
-  if (csv_path) {
-    csv.open(csv_path);
-    if (!csv) {
-      sx::set_error(std::string("cannot open ") + csv_path);
-      return SX_CODE_CONFIG;
+  std::ofstream csv, diag, metric;
+  auto open = [&](std::ofstream& f, const char* path) {
+    f.open(path);
+    if (!f) {
+      sx::set_error(std::string("cannot open ") + path);
+      return false;
     }
-    if (sig_figs > 0) csv << std::setprecision(sig_figs);
+    if (sig_figs > 0) f << std::setprecision(sig_figs);
+    return true;
+  };
+  if (csv_path) {
+    if (!open(csv, csv_path)) return SX_CODE_CONFIG;
     write_csv_config(csv, model.model_name(), cfg, adapt, dense, sig_figs);
   }
-  stanx::callbacks::stream_writer csv_writer(csv, "# ");
+  if (diagnostic_path) {
+    if (!open(diag, diagnostic_path)) return SX_CODE_CONFIG;
+    write_csv_config(diag, model.model_name(), cfg, adapt, dense, sig_figs);
+  }
+  if (metric_path) {
+    metric.open(metric_path);
+    if (!metric) {
+      sx::set_error(std::string("cannot open ") + metric_path);
+      return SX_CODE_CONFIG;
+    }
+  }
+  stanx::callbacks::stream_writer csv_writer(csv, "# "), diag_file_writer(diag, "# ");
   stanx::callbacks::writer null_writer;
   tee_writer out_writer(sample_writer, csv_path ? static_cast<stanx::callbacks::writer&>(csv_writer)
                                                 : null_writer);
+  stanx::callbacks::writer& diag_writer =
+      diagnostic_path ? static_cast<stanx::callbacks::writer&>(diag_file_writer) : null_writer;
+  stanx::callbacks::json_writer metric_json(metric);
+  stanx::callbacks::structured_writer null_struct;
+  stanx::callbacks::structured_writer& metric_writer =
+      metric_path ? static_cast<stanx::callbacks::structured_writer&>(metric_json) : null_struct;
   double eps = 0.0, warm = 0.0, samp = 0.0;
   std::vector<double> minv_out;
   long long n_grad = 0;
@@ -110,14 +132,14 @@ int run_service(const Model& model, const sx_nuts_config* cfg, const double* ini
           cfg->num_samples, thin, cfg->save_warmup != 0, cfg->refresh, cfg->stepsize,
           cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma, cfg->kappa, cfg->t0,
           cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt, logger, init_writer,
-          out_writer, diag_writer, &eps, &minv_out, &n_grad, &warm, &samp);
+          out_writer, diag_writer, metric_writer, &eps, &minv_out, &n_grad, &warm, &samp);
     } else if (adapt) {
       rc = smp::hmc_nuts_dense_e_adapt(
           model, iv, mv, cfg->random_seed, cfg->chain, cfg->init_radius, cfg->num_warmup,
           cfg->num_samples, thin, cfg->save_warmup != 0, cfg->refresh, cfg->stepsize,
           cfg->stepsize_jitter, cfg->max_depth, cfg->delta, cfg->gamma, cfg->kappa, cfg->t0,
           cfg->init_buffer, cfg->term_buffer, cfg->window, interrupt, logger, init_writer,
-          out_writer, diag_writer, &eps, &minv_out, &n_grad, &warm, &samp);
+          out_writer, diag_writer, metric_writer, &eps, &minv_out, &n_grad, &warm, &samp);
     } else {
       rc = dense ? smp::hmc_nuts_dense_e(model, iv, mv, cfg->random_seed, cfg->chain,
                                          cfg->init_radius, cfg->num_warmup, cfg->num_samples,
@@ -242,6 +264,54 @@ int sx_hmc_nuts_diag_e_cb(sx_lpgrad_fn f, void* ctx, int D, const sx_nuts_config
   }
   stanx::model::callback_model model(f, ctx, D);
   return run_service(model, cfg, init, init_inv_metric, out, false);
+}
+
+// ---- file outputs: sample CSV, diagnostic CSV, metric JSON ----
+int sx_hmc_nuts_diag_e_adapt_files(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                                   const double* init_inv_metric, const sx_output_files* files,
+                                   sx_nuts_result* out) {
+  if (!m || !files) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt_files: NULL model or files");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, true, files->sample_csv,
+                       files->sig_figs, false, files->diagnostic_csv, files->metric_json);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
+}
+
+int sx_hmc_nuts_diag_e_adapt_cb_files(sx_lpgrad_fn f, void* ctx, int D,
+                                      const sx_nuts_config* cfg, const double* init,
+                                      const double* init_inv_metric,
+                                      const sx_output_files* files, sx_nuts_result* out) {
+  if (!f || D < 0 || !files) {
+    sx::set_error("sx_hmc_nuts_diag_e_adapt_cb_files: invalid model or files");
+    return SX_CODE_CONFIG;
+  }
+  stanx::model::callback_model model(f, ctx, D);
+  return run_service(model, cfg, init, init_inv_metric, out, true, files->sample_csv,
+                     files->sig_figs, false, files->diagnostic_csv, files->metric_json);
+}
+
+int sx_hmc_nuts_diag_e_files(sx_glm* m, const sx_nuts_config* cfg, const double* init,
+                             const double* init_inv_metric, const sx_output_files* files,
+                             sx_nuts_result* out) {
+  if (!m || !files) {
+    sx::set_error("sx_hmc_nuts_diag_e_files: NULL model or files");
+    return SX_CODE_CONFIG;
+  }
+  try {
+    stanx::model::gpu_glm_model model(m);
+    return run_service(model, cfg, init, init_inv_metric, out, false, files->sample_csv,
+                       files->sig_figs, false, files->diagnostic_csv, nullptr);
+  } catch (const std::exception& e) {
+    sx::set_error(e.what());
+    return SX_CODE_SOFTWARE;
+  }
 }
 
 // ---- dense_e metric ----

@@ -1,9 +1,12 @@
 // Output and control callbacks of the sampler services, mirroring the
 // reference's interfaces (src/stan/callbacks/writer.hpp:16-61,
-// logger.hpp:24-97, interrupt.hpp:14-28, stream_writer.hpp:14-93).
+// logger.hpp:24-97, interrupt.hpp:14-28, stream_writer.hpp:14-93,
+// structured_writer.hpp:16-170).
 #ifndef STANX_CALLBACKS_HPP
 #define STANX_CALLBACKS_HPP
 
+#include <cmath>
+#include <cstdio>
 #include <ostream>
 #include <sstream>
 #include <string>
@@ -19,6 +22,91 @@ class writer {
   virtual void operator()(const std::vector<double>& state) {}
   virtual void operator()() {}
   virtual void operator()(const std::string& message) {}
+};
+
+// Key-value records (the metric file: step size, metric type, inverse
+// metric).  The base class ignores everything.
+class structured_writer {
+ public:
+  virtual ~structured_writer() {}
+  virtual void begin_record() {}
+  virtual void end_record() {}
+  virtual void write(const std::string& key, const std::string& value) {}
+  virtual void write(const std::string& key, double value) {}
+  virtual void write(const std::string& key, const std::vector<double>& values) {}
+  // a row-major rows x cols matrix
+  virtual void write(const std::string& key, const std::vector<double>& values, size_t rows,
+                     size_t cols) {}
+};
+
+// Writes each record as one JSON object: {"key": value, ...}.  Numbers
+// carry 17 significant digits (a double survives the round trip);
+// non-finite values are written as the strings "NaN", "Inf", "-Inf".
+class json_writer : public structured_writer {
+ public:
+  explicit json_writer(std::ostream& out) : out_(out) {}
+  void begin_record() override {
+    out_ << "{";
+    first_ = true;
+  }
+  void end_record() override { out_ << "\n}\n"; }
+  void write(const std::string& key, const std::string& value) override {
+    this->key(key);
+    out_ << '"' << escape(value) << '"';
+  }
+  void write(const std::string& key, double value) override {
+    this->key(key);
+    number(value);
+  }
+  void write(const std::string& key, const std::vector<double>& values) override {
+    this->key(key);
+    row(values.data(), values.size());
+  }
+  void write(const std::string& key, const std::vector<double>& values, size_t rows,
+             size_t cols) override {
+    this->key(key);
+    out_ << "[";
+    for (size_t r = 0; r < rows; ++r) {
+      if (r) out_ << ", ";
+      row(values.data() + r * cols, cols);
+    }
+    out_ << "]";
+  }
+
+ private:
+  void key(const std::string& k) {
+    out_ << (first_ ? "\n  " : ",\n  ") << '"' << escape(k) << "\": ";
+    first_ = false;
+  }
+  void row(const double* v, size_t n) {
+    out_ << "[";
+    for (size_t i = 0; i < n; ++i) {
+      if (i) out_ << ", ";
+      number(v[i]);
+    }
+    out_ << "]";
+  }
+  void number(double v) {
+    if (std::isnan(v)) {
+      out_ << "\"NaN\"";
+    } else if (std::isinf(v)) {
+      out_ << (v > 0 ? "\"Inf\"" : "\"-Inf\"");
+    } else {
+      char buf[32];
+      std::snprintf(buf, sizeof buf, "%.17g", v);
+      out_ << buf;
+    }
+  }
+  static std::string escape(const std::string& s) {
+    std::string o;
+    for (char c : s) {
+      if (c == '"' || c == '\\') o += '\\';
+      o += c;
+    }
+    return o;
+  }
+  std::ostream& out_;
+  bool first_ = true;
 };
 
 class logger {

@@ -247,6 +247,29 @@ class base_nuts {
     values.push_back(this->energy_);
   }
 
+  // diagnostic columns: position, momentum and potential gradient of the
+  // current point, named after the model's unconstrained parameters with
+  // p_ and g_ prefixes (ps_point.hpp:25-44)
+  void get_sampler_diagnostic_names(const std::vector<std::string>& model_names,
+                                    std::vector<std::string>& names) const {
+    for (const auto& n : model_names) names.push_back(n);
+    for (const auto& n : model_names) names.push_back("p_" + n);
+    for (const auto& n : model_names) names.push_back("g_" + n);
+  }
+  void get_sampler_diagnostics(std::vector<double>& values) const {
+    values.insert(values.end(), z_.q.begin(), z_.q.end());
+    values.insert(values.end(), z_.p.begin(), z_.p.end());
+    values.insert(values.end(), z_.g.begin(), z_.g.end());
+  }
+  // the adapted tuning parameters as one record (base_hmc.hpp:70-76)
+  void write_sampler_state_struct(callbacks::structured_writer& w) const {
+    w.begin_record();
+    w.write("stepsize", nom_epsilon_);
+    w.write("metric_type", std::string(point_t::metric_type()));
+    z_.write_inv_metric(w);
+    w.end_record();
+  }
+
   static bool compute_criterion(const vec_t& p_sharp_minus,
                                 const vec_t& p_sharp_plus, const vec_t& rho) {
     return dot(p_sharp_plus, rho) > 0 && dot(p_sharp_minus, rho) > 0;

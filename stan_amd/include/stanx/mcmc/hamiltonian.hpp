@@ -45,6 +45,8 @@ class diag_e_point : public ps_point {
   explicit diag_e_point(int n) : ps_point(n), inv_e_metric_(n, 1.0) {}
   vec_t inv_e_metric_;
   void set_metric(const vec_t& m) { inv_e_metric_ = m; }
+  static const char* metric_type() { return "diag_e"; }
+  void write_inv_metric(callbacks::structured_writer& w) const { w.write("inv_metric", inv_e_metric_); }
   // diag_e_point::write_metric (diag_e_point.hpp:44-54)
   void write_metric(callbacks::writer& writer) const {
     writer("Diagonal elements of inverse mass matrix:");
@@ -84,6 +86,10 @@ class dense_e_point : public ps_point {
   }
   vec_t inv_e_metric_;
   vec_t factor_;  // lower Cholesky factor of inv_e_metric_
+  static const char* metric_type() { return "dense_e"; }
+  void write_inv_metric(callbacks::structured_writer& w) const {
+    w.write("inv_metric", inv_e_metric_, q.size(), q.size());
+  }
   void set_metric(const vec_t& m) {
     inv_e_metric_ = m;
     factorize();

@@ -8,10 +8,14 @@
 //   log_prob_grad        src/stan/model/log_prob_grad.hpp:28-50
 //   log_prob_propto      src/stan/model/log_prob_propto.hpp:32-52
 //   gradient             src/stan/model/gradient.hpp:14-35
-// The reference's reverse-mode (var) overloads need Stan Math, which is not
-// part of this library: a model here supplies its value and gradient
-// directly through log_prob_grad_impl, which is what the GPU-backed model
-// does with one fused kernel instead of an AD tape.
+//   model_base_crtp      src/stan/model/model_base_crtp.hpp:73-301
+// A model supplies its value and gradient through log_prob_grad_impl (the
+// GPU-backed model does so with one fused kernel instead of an AD tape).
+// The reverse-mode overloads of the reference take this library's own var
+// (stanx/var.hpp): model_base answers them with one precomputed-gradient
+// entry on the caller's tape, and model_base_crtp runs a model written in
+// the generated style (a log_prob template over double and var) through
+// that tape.
 #ifndef STANX_MODEL_HPP
 #define STANX_MODEL_HPP
 
@@ -25,6 +29,7 @@
 #include "callbacks.hpp"
 #include "io.hpp"
 #include "philox.hpp"
+#include "var.hpp"
 
 namespace stanx {
 namespace model {
@@ -70,31 +75,52 @@ class model_base : public prob_grad {
                                     bool propto, bool jacobian,
                                     std::ostream* msgs) const = 0;
 
-  // The four double overloads of the reference (:404-450).
-  double log_prob(std::vector<double>& params_r, std::vector<int>& params_i,
-                  std::ostream* msgs) const {
+  // The four double overloads of the reference (:404-458).
+  virtual double log_prob(std::vector<double>& params_r, std::vector<int>& params_i,
+                          std::ostream* msgs) const {
     return log_prob_grad_impl(params_r, nullptr, false, false, msgs);
   }
-  double log_prob_jacobian(std::vector<double>& params_r,
-                           std::vector<int>& params_i,
-                           std::ostream* msgs) const {
+  virtual double log_prob_jacobian(std::vector<double>& params_r,
+                                   std::vector<int>& params_i, std::ostream* msgs) const {
     return log_prob_grad_impl(params_r, nullptr, false, true, msgs);
   }
-  double log_prob_propto(std::vector<double>& params_r,
-                         std::vector<int>& params_i,
-                         std::ostream* msgs) const {
+  virtual double log_prob_propto(std::vector<double>& params_r,
+                                 std::vector<int>& params_i, std::ostream* msgs) const {
     return log_prob_grad_impl(params_r, nullptr, true, false, msgs);
   }
-  double log_prob_propto_jacobian(std::vector<double>& params_r,
-                                  std::vector<int>& params_i,
-                                  std::ostream* msgs) const {
+  virtual double log_prob_propto_jacobian(std::vector<double>& params_r,
+                                          std::vector<int>& params_i,
+                                          std::ostream* msgs) const {
     return log_prob_grad_impl(params_r, nullptr, true, true, msgs);
   }
-  // log_prob<propto, jacobian> dispatch as in model_base_crtp
-  template <bool propto, bool jacobian>
-  double log_prob(std::vector<double>& params_r, std::vector<int>& params_i,
-                  std::ostream* msgs = nullptr) const {
-    return log_prob_grad_impl(params_r, nullptr, propto, jacobian, msgs);
+  // The four reverse-mode overloads (:460-544): the log density as one
+  // tape entry whose edges to params_r carry the gradient.
+  virtual math::var log_prob(std::vector<math::var>& params_r, std::vector<int>& params_i,
+                             std::ostream* msgs) const {
+    return var_log_prob(params_r, false, false, msgs);
+  }
+  virtual math::var log_prob_jacobian(std::vector<math::var>& params_r,
+                                      std::vector<int>& params_i, std::ostream* msgs) const {
+    return var_log_prob(params_r, false, true, msgs);
+  }
+  virtual math::var log_prob_propto(std::vector<math::var>& params_r,
+                                    std::vector<int>& params_i, std::ostream* msgs) const {
+    return var_log_prob(params_r, true, false, msgs);
+  }
+  virtual math::var log_prob_propto_jacobian(std::vector<math::var>& params_r,
+                                             std::vector<int>& params_i,
+                                             std::ostream* msgs) const {
+    return var_log_prob(params_r, true, true, msgs);
+  }
+  // log_prob<propto, jacobian> dispatch to the overloads (model_base.hpp:306-340)
+  template <bool propto, bool jacobian, typename T>
+  T log_prob(std::vector<T>& params_r, std::vector<int>& params_i,
+             std::ostream* msgs = nullptr) const {
+    if (propto)
+      return jacobian ? log_prob_propto_jacobian(params_r, params_i, msgs)
+                      : log_prob_propto(params_r, params_i, msgs);
+    return jacobian ? log_prob_jacobian(params_r, params_i, msgs)
+                    : log_prob(params_r, params_i, msgs);
   }
 
   virtual void transform_inits(const io::var_context& context,
@@ -110,6 +136,89 @@ class model_base : public prob_grad {
   virtual void unconstrain_array(const std::vector<double>& params_constrained,
                                  std::vector<double>& params_unconstrained,
                                  std::ostream* msgs = nullptr) const = 0;
+
+ protected:
+  math::var var_log_prob(std::vector<math::var>& params_r, bool propto, bool jacobian,
+                         std::ostream* msgs) const {
+    std::vector<double> x(params_r.size()), g(params_r.size());
+    for (size_t i = 0; i < x.size(); ++i) x[i] = params_r[i].val();
+    const double lp = log_prob_grad_impl(x, &g, propto, jacobian, msgs);
+    return math::precomputed_gradients(lp, params_r, g);
+  }
+};
+
+// A model written in the generated style: M defines
+//   template <bool propto, bool jacobian, typename T>
+//   T log_prob(std::vector<T>& params_r, std::vector<int>& params_i,
+//              std::ostream* msgs) const;
+// for T = double and T = math::var, plus the metadata methods.  Every
+// virtual of model_base forwards to that template; the gradient comes from
+// the var instantiation on this thread's tape (log_prob_grad.hpp:28-50),
+// which is cleared afterwards, also when the model throws.
+template <class M>
+class model_base_crtp : public model_base {
+ public:
+  explicit model_base_crtp(size_t num_params_r) : model_base(num_params_r) {}
+
+  double log_prob(std::vector<double>& theta, std::vector<int>& theta_i,
+                  std::ostream* msgs) const override {
+    return self().template log_prob<false, false>(theta, theta_i, msgs);
+  }
+  math::var log_prob(std::vector<math::var>& theta, std::vector<int>& theta_i,
+                     std::ostream* msgs) const override {
+    return self().template log_prob<false, false>(theta, theta_i, msgs);
+  }
+  double log_prob_jacobian(std::vector<double>& theta, std::vector<int>& theta_i,
+                           std::ostream* msgs) const override {
+    return self().template log_prob<false, true>(theta, theta_i, msgs);
+  }
+  math::var log_prob_jacobian(std::vector<math::var>& theta, std::vector<int>& theta_i,
+                              std::ostream* msgs) const override {
+    return self().template log_prob<false, true>(theta, theta_i, msgs);
+  }
+  double log_prob_propto(std::vector<double>& theta, std::vector<int>& theta_i,
+                         std::ostream* msgs) const override {
+    return self().template log_prob<true, false>(theta, theta_i, msgs);
+  }
+  math::var log_prob_propto(std::vector<math::var>& theta, std::vector<int>& theta_i,
+                            std::ostream* msgs) const override {
+    return self().template log_prob<true, false>(theta, theta_i, msgs);
+  }
+  double log_prob_propto_jacobian(std::vector<double>& theta, std::vector<int>& theta_i,
+                                  std::ostream* msgs) const override {
+    return self().template log_prob<true, true>(theta, theta_i, msgs);
+  }
+  math::var log_prob_propto_jacobian(std::vector<math::var>& theta,
+                                     std::vector<int>& theta_i,
+                                     std::ostream* msgs) const override {
+    return self().template log_prob<true, true>(theta, theta_i, msgs);
+  }
+
+  double log_prob_grad_impl(const std::vector<double>& params_r, std::vector<double>* gradient,
+                            bool propto, bool jacobian, std::ostream* msgs) const override {
+    std::vector<int> params_i;
+    if (!gradient) {
+      std::vector<double> x(params_r);
+      return propto ? (jacobian ? self().template log_prob<true, true>(x, params_i, msgs)
+                                : self().template log_prob<true, false>(x, params_i, msgs))
+                    : (jacobian ? self().template log_prob<false, true>(x, params_i, msgs)
+                                : self().template log_prob<false, false>(x, params_i, msgs));
+    }
+    math::tape_scope scope;
+    std::vector<math::var> v(params_r.begin(), params_r.end());
+    const math::var lp =
+        propto ? (jacobian ? self().template log_prob<true, true>(v, params_i, msgs)
+                           : self().template log_prob<true, false>(v, params_i, msgs))
+               : (jacobian ? self().template log_prob<false, true>(v, params_i, msgs)
+                           : self().template log_prob<false, false>(v, params_i, msgs));
+    lp.grad();
+    gradient->resize(v.size());
+    for (size_t i = 0; i < v.size(); ++i) (*gradient)[i] = v[i].adj();
+    return lp.val();
+  }
+
+ private:
+  const M& self() const { return *static_cast<const M*>(this); }
 };
 
 // stan::model::log_prob_grad<propto, jacobian> (log_prob_grad.hpp:28-50):

@@ -146,6 +146,23 @@ class mcmc_writer {
     values.insert(values.end(), model_values.begin(), model_values.end());
     sample_writer_(values);
   }
+  // unconstrained parameters, then their momenta and gradients
+  // (mcmc_writer.hpp:157-190)
+  void write_diagnostic_names(Sampler& sampler, const Model& model) {
+    std::vector<std::string> names, model_names;
+    mcmc::sample::get_sample_param_names(names);
+    sampler.get_sampler_param_names(names);
+    model.unconstrained_param_names(model_names, false, false);
+    sampler.get_sampler_diagnostic_names(model_names, names);
+    diagnostic_writer_(names);
+  }
+  void write_diagnostic_params(const mcmc::sample& s, Sampler& sampler) {
+    std::vector<double> values;
+    s.get_sample_params(values);
+    sampler.get_sampler_params(values);
+    sampler.get_sampler_diagnostics(values);
+    diagnostic_writer_(values);
+  }
   void write_adapt_finish(Sampler& sampler) {
     sample_writer_("Adaptation terminated");
     std::stringstream ss;
@@ -197,7 +214,10 @@ void generate_transitions(Sampler& sampler, int num_iterations, int start, int f
       logger.info(message);
     }
     init_s = sampler.transition(init_s, logger);
-    if (save && ((m % num_thin) == 0)) writer.write_sample_params(base_rng, init_s, sampler, model);
+    if (save && ((m % num_thin) == 0)) {
+      writer.write_sample_params(base_rng, init_s, sampler, model);
+      writer.write_diagnostic_params(init_s, sampler);
+    }
   }
 }
 
@@ -207,7 +227,8 @@ void run_adaptive_sampler(Sampler& sampler, const Model& model,
                           int num_samples, int num_thin, int refresh, bool save_warmup,
                           rng_t& rng, callbacks::interrupt& interrupt,
                           callbacks::logger& logger, callbacks::writer& sample_writer,
-                          callbacks::writer& diagnostic_writer, double* warm_s = nullptr,
+                          callbacks::writer& diagnostic_writer,
+                          callbacks::structured_writer& metric_writer, double* warm_s = nullptr,
                           double* samp_s = nullptr, size_t chain_id = 1,
                           size_t num_chains = 1) {
   sampler.engage_adaptation();
@@ -222,6 +243,7 @@ void run_adaptive_sampler(Sampler& sampler, const Model& model,
   mcmc_writer<Sampler, Model> writer(sample_writer, diagnostic_writer, logger);
   mcmc::sample s(cont_vector, 0, 0);
   writer.write_sample_names(sampler, model);
+  writer.write_diagnostic_names(sampler, model);
   auto t0 = std::chrono::steady_clock::now();
   generate_transitions(sampler, num_warmup, 0, num_warmup + num_samples, num_thin,
                        refresh, save_warmup, true, writer, s, model, rng, interrupt,
@@ -229,6 +251,7 @@ void run_adaptive_sampler(Sampler& sampler, const Model& model,
   auto t1 = std::chrono::steady_clock::now();
   sampler.disengage_adaptation();
   writer.write_adapt_finish(sampler);
+  sampler.write_sampler_state_struct(metric_writer);
   auto t2 = std::chrono::steady_clock::now();
   generate_transitions(sampler, num_samples, num_warmup, num_warmup + num_samples,
                        num_thin, refresh, true, false, writer, s, model, rng, interrupt,
@@ -254,6 +277,7 @@ void run_sampler(Sampler& sampler, const Model& model, std::vector<double>& cont
   mcmc::sample s(cont_vector, 0, 0);
   sampler.z().q = cont_vector;
   writer.write_sample_names(sampler, model);
+  writer.write_diagnostic_names(sampler, model);
   auto t0 = std::chrono::steady_clock::now();
   generate_transitions(sampler, num_warmup, 0, num_warmup + num_samples, num_thin,
                        refresh, save_warmup, true, writer, s, model, rng, interrupt,
@@ -341,6 +365,7 @@ int hmc_nuts_diag_e_adapt(const Model& model, const std::vector<double>* init,
                           callbacks::logger& logger, callbacks::writer& init_writer,
                           callbacks::writer& sample_writer,
                           callbacks::writer& diagnostic_writer,
+                          callbacks::structured_writer& metric_writer,
                           double* stepsize_out = nullptr,
                           std::vector<double>* inv_metric_out = nullptr,
                           long long* n_grad_out = nullptr,
@@ -382,7 +407,8 @@ int hmc_nuts_diag_e_adapt(const Model& model, const std::vector<double>* init,
   try {
     util::run_adaptive_sampler(*sampler, model, cont_vector, num_warmup, num_samples,
                                num_thin, refresh, save_warmup, rng, interrupt, logger,
-                               sample_writer, diagnostic_writer, warm_s, samp_s);
+                               sample_writer, diagnostic_writer, metric_writer, warm_s,
+                               samp_s);
   } catch (const std::exception& e) {
     logger.error(e.what());
     rc = error_codes::SOFTWARE;
@@ -440,6 +466,7 @@ int hmc_nuts_dense_e_adapt(const Model& model, const std::vector<double>* init,
                            callbacks::logger& logger, callbacks::writer& init_writer,
                            callbacks::writer& sample_writer,
                            callbacks::writer& diagnostic_writer,
+                           callbacks::structured_writer& metric_writer,
                            double* stepsize_out = nullptr,
                            std::vector<double>* inv_metric_out = nullptr,
                            long long* n_grad_out = nullptr,
@@ -471,7 +498,8 @@ int hmc_nuts_dense_e_adapt(const Model& model, const std::vector<double>* init,
   try {
     util::run_adaptive_sampler(sampler, model, cont_vector, num_warmup, num_samples,
                                num_thin, refresh, save_warmup, rng, interrupt, logger,
-                               sample_writer, diagnostic_writer, warm_s, samp_s);
+                               sample_writer, diagnostic_writer, metric_writer, warm_s,
+                               samp_s);
   } catch (const std::exception& e) {
     logger.error(e.what());
     rc = error_codes::SOFTWARE;

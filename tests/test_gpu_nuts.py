@@ -55,6 +55,49 @@ def test_gpu_nuts_posterior_matches_oracle_sampler(family):
     m.close()
 
 
+def test_gpu_non_adaptive_service_matches_oracle(tmp_path):
+    """hmc_nuts_diag_e (hmc_nuts_diag_e.hpp:49-91) over the GPU model: a
+    fixed step size and metric (taken from an adaptive run), the draws'
+    moments against the oracle's non-adaptive sampler with the same tuning
+    (4 MCSE), and its diagnostic file: the position columns equal the draws
+    (identity write_array for the logistic model) and the g_ columns equal
+    minus the log-density gradient at them (the GPU gradient, 1e-12)."""
+    print("gpu non-adaptive service", flush=True)
+    d = W.logistic(2000, 5, 1.0 / np.sqrt(5), 0.5, seed=79)
+    m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"])
+    tune = S.hmc_nuts_diag_e_adapt(m, random_seed=5, chain=1, num_warmup=500, num_samples=10)
+    assert tune["rc"] == 0
+    eps, minv = tune["stepsize"], tune["inv_metric"]
+    ref = O.GlmData(S.LOGIT, d["X"], d["y"], prior=d["prior"])
+    f = O.model_fn("orc_glm_model_cb")
+    ctx = ref.ctx(mode=0)
+    gpu, cpu = [], []
+    for chain in range(1, 5):
+        diag = tmp_path / f"diag{chain}.csv" if chain == 1 else None
+        out = S.hmc_nuts_diag_e_files(m, diagnostic_csv=diag, sig_figs=18, random_seed=31,
+                                      chain=chain, num_warmup=100, num_samples=1000,
+                                      stepsize=eps, inv_metric=minv)
+        assert out["rc"] == 0, out["error"]
+        assert np.all(out["draws"][:, 2] == eps)  # no adaptation: stepsize__ fixed
+        gpu.append(out["draws"][:, 7:])
+        rc, dr, _ = O.hmc_nuts_diag_e(f, ctx, m.D, init_inv_metric=minv, seed=131, chain=chain,
+                                      num_warmup=100, num_samples=1000, stepsize=eps)
+        assert rc == 0
+        cpu.append(dr[:, 7:])
+        if diag is not None:
+            rows = np.array([[float(v) for v in ln.split(",")] for ln in open(diag)
+                             if ln[0] not in "#l" and ln.strip()])
+            D = m.D
+            assert rows.shape == (1000, 7 + 3 * D)
+            q, g = rows[:, 7:7 + D], rows[:, 7 + 2 * D:]
+            np.testing.assert_array_equal(q, out["draws"][:, 7:])
+            for i in range(0, 1000, 250):
+                _, gl = m.log_prob_grad(q[i])
+                np.testing.assert_allclose(-g[i], gl, rtol=1e-12, atol=1e-12)
+    _moments_agree(np.stack(gpu, 1), np.stack(cpu, 1), "non-adaptive")
+    m.close()
+
+
 def test_gpu_nuts_rejects_invalid_metric():
     d = W.logistic(100, 2, 0.5, 0.5, seed=1)
     m = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"])
