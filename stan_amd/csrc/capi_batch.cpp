@@ -76,6 +76,7 @@ void release(sx_nuts_batch* b) {
   (void)hipFree(b->a.draws);
   (void)hipFree(b->a.done_count);
   (void)hipFree(b->a.xch);
+  (void)hipFree(b->a.xfail_g);
   (void)hipFree(b->d_partials);
   (void)hipFree(b->d_out);
   (void)hipFree(b->d_theta);
@@ -191,6 +192,19 @@ int create_batch(sx_nuts_batch** out, sx_glm* m, const sx_nuts_config* cfg, int6
         return fail(SX_ENOMEM, "exchange buffers");
       if (hipMemsetAsync(b->a.xch, 0, sizeof(unsigned long long) * nx, b->stream) != hipSuccess)
         return fail(SX_EDEVICE, "clear");
+      if (hipMalloc(&b->a.xfail_g, sizeof(unsigned) * chains) != hipSuccess)
+        return fail(SX_ENOMEM, "exchange failure words");
+      if (hipMemsetAsync(b->a.xfail_g, 0, sizeof(unsigned) * chains, b->stream) != hipSuccess)
+        return fail(SX_EDEVICE, "clear");
+      // failure injection (tests): every exchange waits for a tag no part
+      // writes and gives up after a short bounded wait, so each chain ends
+      // with status 70 while every launch still runs to completion
+      if (const char* e = std::getenv("STANX_NUTS_XFAIL_TEST")) {
+        if (std::atoi(e) != 0) {
+          b->a.xnever = 1;
+          b->a.spin_limit = 256;
+        }
+      }
     }
   }
   if (hipMemsetAsync(b->a.vecs, 0, sizeof(double) * nvec, b->stream) != hipSuccess ||

@@ -110,10 +110,13 @@ __device__ void part_sum(const ctx& k, double (&x)[NS]) {
   const int P = a->parts;  // <= 4: 2 NS P <= 64 words, one per lane of wave 0
   chain_state* s = k.s;
   __syncthreads();
-  if (k.lane < 64) {
+  // a part that has already failed an exchange neither publishes nor waits
+  // again: its sums are void and the chain ends with this phase turn
+  if (k.lane < 64 && !s->xfail) {
     const int ln = k.lane;
     const uint64_t ep = s->xepoch;
     const unsigned long long tag = ((ep + 1) & 0xffffffffull) << 32;
+    const unsigned long long want = a->xnever ? (tag ^ (1ull << 63)) : tag;
     unsigned long long* base = a->xch + (static_cast<int64_t>(k.c) * 2 + (ep & 1)) * 16 * P;
     // publish: lane q < 2 NS writes word q of this part (half q % 2 of total
     // q / 2); the totals reach the lanes through LDS (every thread holds
@@ -138,9 +141,13 @@ __device__ void part_sum(const ctx& k, double (&x)[NS]) {
       unsigned long long* w = base + 16 * (ln / (2 * NS)) + ln % (2 * NS);
       unsigned long long h = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       int spins = 0;
-      while ((h & ~0xffffffffull) != tag) {
+      while ((h & ~0xffffffffull) != want) {
         if (++spins > a->spin_limit) {
+          // LDS flag for this part; the chain's global word tells the
+          // other parts (they check it at every launch start and stop)
           s->xfail = 1;
+          __hip_atomic_fetch_or(a->xfail_g + k.c, 1u, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -196,6 +203,7 @@ __device__ __forceinline__ double block_sum(const ctx& k, double x) {
 constexpr int kU = 4;
 template <class T, class Ld, class St>
 __device__ __forceinline__ void vpass(const ctx& k, Ld ld, St st) {
+  if (k.s->xfail) return;  // a failed exchange: leave the chain's vectors alone
   const int D = k.d1(), nt = k.nt;
   for (int i0 = k.d0() + k.lane; i0 < D; i0 += kU * nt) {
     T v[kU];
@@ -236,6 +244,7 @@ __device__ __forceinline__ double vsum(const ctx& k, Ld term) {
 // bitwise)
 template <int NS, class T, class Ld, class St>
 __device__ __forceinline__ void vpass_acc(const ctx& k, double (&acc)[NS], Ld ld, St st) {
+  if (k.s->xfail) return;  // a failed exchange: leave the chain's vectors alone
   const int D = k.d1(), nt = k.nt;
   for (int i0 = k.d0() + k.lane; i0 < D; i0 += kU * nt) {
     T v[kU];
@@ -733,6 +742,24 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   ctx k{&sa, &s, a.vecs + static_cast<int64_t>(c) * a.nslot * a.Dpad, c, lane, nt, red,
         a.chain0 + static_cast<uint32_t>(c), slice};
   if (s.phase == PH_DONE) return;
+  // another part of this chain timed out in an exchange: stop here, before
+  // any vector is touched
+  if (P > 1) {
+    __shared__ int stop;
+    if (lane == 0)
+      stop = __hip_atomic_load(a.xfail_g + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    __syncthreads();
+    if (stop) {
+      finish(k, 70);
+      __syncthreads();
+      if (part == 0) {
+        const int* src = reinterpret_cast<const int*>(&s);
+        int* dst = reinterpret_cast<int*>(a.states + c);
+        for (int i = lane; i < static_cast<int>(sizeof(chain_state) / 4); i += nt) dst[i] = src[i];
+      }
+      return;
+    }
+  }
   // a PH_LEAF gradient is consumed inside leaf(), fused with its pass
   bool leaf_ok = false;
   if (s.pending) {
@@ -1268,7 +1295,12 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
     }
     __syncthreads();
   }
-  // a part did not answer an exchange: end the chain
+  // a part did not answer an exchange: end the chain (this part failed, or
+  // another part's failure is already visible)
+  if (P > 1 && lane == 0 && !s.xfail &&
+      __hip_atomic_load(a.xfail_g + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    s.xfail = 1;
+  __syncthreads();
   if (s.xfail && s.phase != PH_DONE) finish(k, 70);
   __syncthreads();
   if (part == 0) {  // every part holds the same state; one writes it back

@@ -75,6 +75,12 @@ struct batch_args {
   int parts;
   int spin_limit;
   unsigned long long* xch;
+  // one word per chain, nonzero once any of its parts timed out in an
+  // exchange: every part stops the chain (status 70) when it sees it
+  unsigned* xfail_g;
+  // failure injection for tests: waits expect a tag no part writes, so
+  // every exchange times out after spin_limit polls
+  int xnever;
   // phase turns a chain may take per launch before it yields (large D:
   // the vector work of a deep subtree's merges would set the step time)
   int phase_budget;

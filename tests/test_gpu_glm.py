@@ -166,3 +166,23 @@ def test_large_logistic_parity_100_points():
     assert worst_lp < TOL
     assert worst_g < TOL
     m.close()
+
+
+def test_autotuned_contexts_are_bitwise_equal():
+    """The autotuner's layout and block count set the FP64 summation order;
+    its first decision per device and shape is reused, so two autotuned
+    contexts of one data set give bitwise-equal results."""
+    print("autotune determinism", flush=True)
+    d = W.logistic(300_000, 40, 1.0 / np.sqrt(40), 0.5, seed=404)
+    tn = dict(autotune=1)
+    a = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"], tuning=tn)
+    b = S.GlmModel(S.LOGIT, d["X"], d["y"], prior=d["prior"], tuning=tn)
+    assert b.tuning_log().startswith("cached:"), b.tuning_log()
+    assert a.kernel_config() == b.kernel_config()
+    for th in W.random_thetas(a.D, 3, seed=12):
+        la, ga = a.log_prob_grad(th)
+        lb, gb = b.log_prob_grad(th)
+        assert la == lb
+        np.testing.assert_array_equal(ga, gb)
+    a.close()
+    b.close()

@@ -295,3 +295,48 @@ def test_phase_budget_yields_leave_draws_unchanged():
     assert np.array_equal(a["draws"], u["draws"])
     assert nsteps["2"] > nsteps["4096"]
     m.close()
+
+
+def test_failed_exchange_ends_every_multi_part_chain():
+    """A cross-part exchange that times out ends its chain with status 70 in
+    every part.  Forced from the host (STANX_NUTS_XFAIL_TEST: every wait
+    expects a tag no part writes and gives up after a short bounded wait), so
+    every launch still runs to completion.  A batch created afterwards, with
+    the hook off, runs normally."""
+    import os
+    d = W.hier_logistic(J=500, K=20, seed=31, mean_log_size=np.log(80))
+    m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                   precision=S.FP32)
+    assert m.D > 8192
+    _say("forced exchange failure, 16 chains x 4 parts")
+    os.environ["STANX_NUTS_PARTS"] = "4"
+    os.environ["STANX_NUTS_XFAIL_TEST"] = "1"
+    try:
+        b = S.NutsBatch(m, 16, grad_precision=S.FP32, random_seed=5, chain=1,
+                        num_warmup=20, num_samples=10, max_depth=5)
+    finally:
+        del os.environ["STANX_NUTS_XFAIL_TEST"]
+    total = 0
+    while True:
+        steps, done = b.run(max_steps=64)
+        total += steps
+        if done >= 16 or steps == 0 or total > 4096:
+            break
+    res = b.results()
+    b.close()
+    assert np.all(res["status"] == 70), res["status"]
+    assert total < 64, total  # each chain stops at its first exchange
+    _say("hook off: the same batch runs normally")
+    try:
+        b = S.NutsBatch(m, 16, grad_precision=S.FP32, random_seed=5, chain=1,
+                        num_warmup=20, num_samples=10, max_depth=5)
+    finally:
+        del os.environ["STANX_NUTS_PARTS"]
+    while True:
+        steps, done = b.run(max_steps=512)
+        if done >= 16 or steps == 0:
+            break
+    res = b.results()
+    b.close()
+    assert np.all(res["status"] == 0), res["status"]
+    m.close()
