@@ -35,22 +35,53 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(d, theta, budget_s=12.0):
-    """The oracle's Stan-equivalent two-pass FP64 evaluation (mode 0), one
-    host core, timed on a bounded number of evaluations of the same data."""
-    import oracle as O
-    ref = O.GlmData(O.LOGIT, d["X"], d["y"], prior=d["prior"])
-    ref.lpgrad(theta, mode=0)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        ref.lpgrad(theta, mode=0)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 200:
-            break
-    return dict(value=n / el, unit="grad evals/s", cores=1, kind="port",
-                sample=f"{n} evaluations of the N=1e6 K=100 logistic log density + gradient "
-                       f"(oracle two-pass FP64, one thread), {el:.1f} s")
+def _host_threads():
+    """Threads the CPU leg may use: the cores this process may run on,
+    bounded by OMP_NUM_THREADS when it is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(d, theta, budget_s=10.0):
+    """The CPU restatement of the reference's two-pass GLM evaluation
+    (oracle/glm_cpu_baseline.c: -O3, x86-64-v3, OpenMP over rows), timed on
+    a bounded number of evaluations of the same data, on every host core
+    this process may use and on one core."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "libcpu_baseline.so"))
+    f = lib.cpu_logit_lpgrad
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                  C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+    X = np.ascontiguousarray(d["X"])
+    y = np.ascontiguousarray(d["y"], dtype=np.int32)
+    N, K = X.shape
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    g = np.zeros(K + 1)
+    lp = C.c_double()
+
+    def rate(threads, budget):
+        f(X.ctypes.data, y.ctypes.data, N, K, d["prior"][0], d["prior"][1], th.ctypes.data,
+          threads, C.byref(lp), g.ctypes.data)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            f(X.ctypes.data, y.ctypes.data, N, K, d["prior"][0], d["prior"][1], th.ctypes.data,
+              threads, C.byref(lp), g.ctypes.data)
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget or n >= 500:
+                return n / el, n, el
+    cores = _host_threads()
+    v_all, n_all, el_all = rate(cores, budget_s)
+    v_one, n_one, el_one = rate(1, budget_s / 2)
+    return dict(value=v_all, unit="grad evals/s", cores=cores, kind="port",
+                sample=f"{n_all} evaluations of the N=1e6 K=100 logistic log density + gradient "
+                       f"(two-pass FP64 CPU restatement, -O3 x86-64-v3, OpenMP over rows) on "
+                       f"{cores} threads in {el_all:.1f} s",
+                one_core=dict(value=v_one, cores=1,
+                              sample=f"{n_one} evaluations on one thread in {el_one:.1f} s"))
 
 
 def nuts_run(model, warmup, samples, seed=1234, **kw):
@@ -103,18 +134,22 @@ def batched_run(model, chains, warmup, samples, grad_precision, seed=1234, **kw)
                 samples=samples)
 
 
-def serving_stream(n_jobs, rate=20.0, seed=7):
-    """Poisson arrivals (rate jobs/s); per job: logistic or normal GLM with
-    equal probability, N log-uniform in [1e3, 2e6], K log-uniform in
-    [8, 512], chains uniform in {4, 8, 16, 32, 64}."""
+def serving_stream(window_s, rate=20.0, seed=7):
+    """Poisson arrivals (rate jobs/s) over the first window_s seconds of the
+    stream; per job: logistic or normal GLM with equal probability, N
+    log-uniform in [1e3, 2e6], K log-uniform in [8, 512], chains uniform in
+    {4, 8, 16, 32, 64}.  The draws are those of the stream's first jobs
+    whatever the window (one generator call per quantity, 4096 jobs)."""
+    n_max = 4096
     rng = np.random.Generator(np.random.PCG64(seed))
-    t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
-    logit = rng.random(n_jobs) < 0.5
-    N = np.exp(rng.uniform(np.log(1e3), np.log(2e6), n_jobs)).astype(np.int64)
-    K = np.exp(rng.uniform(np.log(8), np.log(512), n_jobs)).astype(np.int64)
-    C = rng.choice([4, 8, 16, 32, 64], n_jobs)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_max))
+    logit = rng.random(n_max) < 0.5
+    N = np.exp(rng.uniform(np.log(1e3), np.log(2e6), n_max)).astype(np.int64)
+    K = np.exp(rng.uniform(np.log(8), np.log(512), n_max)).astype(np.int64)
+    C = rng.choice([4, 8, 16, 32, 64], n_max)
+    n = int(np.searchsorted(t, window_s))
     return [dict(t=float(t[i]), logit=bool(logit[i]), N=int(N[i]), K=int(K[i]), C=int(C[i]),
-                 seed=1000 + i) for i in range(n_jobs)]
+                 seed=1000 + i) for i in range(n)]
 
 
 def serving_data(job):
@@ -131,20 +166,28 @@ def serving_data(job):
     return X, y
 
 
-def serving_section(n_jobs, target_ess=1000.0, warmup=200, deadline_s=120.0):
-    """C5: a bounded prefix of the arrival stream through the serving loop."""
+def serving_section(window_s=3.0, target_ess=1000.0, warmup=200, deadline_s=60.0):
+    """C5: the arrival stream at 20 jobs/s for window_s seconds, replayed in
+    real time against the serving loop (jobs queue while the device is
+    busy), with a deadline of deadline_s after the last arrival.  Latency is
+    submit to result and includes the queueing; jobs unfinished at the
+    deadline are reported, and then the p99 is only a lower bound.  The
+    sustainable rate is completed jobs per second of the busy period."""
+    import concurrent.futures as cf
     import stan_amd as S
     from stan_amd import workloads as W
-    jobs = serving_stream(n_jobs)
-    log(f"serving: generating {n_jobs} jobs")
-    data = [serving_data(j) for j in jobs]
+    jobs = serving_stream(window_s)
+    log(f"serving: generating the {len(jobs)} jobs of a {window_s:.0f} s window")
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        data = list(ex.map(serving_data, jobs))
     srv = S.Server(200 * (1 << 30), max_active=16)
     ids = []
     t0 = time.perf_counter()
-    for j, (X, y) in zip(jobs, data):
+    for j in jobs:
         delay = j["t"] - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
+        X, y = data[len(ids)]
         fam = S.LOGIT if j["logit"] else S.NORMAL
         prior = W.PRIOR_LOGISTIC if j["logit"] else W.PRIOR_NORMAL
         first = int(np.ceil(1.5 * target_ess / j["C"])) + 20
@@ -152,21 +195,15 @@ def serving_section(n_jobs, target_ess=1000.0, warmup=200, deadline_s=120.0):
                               target_min_ess=target_ess, max_samples=max(400, 8 * first),
                               num_warmup=warmup, num_samples=first, random_seed=j["seed"],
                               chain=1))
-    del data
+        data[len(ids) - 1] = None  # the server holds its own copy until admission
     t_submitted = time.perf_counter()
+    lag = t_submitted - t0 - jobs[-1]["t"]
+    log(f"serving: {len(ids)} jobs submitted over {t_submitted - t0:.1f} s "
+        f"(submission lag {lag:.2f} s)")
     last = t_submitted
     pending = list(ids)
     while pending and time.perf_counter() - t_submitted < deadline_s:
-        still = []
-        for i in pending:
-            if srv.wait(i, 0.0):
-                inf = srv.info(i)
-                log(f"  job {i}: {inf['t_done'] - inf['t_submit']:.1f} s, {inf['steps']} steps, "
-                    f"min ESS {inf['min_ess']:.0f}, {inf['extensions']} extensions, "
-                    f"{'BF16 hi+lo' if inf['grad_precision'] == S.BF16X2 else 'FP64'}")
-            else:
-                still.append(i)
-        pending = still
+        pending = [i for i in pending if not srv.wait(i, 0.0)]
         time.sleep(0.2)
         if time.perf_counter() - last > 10:
             last = time.perf_counter()
@@ -175,26 +212,30 @@ def serving_section(n_jobs, target_ess=1000.0, warmup=200, deadline_s=120.0):
     srv.close()  # cancels what is still running at the deadline
     unfinished = len(pending)
     infos = [infos_all[i] for i in ids if i not in pending]
+    out = dict(window_s=window_s, jobs=len(ids), completed=len(infos),
+               unfinished_at_deadline=unfinished, deadline_s=deadline_s,
+               arrival_rate_per_s=20.0, seed=7, target_min_ess=target_ess, warmup=warmup)
     if not infos:
-        return dict(jobs=n_jobs, completed=0, unfinished_at_deadline=unfinished,
-                    deadline_s=deadline_s)
+        return out
     lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    # unfinished jobs count as latencies beyond the deadline
+    cens = np.concatenate([lat, np.full(unfinished, np.inf)])
     queue = np.array([i["t_admit"] - i["t_submit"] for i in infos])
-    t_first = min(i["t_submit"] for i in infos)
-    makespan = max(i["t_done"] for i in infos) - t_first
+    t_first = min(i["t_submit"] for i in infos_all.values())
+    busy = max(i["t_done"] for i in infos) - t_first
     n_grad = sum(i["n_grad"] for i in infos)
     ess = [i["min_ess"] for i in infos]
-    return dict(jobs=n_jobs, completed=len(infos), unfinished_at_deadline=unfinished,
-                deadline_s=deadline_s, arrival_rate_per_s=20.0, seed=7, target_min_ess=target_ess,
-                warmup=warmup, p50_s=float(np.percentile(lat, 50)),
-                p99_s=float(np.percentile(lat, 99)), max_s=float(lat.max()),
-                p50_queue_s=float(np.percentile(queue, 50)), makespan_s=makespan,
-                grad_evals_per_s=n_grad / makespan,
-                min_ess_per_s=float(np.nansum(ess)) / makespan,
-                jobs_reaching_target=int(sum(e >= target_ess for e in ess)),
-                min_ess_min=float(np.nanmin(ess)),
-                extensions=int(sum(i["extensions"] for i in infos)),
-                mfma_jobs=int(sum(1 for i in infos if i["grad_precision"] == S.BF16X2)))
+    out.update(p50_s=float(np.percentile(cens, 50)), p99_s=float(np.percentile(cens, 99)),
+               p99_is_lower_bound=bool(unfinished > 0), max_completed_s=float(lat.max()),
+               p50_queue_s=float(np.percentile(queue, 50)),
+               p99_queue_s=float(np.percentile(queue, 99)),
+               busy_period_s=busy, sustainable_jobs_per_s=len(infos) / busy,
+               grad_evals_per_s=n_grad / busy, min_ess_per_s=float(np.nansum(ess)) / busy,
+               jobs_reaching_target=int(sum(e >= target_ess for e in ess)),
+               min_ess_min=float(np.nanmin(ess)),
+               extensions=int(sum(i["extensions"] for i in infos)),
+               mfma_jobs=int(sum(1 for i in infos if i["grad_precision"] == S.BF16X2)))
+    return out
 
 
 def mixed_error(model, ref_data, family, thetas):
@@ -212,22 +253,63 @@ def mixed_error(model, ref_data, family, thetas):
     return e_lp, e_g
 
 
-def hier_section(warm, samp, chains=64, max_depth=8):
+def split_rhat(draws):
+    """draws: iterations x chains x params -> split R-hat per parameter."""
+    n = draws.shape[0] // 2
+    s = np.concatenate([draws[:n], draws[n:2 * n]], axis=1)
+    w = s.var(axis=0, ddof=1).mean(axis=0)
+    b = n * s.mean(axis=0).var(axis=0, ddof=1)
+    return np.sqrt(((n - 1) / n * w + b / n) / w)
+
+
+def hier_section(warm, samp, chains=64, max_depth=10, plain_samples=200):
     """Hierarchical logistic model, J=1000 ragged groups, K=20: batched FP32
     evaluation and device NUTS over 64 chains, against the single-chain FP64
-    service on the same context."""
+    service on the same context (same warm-up, fewer draws)."""
     import stan_amd as S
     from stan_amd import workloads as W
     dh = W.hier_logistic()
     N, K = dh["X"].shape
     J = len(dh["offsets"]) - 1
-    log(f"hierarchical logistic J={J} N={N} K={K}, {chains} chains FP32")
+    log(f"hierarchical logistic J={J} N={N} K={K}, {chains} chains FP32, "
+        f"{warm} + {samp} iterations, max_depth {max_depth}")
     mh = S.GlmModel(S.HIER_LOGIT, dh["X"], dh["y"], offsets=dh["offsets"], prior=dh["prior"],
                     precision=S.FP32)
     th = W.random_thetas(mh.D, chains, seed=9) * 0.5
     ms_b, _ = mh.time_log_prob_grad(th, iters=20, warmup=3, chains=chains)
     ms_1, _ = mh.time_log_prob_grad(th[0], iters=20, warmup=3)
-    out = batched_run(mh, chains, warm, samp, S.FP32, max_depth=max_depth)
+    b = S.NutsBatch(mh, chains, grad_precision=S.FP32, random_seed=1234, chain=1,
+                    num_warmup=warm, num_samples=samp, max_depth=max_depth)
+    t0 = last = time.perf_counter()
+    while True:
+        steps, done = b.run(max_steps=2048)
+        if done >= chains or steps == 0:
+            break
+        if time.perf_counter() - last > 10:
+            last = time.perf_counter()
+            log(f"  ... {done}/{chains} chains done, {last - t0:.0f} s")
+    res = b.results()
+    b.close()
+    if not (res["status"] == 0).all():
+        raise RuntimeError(f"chain status {res['status']}")
+    from stan_amd import analyze
+    q = np.transpose(res["draws"][:, :, 7:], (1, 0, 2))
+    ess = analyze.min_bulk_ess(q)
+    secs = res["seconds"]
+    rh = split_rhat(q)
+    out = dict(chains=chains, min_ess=ess, seconds=secs, min_ess_per_s=ess / secs,
+               grad_evals=int(res["n_grad"].sum()),
+               grad_evals_per_s=float(res["n_grad"].sum()) / secs, steps=int(res["steps"]),
+               ms_per_step=1e3 * secs / max(1, res["steps"]),
+               mean_n_leapfrog=float(res["draws"][:, :, 4].mean()),
+               mean_treedepth=float(res["draws"][:, :, 3].mean()),
+               stepsize_median=float(np.median(res["stepsize"])),
+               stepsize_max_over_median=float(np.max(res["stepsize"]) /
+                                               np.median(res["stepsize"])),
+               divergent_draws=int(res["draws"][:, :, 5].sum()),
+               split_rhat_max=float(rh.max()),
+               split_rhat_max_global=float(rh[: 2 * K + 2].max()),
+               warmup=warm, samples=samp)
     # X (FP32) + y (int8) + chunk tables; theta/out of every chain
     alg_bytes = 4.0 * N * K + N + 16.0 * (N / 256 + J) + 8.0 * chains * (2 * mh.D + 2)
     out.update(N=N, K=K, J=J, D=mh.D, max_depth=max_depth, ms_per_batched_eval=ms_b,
@@ -237,8 +319,8 @@ def hier_section(warm, samp, chains=64, max_depth=8):
     e_lp, e_g = mixed_error(mh, dh, S.HIER_LOGIT, th[:3])
     out["max_rel_err_lp"] = e_lp
     out["max_rel_err_grad"] = e_g
-    log("hierarchical: single-chain FP64 service for comparison")
-    out["plain_fp64_single_chain"] = nuts_run(mh, warm, samp, max_depth=max_depth)
+    log(f"hierarchical: single-chain FP64 service for comparison ({warm} + {plain_samples})")
+    out["plain_fp64_single_chain"] = nuts_run(mh, warm, plain_samples, max_depth=max_depth)
     out["min_ess_per_s_vs_plain"] = (out["min_ess_per_s"] /
                                      out["plain_fp64_single_chain"]["min_ess_per_s"])
     mh.close()
@@ -331,8 +413,13 @@ def main():
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--no-hier", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=16)
+    ap.add_argument("--serve-window", type=float, default=3.0,
+                    help="seconds of the 20 jobs/s arrival stream to replay")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only the copy kernel of known size (the measured HBM ceiling; run "
+                         "under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it gives the counter "
+                         "scale)")
     ap.add_argument("--kernels", action="store_true",
                     help="only short runs of the batched kernels (for kernel traces)")
     ap.add_argument("--case", action="append",
@@ -346,6 +433,15 @@ def main():
                           "(across GPUs, the plain path would shard rows and the batched "
                           "sampler whole chains)"}))
         sys.exit(2)
+    if args.calibrate:
+        import stan_amd as S
+        nbytes = 800_000_000
+        gbs, ms = S.device_copy_bandwidth(nbytes, iters=max(3, args.steps))
+        print(json.dumps({"calibrate": dict(copy_bytes=nbytes, read_kib_per_copy=nbytes / 1024,
+                                            write_kib_per_copy=nbytes / 1024, copies=max(3, args.steps) + 1,
+                                            ms_per_copy=ms, gbs_read_plus_write=gbs,
+                                            peak_gbs=PEAK_HBM_GBS)}))
+        return
     if args.kernels:
         print(json.dumps({"kernels": kernels_section(args.steps, args.sampler_iters,
                                                      args.case)}))
@@ -383,10 +479,13 @@ def main():
     achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
     kc = m.kernel_config()
     traffic = PMC_TRAFFIC_BYTES.get((kc["lanes_per_row"], kc["slots_per_lane"]))
+    copy_gbs, _ = S.device_copy_bandwidth(800_000_000, iters=20)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS,
                     unit="GB/s", frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic,
                     kernel="glm_pass_f64 (logistic)", bytes_per_launch=alg_bytes,
-                    kernel_ms=ms_kernel, kernel_config=m.kernel_config())
+                    kernel_ms=ms_kernel, kernel_config=m.kernel_config(),
+                    measured_copy_gbs=round(copy_gbs, 1),
+                    frac_of_measured_copy=round(achieved / copy_gbs, 4))
 
     result = {
         "metric": "grad evals/s + min-ESS/s, batched vs plain on 1 GPU; p50/p99 request latency",
@@ -449,14 +548,10 @@ def main():
         result["normal_glm_batched_256"] = bn
         mn.close()
     if not args.no_serve:
-        log(f"serving: first {args.serve_jobs} jobs of the arrival stream")
-        result["serving"] = serving_section(args.serve_jobs)
+        log(f"serving: {args.serve_window:.0f} s of the 20 jobs/s arrival stream")
+        result["serving"] = serving_section(args.serve_window)
     if not args.no_hier:
-        # 150 warm-up iterations: with 100, one of the 64 chains (seed 1234)
-        # left warm-up with a step size ten times the others' and diverged
-        # on every draw of its second half; a constant split half leaves
-        # the split-rank ESS undefined
-        result["hier_logistic_64"] = hier_section(150, 100)
+        result["hier_logistic_64"] = hier_section(1000, 500)
     if not args.no_c1:
         log("reference CPU case N=1e4 K=50, 4 chains x (1000 + 1000)")
         result["c1_logistic_4_chains"] = c1_section(with_cpu=not args.no_cpu)

@@ -346,6 +346,13 @@ int sx_hmc_nuts_dense_e_cb(sx_lpgrad_fn f, void* ctx, int D,
                            const sx_nuts_config* cfg, const double* init,
                            const double* init_inv_metric, sx_nuts_result* out);
 
+/* A measured HBM ceiling for roofline figures: times `iters` launches of a
+ * plain copy of `bytes` (16-byte loads and stores, device to device) and
+ * returns GB/s counting both the read and the write (2 * bytes per copy),
+ * and the milliseconds per copy.  No counterpart in the reference.       */
+int sx_device_copy_bandwidth(int device, int64_t bytes, int iters, double* gbs,
+                             double* ms_per_copy);
+
 /* ---------------- batched device-resident sampler ---------------- */
 /* `chains` chains of adaptive diag_e NUTS on the device, chain ids
  * cfg->chain, cfg->chain + 1, ... (the reference's multi-chain service,

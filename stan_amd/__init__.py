@@ -32,7 +32,7 @@ EXPORTED = [
     "sx_hmc_nuts_dense_e_adapt", "sx_hmc_nuts_dense_e_adapt_cb", "sx_hmc_nuts_dense_e_adapt_csv",
     "sx_hmc_nuts_dense_e", "sx_hmc_nuts_dense_e_cb", "sx_hmc_nuts_dense_e_adapt_cb_csv",
     "sx_hmc_nuts_diag_e_adapt_files", "sx_hmc_nuts_diag_e_adapt_cb_files",
-    "sx_hmc_nuts_diag_e_files",
+    "sx_hmc_nuts_diag_e_files", "sx_device_copy_bandwidth",
 ]
 
 _lib = None
@@ -147,6 +147,7 @@ def lib():
                                                         vp, vp, C.POINTER(OutputFiles),
                                                         C.POINTER(NutsResult)]
         L.sx_hmc_nuts_diag_e_files.argtypes = L.sx_hmc_nuts_diag_e_adapt_files.argtypes
+        L.sx_device_copy_bandwidth.argtypes = [C.c_int, i64, C.c_int, dp, dp]
         L.sx_hmc_nuts_dense_e_adapt.argtypes = L.sx_hmc_nuts_diag_e_adapt.argtypes
         L.sx_hmc_nuts_dense_e_adapt_cb.argtypes = L.sx_hmc_nuts_diag_e_adapt_cb.argtypes
         L.sx_hmc_nuts_dense_e_adapt_csv.argtypes = L.sx_hmc_nuts_diag_e_adapt_csv.argtypes
@@ -177,6 +178,15 @@ def lib():
         L.sx_nuts_batch_destroy.restype = None
         _lib = L
     return _lib
+
+
+def device_copy_bandwidth(nbytes=1 << 31, iters=20, device=0):
+    """Measured copy bandwidth of the device (GB/s, read + write) and the
+    milliseconds per copy of `nbytes`."""
+    gbs, ms = C.c_double(), C.c_double()
+    _check(lib().sx_device_copy_bandwidth(device, int(nbytes), int(iters), C.byref(gbs),
+                                          C.byref(ms)))
+    return gbs.value, ms.value
 
 
 def last_error():

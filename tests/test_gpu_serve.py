@@ -131,3 +131,38 @@ def test_min_ess_target_and_native_ess():
     q = np.transpose(r["draws"][:, :n, 7:], (1, 0, 2))
     assert abs(analyze.min_bulk_ess(q) - inf["min_ess"]) <= 1e-8 * inf["min_ess"]
     srv.close()
+
+
+def test_served_posterior_matches_oracle_sampler():
+    """A job run through the serving loop, sharing the device with another
+    job, against the oracle sampler on the same posterior: means and sds
+    within 4 MCSE."""
+    a = W.logistic(2000, 5, 1.0 / np.sqrt(5), 0.5, seed=501)
+    other = W.normal(20000, 32, seed=502)
+    srv = S.Server(64 * GB)
+    jid = srv.submit(S.LOGIT, a["X"], a["y"], 8, prior=a["prior"], num_warmup=500,
+                     num_samples=1000, random_seed=55, chain=1)
+    oid = srv.submit(S.NORMAL, other["X"], other["y"], 16, prior=other["prior"],
+                     grad_precision=S.BF16X2, num_warmup=200, num_samples=200, random_seed=56)
+    assert srv.wait(jid, 300) and srv.wait(oid, 300)
+    r = srv.results(jid)
+    srv.close()
+    assert np.all(r["status"] == 0)
+    dev = np.transpose(r["draws"][:, :1000, 7:], (1, 0, 2))
+    import oracle as O
+    ref = O.GlmData(O.LOGIT, a["X"], a["y"], prior=a["prior"])
+    f = O.model_fn("orc_glm_model_cb")
+    ctx = ref.ctx(mode=0)
+    cpu = []
+    for chain in range(1, 5):
+        rc, dr, _, _, _ = O.hmc_nuts_diag_e_adapt(f, ctx, 6, seed=5555, chain=chain,
+                                                  num_warmup=500, num_samples=1000)
+        assert rc == 0
+        cpu.append(dr[:, 7:])
+    cpu = np.stack(cpu, 1)
+    for p in range(6):
+        se = np.hypot(analyze.mcse_mean(dev[:, :, p]), analyze.mcse_mean(cpu[:, :, p]))
+        assert abs(dev[:, :, p].mean() - cpu[:, :, p].mean()) < 4 * se, p
+        se_sd = np.hypot(analyze.mcse_sd(dev[:, :, p]), analyze.mcse_sd(cpu[:, :, p]))
+        assert abs(dev[:, :, p].std() - cpu[:, :, p].std()) < 4 * se_sd, p
+    _say(f"served job: min bulk ESS {analyze.min_bulk_ess(dev):.0f}")

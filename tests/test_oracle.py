@@ -336,3 +336,26 @@ def test_oracle_sampler_output_shape_and_thinning():
                                              num_thin=5, save_warmup=1)
     assert rc == 0
     assert d.shape == ((200 + 400) // 5, 9)
+
+
+def test_cpu_baseline_leg_agrees_with_oracle():
+    """The benchmark's timed CPU leg (oracle/glm_cpu_baseline.c, -O3 OpenMP)
+    computes the same density as the oracle; threads only regroup the sums."""
+    import ctypes as Cc
+    import os
+    lib = Cc.CDLL(os.path.join(os.path.dirname(O.__file__), "libcpu_baseline.so"))
+    f = lib.cpu_logit_lpgrad
+    f.argtypes = [Cc.c_void_p, Cc.c_void_p, Cc.c_int64, Cc.c_int64, Cc.c_double, Cc.c_double,
+                  Cc.c_void_p, Cc.c_int, Cc.POINTER(Cc.c_double), Cc.c_void_p]
+    X, y = _rand_logit(3001, 23, 5)
+    rng = np.random.default_rng(6)
+    th = rng.standard_normal(24)
+    ref = O.GlmData(O.LOGIT, X, y, prior=(5.0, 2.0, 2.0, 1.0))
+    st, lp_r, g_r = ref.lpgrad(th, mode=1)
+    for threads in (1, 3):
+        g = np.zeros(24)
+        lp = Cc.c_double()
+        assert f(X.ctypes.data, y.ctypes.data, 3001, 23, 5.0, 2.0, th.ctypes.data, threads,
+                 Cc.byref(lp), g.ctypes.data) == 0
+        assert abs(lp.value - lp_r) <= 1e-12 * abs(lp_r)
+        assert np.linalg.norm(g - g_r) <= 1e-12 * np.linalg.norm(g_r)
