@@ -433,12 +433,13 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     // blocks of 256 + 64 j)
     m->Kp = K <= 256 ? std::max<int64_t>(32, (K + 31) / 32 * 32) : (K + 63) / 64 * 64;
     const size_t elems = static_cast<size_t>(m->Np) * m->Kp;
-    if (hipMalloc(&m->d_X1h, elems * 2) != hipSuccess || hipMalloc(&m->d_X1l, elems * 2) != hipSuccess ||
-        hipMalloc(&m->d_X2h, elems * 2) != hipSuccess || hipMalloc(&m->d_X2l, elems * 2) != hipSuccess) {
+    if (hipMalloc(&m->d_Xh, elems * 2) != hipSuccess || hipMalloc(&m->d_Xl, elems * 2) != hipSuccess ||
+        hipMalloc(&m->d_yf, sizeof(float) * m->Np) != hipSuccess) {
       set_error("hipMalloc(BF16 X) failed");
       rc = SX_ENOMEM;
-    } else if (sx::launch_split_x(staging, m->d_X1h, m->d_X1l, m->d_X2h, m->d_X2l, N, K, m->Np,
-                                  m->Kp, m->stream) != hipSuccess) {
+    } else if (sx::launch_split_x(staging, m->d_Xh, m->d_Xl, N, K, m->Np, m->Kp, m->stream) !=
+                   hipSuccess ||
+               sx::launch_prep_y(m->d_y, m->d_yf, family, N, m->Np, m->stream) != hipSuccess) {
       set_error("BF16 split of X failed");
       rc = SX_EDEVICE;
     }
@@ -461,10 +462,9 @@ void sx_glm_destroy(sx_glm* m) {
   (void)hipFree(m->d_theta);
   (void)hipFree(m->d_partials);
   (void)hipFree(m->d_out);
-  (void)hipFree(m->d_X1h);
-  (void)hipFree(m->d_X1l);
-  (void)hipFree(m->d_X2h);
-  (void)hipFree(m->d_X2l);
+  (void)hipFree(m->d_Xh);
+  (void)hipFree(m->d_Xl);
+  (void)hipFree(m->d_yf);
   sx::free_workspace(m->mx);
   (void)hipFree(m->d_btheta);
   (void)hipFree(m->d_bout);
@@ -581,6 +581,7 @@ namespace sx {
 void free_workspace(mx_workspace& w) {
   (void)hipFree(w.Bh);
   (void)hipFree(w.Bl);
+  (void)hipFree(w.Bm);
   (void)hipFree(w.Rh);
   (void)hipFree(w.Rl);
   (void)hipFree(w.alpha);
@@ -605,7 +606,7 @@ int mx_chain_pad(const sx_glm* m, int C) {
 }
 
 int mx_prepare(sx_glm* m, mx_workspace& w, int C) {
-  if (!m->d_X1h) {
+  if (!m->d_Xh) {
     set_error("batched BF16 evaluation needs a context created with SX_BF16X2");
     return SX_EINVAL;
   }
@@ -617,6 +618,7 @@ int mx_prepare(sx_glm* m, mx_workspace& w, int C) {
   const size_t ck = static_cast<size_t>(Cp) * m->Kp;
   SX_HIP(hipMalloc(&w.Bh, ck * 2));
   SX_HIP(hipMalloc(&w.Bl, ck * 2));
+  SX_HIP(hipMalloc(&w.Bm, ck * 2));
   SX_HIP(hipMalloc(&w.alpha, sizeof(float) * Cp));
   SX_HIP(hipMalloc(&w.gpart, sizeof(float) * ck * splits));
   SX_HIP(hipMalloc(&w.spart, sizeof(double) * 3 * static_cast<size_t>(Cp) * splits));
@@ -636,13 +638,13 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   int rc = mx_prepare(m, w, C);
   if (rc) return rc;
   mx_args a{};
-  a.X1h = m->d_X1h;
-  a.X1l = m->d_X1l;
-  a.X2h = m->d_X2h;
-  a.X2l = m->d_X2l;
+  a.Xh = m->d_Xh;
+  a.Xl = m->d_Xl;
   a.y = m->d_y;
+  a.yf = m->d_yf;
   a.Bh = w.Bh;
   a.Bl = w.Bl;
+  a.Bm = w.Bm;
   a.alpha = w.alpha;
   a.gpart = w.gpart;
   a.spart = w.spart;
@@ -654,6 +656,8 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   a.Cp = mx_chain_pad(m, C);
   a.splits = w.splits;
   a.tiles_per_split = (a.n_tiles + a.splits - 1) / a.splits;
+  a.variant = 0;
+  if (const char* e = std::getenv("STANX_MFMA_VARIANT")) a.variant = std::atoi(e);
   pass_args pa{};
   pa.theta = d_theta;
   pa.out = d_out;

@@ -124,8 +124,8 @@ int64_t footprint_of(const sx_job_spec& s) {
       const int64_t Np = ceil_div(N, 32) * 32;
       const int64_t Kp = K <= 256 ? std::max<int64_t>(32, ceil_div(K, 32) * 32) : ceil_div(K, 64) * 64;
       const int64_t Cp = ceil_div(C, 128) * 128;
-      bytes += 4 * Np * Kp * 2;                                   // X1/X2 hi/lo
-      bytes += Cp * Kp * 4 + Cp * 4 + Cp * Kp * 4 * 512 + 3 * Cp * 512 * 8;  // workspace
+      bytes += 2 * Np * Kp * 2 + 4 * Np;                          // X hi/lo, y per row
+      bytes += Cp * Kp * 6 + Cp * 4 + Cp * Kp * 4 * 512 + 3 * Cp * 512 * 8;  // workspace
       if (Kp > 256) bytes += Np * Cp * 4;                         // residuals hi/lo
     } else {
       bytes += 2048 * (K + 3) * 8 * C;                            // per-chain partials

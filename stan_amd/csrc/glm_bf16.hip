@@ -5,20 +5,34 @@
 // contraction H = X B and the adjoint is G = X^T R with R the residual
 // matrix: two GEMMs per evaluation, which is where MFMA pays (the single-
 // chain pass is a matrix-vector product and stays on the HBM roofline).
-// Each product of a split pair is formed as hi*hi + hi*lo + lo*hi (the
-// lo*lo term is below the split's own representation error) with
-// v_mfma_f32_32x32x16_bf16.
+// All products run on v_mfma_f32_32x32x16_bf16.
 //
-// The two contractions run over different indices of X (k forward, n in
-// the adjoint) and an MFMA operand holds 8 consecutive inner-index values
-// per lane, so X is stored twice, in HBM, as BF16 hi+lo:
-//   X1[n][k]            k-packed rows      (forward operand)
-//   X2[n/8][k][n % 8]   8-row-packed       (adjoint operand)
-// Workgroup = 4 waves x 32 chains (glm_mfma32); a 1-D grid of row splits x
-// chain blocks, mapped so that the blocks sharing a row range land on the
-// same XCD and share X in its L2.  Per-block FP32 partials are summed
-// across blocks in FP64 by a second launch that also adds priors and
-// Jacobian (shared finalize).
+// Precision.  X = Xh + Xl (two BF16 terms, about 16 significant bits).
+// The chains' coefficients carry three BF16 terms, B = Bh + Bl + Bm (the
+// FP32 value exactly), because an error in B is the same for every row:
+// it does not average out over the N rows the way the per-row errors of X
+// and R do, and with two terms it sets the gradient error (about 1e-4
+// relative near the posterior mode at N = 1e6).  The forward therefore
+// forms Xh Bh + Xh Bl + Xl Bh + Xh Bm, the adjoint Rh Xh + Rh Xl + Rl Xh
+// (R = Rh + Rl); the dropped terms (Xl Bl, Xl Bm, Rl Xl) are below the
+// representation error of the two-term splits.
+//
+// Layout.  X is stored once, k-packed (row n: Kp contiguous BF16 values),
+// hi and lo in separate arrays.  The forward operand takes 8 consecutive k
+// of one row per lane (one 16-byte LDS read); the adjoint contracts over
+// rows and takes 4 consecutive rows of one column per lane with gfx950's
+// transposing LDS read (ds_read_b64_tr_b16), from the same staged tile.
+// In LDS, 16-byte chunk c of tile row r sits at c ^ (r & SWZ), which
+// spreads both access patterns over the banks.
+//
+// Workgroup = NW waves x 32 chains (NW = 4 for 128-chain tiles, 2 for
+// batches of at most 64 chains); each 32-row tile of X is staged once per
+// workgroup, straight from global memory into LDS (global_load_lds), one
+// tile ahead in the other of two buffers.  1-D grid of row splits x chain
+// blocks, mapped so that the blocks sharing a row range land on the same
+// XCD and share X in its L2.  Per-block FP32 partials are summed across
+// blocks in FP64 by a second launch that also adds priors and Jacobian
+// (shared finalize).
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -33,6 +47,10 @@ namespace sx {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of global_load_lds
+typedef __attribute__((address_space(3))) bf16x4* lds_bf16x4;
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
 
@@ -61,9 +79,9 @@ __device__ __forceinline__ void logit_row_fast(float eta, float s, float* term, 
   *r = s * (se >= 0.0f ? t * inv : inv);
 }
 
-// Splits row-major FP64 X into the two BF16 hi+lo layouts (zero padded).
-__global__ void split_x(const double* __restrict__ Xrm, __bf16* X1h, __bf16* X1l, __bf16* X2h,
-                        __bf16* X2l, int64_t N, int64_t K, int64_t Kp, int64_t total) {
+// Row-major FP64 X -> k-packed BF16 hi and lo (Np x Kp, zero padded).
+__global__ void split_x(const double* __restrict__ Xrm, __bf16* Xh, __bf16* Xl, int64_t N,
+                        int64_t K, int64_t Kp, int64_t total) {
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const int64_t n = idx / Kp;
@@ -71,64 +89,190 @@ __global__ void split_x(const double* __restrict__ Xrm, __bf16* X1h, __bf16* X1l
   const float x = (n < N && k < K) ? static_cast<float>(Xrm[n * K + k]) : 0.0f;
   __bf16 h, l;
   split(x, &h, &l);
-  X1h[idx] = h;
-  X1l[idx] = l;
-  const int64_t j = ((n >> 3) * Kp + k) * 8 + (n & 7);
-  X2h[j] = h;
-  X2l[j] = l;
+  Xh[idx] = h;
+  Xl[idx] = l;
 }
 
-// theta (C x D, FP64) -> B hi/lo (Cp x Kp), alpha (Cp)
+// y -> one float per padded row: the sign +-1 of the logistic model's
+// outcome, the normal model's value; 0 past N.
+__global__ void prep_y(const void* __restrict__ y, float* yf, int family, int64_t N,
+                       int64_t Np) {
+  const int64_t n = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= Np) return;
+  float v = 0.f;
+  if (n < N) {
+    if (family == SX_LOGIT)
+      v = static_cast<const int8_t*>(y)[n] ? 1.0f : -1.0f;
+    else
+      v = static_cast<float>(static_cast<const double*>(y)[n]);
+  }
+  yf[n] = v;
+}
+
+// theta (C x D, FP64) -> B as three BF16 terms (Cp x Kp each: the FP32
+// value of beta exactly), alpha (Cp)
 __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, int K, int Kp,
-                            __bf16* Bh, __bf16* Bl, float* alpha) {
+                            __bf16* Bh, __bf16* Bl, __bf16* Bm, float* alpha) {
   const int c = blockIdx.x;
   for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
     const float b = (c < C && k < K) ? static_cast<float>(theta[c * D + 1 + k]) : 0.0f;
-    __bf16 h, l;
-    split(b, &h, &l);
-    Bh[static_cast<int64_t>(c) * Kp + k] = h;
-    Bl[static_cast<int64_t>(c) * Kp + k] = l;
+    const __bf16 h = to_bf16(b);
+    const float r1 = b - static_cast<float>(h);  // exact: at most 16 significant bits
+    const __bf16 l = to_bf16(r1);
+    const __bf16 m = to_bf16(r1 - static_cast<float>(l));  // exact: at most 8 bits
+    const int64_t o = static_cast<int64_t>(c) * Kp + k;
+    Bh[o] = h;
+    Bl[o] = l;
+    Bm[o] = m;
   }
   if (threadIdx.x == 0) alpha[c] = c < C ? static_cast<float>(theta[c * D]) : 0.0f;
 }
 
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));  // a 16-byte staging block
-typedef __attribute__((address_space(3))) void* lds_ptr;  // LDS pointer of global_load_lds
+// The adjoint's X operand from a staged tile (row stride RS elements, CPR
+// chunks of 8 per row, swizzle SWZ): for the lane's column c = c0 + li
+// (li = lane % 32, h = lane / 32), rows r0 + 4 h + (0..3) and
+// r0 + 8 + 4 h + (0..3), i.e. element j of the MFMA B operand is row
+// r0 + 8 (j / 4) + 4 h + j % 4 (the rows of the forward's accumulator
+// element 8 st + j when r0 = 16 st).  Two transposing reads: within each
+// group of 16 lanes, lane i reads 4 consecutive columns (8 bytes) of one
+// row, and lane i receives column i of the 4 rows the group read
+// (result element e = row e).
+template <int RS, int SWZ>
+__device__ __forceinline__ bf16x8 tr_operand(const __bf16* t, int r0, int c0, int lane) {
+  const int i = lane & 15, g = lane >> 4;
+  const int hh = g >> 1, ch = g & 1;
+  bf16x8 v;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int r = r0 + 8 * q + 4 * hh + (i >> 2);
+    const int c = c0 + 16 * ch + 4 * (i & 3);
+    const int off = r * RS + 8 * ((c >> 3) ^ (r & SWZ)) + (c & 7);
+    const bf16x4 p = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4)(t + off));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[4 * q + e] = p[e];
+  }
+  return v;
+}
 
-// K <= 256 (KQ = Kp / 32 <= 8): workgroup = NW waves x 32 chains (NW = 4
-// for 128-chain tiles, 2 for batches of at most 64 chains).  Each 32-row tile of X (both layouts, hi and lo) is staged once
-// per workgroup in LDS, straight from global memory (global_load_lds), and
-// read by all four waves, so X crosses L2 once per 128 chains.  A software
-// pipeline over tiles: while the link and adjoint of tile t run, the
-// forward MFMAs of tile t + 1 (independent of them) interleave with the
-// link's VALU work, X1 of tile t + 2 and X2 of tile t + 1 stream in; two
-// buffers per layout suffice.  v_mfma_f32_32x32x16_bf16 operands (lane l,
-// j < 8): A[l % 32][8 (l / 32) + j], B[8 (l / 32) + j][l % 32]; result
-// d[j] = D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
-//   forward  H (32 rows x 32 chains) = X1 tile (A, from LDS) . B (registers)
+// K <= 256 (KQ = Kp / 32 <= 8): workgroup = NW waves x 32 chains.  Per
+// 32-row tile a wave forms
+//   forward  H (32 rows x 32 chains) = X tile (A, LDS) . B (registers):
 //            lane l holds chain l % 32 and rows 8 (j/4) + 4 (l/32) + j%4
-//   adjoint  G^T (32 chains x 32 k) += R^T . X: the contraction runs over
-//            rows in the order the forward left them in the lane (step st
-//            uses the lane's own r[8st .. 8st+7]), and the X operand takes
-//            the same rows from the 8-row-packed tile: rows 16st + 4 (l/32)
-//            + 0..3 of groups 2st and 2st+1 (two 8-byte LDS reads).
+//   link     r and the row terms on the VALU (FP32)
+//   adjoint  G^T (32 chains x 32 k) += R^T . X: step st contracts the
+//            lane's own r[8st .. 8st+7] with the same rows of X
+// v_mfma_f32_32x32x16_bf16 operands (lane l, j < 8): A[l % 32][8 (l / 32)
+// + j], B[8 (l / 32) + j][l % 32]; result d[j] = D[8 (j / 4) + 4 (l / 32) +
+// j % 4][l % 32].
+template <int KQ, int FAM, int NW>
+__device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const float* Y,
+                                        int64_t t, int lane, const bf16x8 (&bh)[2 * KQ],
+                                        const bf16x8 (&bl)[2 * KQ], const bf16x8 (&bm)[2 * KQ],
+                                        float alpha, f32x16 (&g)[KQ], double& lp, double& sr,
+                                        float& nf) {
+  constexpr int Kp = 32 * KQ;
+  constexpr int S16 = 2 * KQ;
+  constexpr int CPR = Kp / 8;
+  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int TE = 32 * Kp;
+  // lane-derived address terms are recomputed per tile rather than kept
+  // live across the tile loop next to the coefficients and accumulators
+  int ln = lane;
+  asm volatile("" : "+v"(ln));
+  const int li = ln & 31, h = ln >> 5;
+  lane = ln;
+  // forward: the operands of step s + 1 are read while step s's products
+  // run (one step of prefetch)
+  auto fwd_off = [&](int s) { return 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ))); };
+  f32x16 H;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) H[j] = 0.f;
+  bf16x8 fh = *reinterpret_cast<const bf16x8*>(X + fwd_off(0));
+  bf16x8 fl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(0));
+#pragma unroll
+  for (int s = 0; s < S16; ++s) {
+    bf16x8 nh = fh, nl = fl;
+    if (s + 1 < S16) {
+      nh = *reinterpret_cast<const bf16x8*>(X + fwd_off(s + 1));
+      nl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(s + 1));
+    }
+    H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bh[s], H, 0, 0, 0);
+    H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bl[s], H, 0, 0, 0);
+    H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, bh[s], H, 0, 0, 0);
+    H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bm[s], H, 0, 0, 0);
+    fh = nh;
+    fl = nl;
+  }
+  // adjoint operands of (st, kb) = (0, 0), read while the link runs
+  bf16x8 ah = tr_operand<Kp, SWZ>(X, 0, 0, lane);
+  bf16x8 al = tr_operand<Kp, SWZ>(X + TE, 0, 0, lane);
+  const int64_t n0 = t * 32;
+  float lpt = 0.f, srt = 0.f;
+#pragma unroll
+  for (int st = 0; st < 2; ++st) {
+    bf16x8 rh, rl;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = 8 * st + jj;
+      const int row = 8 * (j >> 2) + 4 * h + (j & 3);
+      const float yv = Y[row];
+      const float eta = H[j] + alpha;
+      float term, rr;
+      if (FAM == SX_LOGIT) {
+        // rows past N carry sign 0: term = r = 0
+        logit_row_fast(eta, yv, &term, &rr);
+      } else {
+        rr = n0 + row < a.N ? yv - eta : 0.f;
+        term = rr * rr;
+      }
+      // NaN once any eta is infinite or NaN; a row past N has eta =
+      // alpha, non-finite only if every row's is
+      nf = fmaf(eta, 0.f, nf);
+      lpt += term;
+      srt += rr;
+      __bf16 hh, ll;
+      split(rr, &hh, &ll);
+      rh[jj] = hh;
+      rl[jj] = ll;
+    }
+    // one (st, kb) of prefetch: the operands of the next column block (or
+    // of the next half's first) are read while this block's products run
+#pragma unroll
+    for (int kb = 0; kb < KQ; ++kb) {
+      bf16x8 nh = ah, nl = al;
+      const int nst = kb + 1 < KQ ? st : st + 1, nkb = kb + 1 < KQ ? kb + 1 : 0;
+      if (nst < 2) {
+        nh = tr_operand<Kp, SWZ>(X, 16 * nst, 32 * nkb, lane);
+        nl = tr_operand<Kp, SWZ>(X + TE, 16 * nst, 32 * nkb, lane);
+      }
+      g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, ah, g[kb], 0, 0, 0);
+      g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, al, g[kb], 0, 0, 0);
+      g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, ah, g[kb], 0, 0, 0);
+      ah = nh;
+      al = nl;
+    }
+  }
+  lp += static_cast<double>(lpt);
+  sr += static_cast<double>(srt);
+}
+
 template <int KQ, int FAM, int NW>
 __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   static_assert(KQ >= 1 && KQ <= 8, "glm_mfma32 handles 32 <= Kp <= 256");
   static_assert(NW == 2 || NW == 4, "2 or 4 waves per workgroup");
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;          // forward k-steps of 16
-  constexpr int CPR = Kp / 8;          // 16-byte chunks per X1 row
-  // XOR swizzle of X1 chunks: the largest power of two (<= 16) dividing
-  // CPR, so c ^ (r & SWZ) stays inside the row
+  constexpr int CPR = Kp / 8;          // 16-byte chunks per row
   constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
   constexpr int TE = 32 * Kp;          // elements of one tile array
   constexpr int BPA = Kp / 16;         // 1 KB blocks per tile array
-  __shared__ __attribute__((aligned(16))) __bf16 x1[2][2][TE];  // [buffer][hi, lo]
-  __shared__ __attribute__((aligned(16))) __bf16 x2[2][2][TE];
-  __shared__ float ys[2][32];  // y of the tile's rows (+-1 for the logit)
+  constexpr int NBW = 2 * BPA / NW;    // staging blocks per wave and tile (hi + lo)
+  // two tile buffers, hi then lo; separate objects, so a staging write into
+  // one is not taken to alias the reads of the other
+  __shared__ __attribute__((aligned(16))) __bf16 xa[2 * TE];
+  __shared__ __attribute__((aligned(16))) __bf16 xb[2 * TE];
+  __shared__ __attribute__((aligned(16))) float ya[32];
+  __shared__ __attribute__((aligned(16))) float yb[32];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -142,12 +286,13 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   const int cblk = widx % nCB;
   const int cbase = cblk * 32 * NW + 32 * w;  // first chain of this wave
   const int chain = cbase + li;
-  bf16x8 bh[S16], bl[S16];
+  bf16x8 bh[S16], bl[S16], bm[S16];
 #pragma unroll
   for (int s = 0; s < S16; ++s) {
     const int64_t off = static_cast<int64_t>(chain) * Kp + 16 * s + 8 * h;
     bh[s] = *reinterpret_cast<const bf16x8*>(a.Bh + off);
     bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
+    bm[s] = *reinterpret_cast<const bf16x8*>(a.Bm + off);
   }
   const float alpha = a.alpha[chain];
   f32x16 g[KQ];
@@ -160,207 +305,46 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
 
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
-  const __bf16* X1h = reinterpret_cast<const __bf16*>(a.X1h);
-  const __bf16* X1l = reinterpret_cast<const __bf16*>(a.X1l);
-  const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
-  const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
-  double yraw = 0.0;  // y of a staged row (tid < 32): converted when committed
-  int yraw_i = 0;
-  bool yvalid = false;
-  float ystage = 0.f;  // (direct staging: converted at once)
-  // Staging.  Up to K = 192 (KQ <= 6) through registers: a tile's X1 (X2,
-  // with its y) is loaded with ordinary 16-byte global loads at the top of
-  // an iteration and written to its LDS buffer at the end, so the loads
-  // have the iteration's compute to land and the only wait for them is at
-  // those writes.  Above, where the loaded blocks would not fit beside B
-  // hi/lo and the gradient accumulators, straight from global memory to LDS
-  // (global_load_lds), waited for before the barrier (the compiler also
-  // waits for them before the next LDS read, so these do not overlap the
-  // tile's compute).  One layout's tile is 2 BPA blocks of 1 KB (hi, then
-  // lo); wave w moves blocks w, w + NW, ... (4 KQ / NW of them), lane l 16
-  // bytes at block offset 16 l.  X1 rows are stored with chunk c of row r at
-  // c ^ (r & SWZ), so the forward's row-per-lane reads spread over the
-  // banks; X2 is copied as is.  y is loaded raw and converted when
-  // committed, so that no early use of its register forces a wait for it
-  // (and for every load issued before it).
-  constexpr bool REG = KQ <= 6;
-  constexpr int NBW = 4 * KQ / NW;     // staging blocks per wave and layout
-  u32x4 r1[REG ? NBW : 1], r2[REG ? NBW : 1];
-#define SX_LOAD_X1(T, BUF)                                                       \
+  const __bf16* Xh = reinterpret_cast<const __bf16*>(a.Xh);
+  const __bf16* Xl = reinterpret_cast<const __bf16*>(a.Xl);
+  // Staging of tile T into buffer XB / YB: wave w moves 1 KB blocks w,
+  // w + NW, ... of the tile (hi blocks, then lo), lane l 16 bytes at block
+  // offset 16 l, reading chunk (p % CPR) ^ (r & SWZ) of row r = p / CPR
+  // for LDS chunk p; 8 lanes of wave 0 move the tile's 32 y values.
+#define SX_STAGE(T, XB, YB)                                                      \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {           \
+    int ln_ = lane; /* recomputed per stage, not kept live */                    \
+    asm volatile("" : "+v"(ln_));                                                \
+    _Pragma("unroll") for (int i = 0; i < NBW; ++i) {                            \
       const int b_ = w + NW * i;                                                 \
-      const int p_ = (b_ % BPA) * 64 + lane;                                     \
+      const int bb_ = b_ % BPA;                                                  \
+      const int p_ = bb_ * 64 + ln_;                                             \
       const int r_ = p_ / CPR;                                                   \
       const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
-      const __bf16* g_ = (b_ < BPA ? X1h : X1l) + base_ + 8 * src_;              \
-      if constexpr (REG)                                                         \
-        r1[i] = *reinterpret_cast<const u32x4*>(g_);                             \
-      else                                                                       \
-        __builtin_amdgcn_global_load_lds(g_,                                     \
-            (lds_ptr)(&x1[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);          \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + base_ + 8 * src_,  \
+          (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
+    if (w == 0 && lane < 8)                                                      \
+      __builtin_amdgcn_global_load_lds(a.yf + (T) * 32 + 4 * lane, (lds_ptr)(YB), \
+                                       16, 0, 0);                                \
   }
-#define SX_LOAD_X2(T, BUF)                                                       \
-  {                                                                              \
-    const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {           \
-      const int b_ = w + NW * i;                                                 \
-      const int p_ = (b_ % BPA) * 64 + lane;                                     \
-      const __bf16* g_ = (b_ < BPA ? X2h : X2l) + base_ + 8 * p_;                \
-      if constexpr (REG)                                                         \
-        r2[i] = *reinterpret_cast<const u32x4*>(g_);                             \
-      else                                                                       \
-        __builtin_amdgcn_global_load_lds(g_,                                     \
-            (lds_ptr)(&x2[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);          \
-    }                                                                            \
-    if (tid < 32) {                                                              \
-      const int64_t n_ = (T) * 32 + tid;                                         \
-      yvalid = n_ < a.N;                                                         \
-      const int64_t m_ = yvalid ? n_ : 0;                                        \
-      if (FAM == SX_LOGIT)                                                       \
-        yraw_i = static_cast<const int8_t*>(a.y)[m_];                            \
-      else                                                                       \
-        yraw = static_cast<const double*>(a.y)[m_];                              \
-      if constexpr (!REG)                                                        \
-        ystage = !yvalid ? 0.f : FAM == SX_LOGIT ? (yraw_i ? 1.0f : -1.0f)      \
-                                                 : static_cast<float>(yraw);     \
-    }                                                                            \
-  }
-#define SX_COMMIT_X1(BUF)                                                        \
-  {                                                                              \
-    if constexpr (REG) {                                                         \
-      _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {         \
-        const int b_ = w + NW * i;                                               \
-        *reinterpret_cast<u32x4*>(&x1[BUF][b_ / BPA][(b_ % BPA) * 512 + 8 * lane]) = r1[i]; \
-      }                                                                          \
-    }                                                                            \
-  }
-  // X2 and the tile's y; the direct loads of both layouts are waited for
-#define SX_COMMIT_X2(BUF)                                                        \
-  {                                                                              \
-    if constexpr (REG) {                                                         \
-      _Pragma("clang loop unroll(full)") for (int i = 0; i < NBW; ++i) {         \
-        const int b_ = w + NW * i;                                               \
-        *reinterpret_cast<u32x4*>(&x2[BUF][b_ / BPA][(b_ % BPA) * 512 + 8 * lane]) = r2[i]; \
-      }                                                                          \
-    } else {                                                                     \
-      __builtin_amdgcn_s_waitcnt(0);                                             \
-    }                                                                            \
-    if (tid < 32) {                                                              \
-      float v_ = ystage;                                                         \
-      if constexpr (REG)                                                         \
-        v_ = !yvalid ? 0.f : FAM == SX_LOGIT ? (yraw_i ? 1.0f : -1.0f)          \
-                                             : static_cast<float>(yraw);         \
-      ys[BUF][tid] = v_;                                                         \
-    }                                                                            \
-  }
-#define SX_FORWARD(BUF, ACC)                                                     \
-  {                                                                              \
-    _Pragma("unroll") for (int j = 0; j < 16; ++j) ACC[j] = 0.f;                 \
-    _Pragma("unroll") for (int s = 0; s < S16; ++s) {                            \
-      const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));               \
-      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1[BUF][0][off]);      \
-      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1[BUF][1][off]);      \
-      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], ACC, 0, 0, 0);    \
-      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], ACC, 0, 0, 0);    \
-      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], ACC, 0, 0, 0);    \
-    }                                                                            \
-  }
-  f32x16 hcur, hnext;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
-  // prologue: X1 and X2 of tile t0, X1 of tile t0 + 1; forward of tile t0
-  if (t0 < t1) {
-    SX_LOAD_X1(t0, 0);
-    SX_COMMIT_X1(0);
-    if (t0 + 1 < t1) {
-      SX_LOAD_X1(t0 + 1, 1);
-      SX_COMMIT_X1(1);
-    }
-    SX_LOAD_X2(t0, 0);
-    SX_COMMIT_X2(0);
-  }
+  if (t0 < t1) SX_STAGE(t0, xa, ya);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (t0 < t1) SX_FORWARD(0, hcur);
-  for (int64_t t = t0; t < t1; ++t) {
-    const int cur = static_cast<int>((t - t0) & 1), nxt = cur ^ 1;
-    // x1[cur] held X1 of tile t (its forward ran an iteration ago) and
-    // x2[nxt] X2 of tile t - 1: both free since the last barrier.  The
-    // forward of the tile after the last reads a stale buffer; its result
-    // is discarded.
-    if (t + 2 < t1) SX_LOAD_X1(t + 2, cur);
-    if (t + 1 < t1) SX_LOAD_X2(t + 1, nxt);
-    SX_FORWARD(nxt, hnext);
-    // link and adjoint of tile t in two halves of the lane's 16 rows: half
-    // st holds elements j = 8 st .. 8 st + 7 (rows 8 (j/4) + 4 h + j%4),
-    // the rows the adjoint's step st contracts.  Rows past N are masked.
-    // The tile's sums are FP32 and are added to the FP64 totals per tile.
-    const int64_t n0 = t * 32;
-    float lpt = 0.f, srt = 0.f;
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-      bf16x8 rh, rl;
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) {
-        const int j = 8 * st + jj;
-        const int row = 8 * (j >> 2) + 4 * h + (j & 3);
-        const float yv = ys[cur][row];
-        const float eta = hcur[j] + alpha;
-        float term, rr;
-        if (FAM == SX_LOGIT) {
-          // rows past N carry sign 0: term = r = 0
-          logit_row_fast(eta, yv, &term, &rr);
-        } else {
-          rr = n0 + row < a.N ? yv - eta : 0.f;
-          term = rr * rr;
-        }
-        // NaN once any eta is infinite or NaN; a row past N has eta =
-        // alpha, non-finite only if every row's is
-        nf = fmaf(eta, 0.f, nf);
-        lpt += term;
-        srt += rr;
-        __bf16 hh, ll;
-        split(rr, &hh, &ll);
-        rh[jj] = hh;
-        rl[jj] = ll;
-      }
-#pragma unroll
-      for (int kb = 0; kb < KQ; ++kb) {
-        const int k = 32 * kb + li;
-        const int o0 = ((2 * st) * Kp + k) * 8 + 4 * h;
-        const int o1 = ((2 * st + 1) * Kp + k) * 8 + 4 * h;
-        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o0]);
-        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o1]);
-        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o0]);
-        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o1]);
-        bf16x8 xh, xl;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          xh[j] = ah0[j];
-          xh[4 + j] = ah1[j];
-          xl[j] = al0[j];
-          xl[4 + j] = al1[j];
-        }
-        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xh, g[kb], 0, 0, 0);
-        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, xl, g[kb], 0, 0, 0);
-        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, xh, g[kb], 0, 0, 0);
-      }
-    }
-    lp += static_cast<double>(lpt);
-    sr += static_cast<double>(srt);
-
-    if (t + 2 < t1) SX_COMMIT_X1(cur);
-    if (t + 1 < t1) SX_COMMIT_X2(nxt);
+  for (int64_t t = t0; t < t1; t += 2) {
+    // xb held tile t - 1, read before the last barrier
+    if (t + 1 < t1) SX_STAGE(t + 1, xb, yb);
+    mx_tile<KQ, FAM, NW>(a, xa, ya, t, lane, bh, bl, bm, alpha, g, lp, sr, nf);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    hcur = hnext;
+    if (t + 1 >= t1) break;
+    if (t + 2 < t1) SX_STAGE(t + 2, xa, ya);
+    mx_tile<KQ, FAM, NW>(a, xb, yb, t + 1, lane, bh, bl, bm, alpha, g, lp, sr, nf);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
-#undef SX_FORWARD
-#undef SX_LOAD_X1
-#undef SX_LOAD_X2
-#undef SX_COMMIT_X1
-#undef SX_COMMIT_X2
+#undef SX_STAGE
   // g[kb][j] = G^T[chain cbase + 8 (j/4) + 4 h + j%4][k = 32 kb + li]
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll
@@ -383,15 +367,216 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   }
 }
 
+// 128 < K <= 256, KQ even: the coefficients of a 32-chain block over the
+// full K (three terms) and its gradient accumulators do not fit one wave's
+// registers beside the operands, so each chain block is split over a pair
+// of waves by k: wave (cb, kh) holds the coefficients and accumulates the
+// gradient of columns [kh Kp/2, (kh+1) Kp/2).  Per tile both waves form
+// their half of the forward product, exchange the 32 x 32 partials through
+// LDS (each then holds the full H; own + other is the same sum in both),
+// run the link on all 32 rows (the lp / residual sums are kept by kh = 0
+// only), and contract R with their half of the columns.  Workgroup = NCB
+// chain blocks x 2 halves (NCB = 4: 8 waves, two per SIMD; NCB = 2 for
+// batches of at most 64 chains).
+template <int KQ, int FAM, int NCB>
+__global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
+  static_assert(KQ % 2 == 0 && KQ >= 2 && KQ <= 8, "pairs for Kp <= 256, KQ even");
+  static_assert(NCB == 2 || NCB == 4, "2 or 4 chain blocks per workgroup");
+  constexpr int NW = 2 * NCB;          // waves per workgroup
+  constexpr int KH = KQ / 2;           // 32-column blocks per half
+  constexpr int Kp = 32 * KQ;
+  constexpr int CPR = Kp / 8;
+  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int TE = 32 * Kp;
+  constexpr int BPA = Kp / 16;
+  constexpr int NBW = 2 * BPA / NW;
+  static_assert((2 * BPA) % NW == 0, "staging blocks per wave");
+  __shared__ __attribute__((aligned(16))) __bf16 xa[2 * TE];
+  __shared__ __attribute__((aligned(16))) __bf16 xb[2 * TE];
+  __shared__ __attribute__((aligned(16))) float ya[32];
+  __shared__ __attribute__((aligned(16))) float yb[32];
+  // forward partials: [chain block][half][4 x 4 elements][lane]
+  __shared__ __attribute__((aligned(16))) float4 hx[NCB][2][4][64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int cbw = w >> 1, kh = w & 1;
+  const int li = lane & 31, h = lane >> 5;
+  const int nCB = a.Cp / (32 * NCB);
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7, widx = bid >> 3;
+  const int spl = (widx / nCB) * 8 + xcd;
+  const int cblk = widx % nCB;
+  const int cbase = cblk * 32 * NCB + 32 * cbw;  // first chain of this pair
+  const int chain = cbase + li;
+  const int s0 = kh * KQ;                   // first k-step of 16 of this half
+  bf16x8 bh[KQ], bl[KQ], bm[KQ];
+#pragma unroll
+  for (int s = 0; s < KQ; ++s) {
+    const int64_t off = static_cast<int64_t>(chain) * Kp + 16 * (s0 + s) + 8 * h;
+    bh[s] = *reinterpret_cast<const bf16x8*>(a.Bh + off);
+    bl[s] = *reinterpret_cast<const bf16x8*>(a.Bl + off);
+    bm[s] = *reinterpret_cast<const bf16x8*>(a.Bm + off);
+  }
+  const float alpha = a.alpha[chain];
+  f32x16 g[KH];
+#pragma unroll
+  for (int i = 0; i < KH; ++i)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) g[i][j] = 0.f;
+  double lp = 0.0, sr = 0.0;
+  float nf = 0.f;
+  const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
+  const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
+  const __bf16* Xh = reinterpret_cast<const __bf16*>(a.Xh);
+  const __bf16* Xl = reinterpret_cast<const __bf16*>(a.Xl);
+#define SX_STAGE(T, XB, YB)                                                      \
+  {                                                                              \
+    const int64_t base_ = (T) * 32 * Kp;                                         \
+    int ln_ = lane; /* recomputed per stage, not kept live */                    \
+    asm volatile("" : "+v"(ln_));                                                \
+    _Pragma("unroll") for (int i = 0; i < NBW; ++i) {                            \
+      const int b_ = w + NW * i;                                                 \
+      const int bb_ = b_ % BPA;                                                  \
+      const int p_ = bb_ * 64 + ln_;                                             \
+      const int r_ = p_ / CPR;                                                   \
+      const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + base_ + 8 * src_,  \
+          (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
+    }                                                                            \
+    if (w == 0 && lane < 8)                                                      \
+      __builtin_amdgcn_global_load_lds(a.yf + (T) * 32 + 4 * lane, (lds_ptr)(YB), \
+                                       16, 0, 0);                                \
+  }
+  auto tile = [&](const __bf16* X, const float* Y, int64_t t) {
+    auto fwd_off = [&](int s) { return 8 * (li * CPR + ((2 * (s0 + s) + h) ^ (li & SWZ))); };
+    f32x16 H;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) H[j] = 0.f;
+    bf16x8 fh = *reinterpret_cast<const bf16x8*>(X + fwd_off(0));
+    bf16x8 fl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(0));
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) {
+      bf16x8 nh = fh, nl = fl;
+      if (s + 1 < KQ) {
+        nh = *reinterpret_cast<const bf16x8*>(X + fwd_off(s + 1));
+        nl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(s + 1));
+      }
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bh[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bl[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, bh[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bm[s], H, 0, 0, 0);
+      fh = nh;
+      fl = nl;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      hx[cbw][kh][q][lane] = make_float4(H[4 * q], H[4 * q + 1], H[4 * q + 2], H[4 * q + 3]);
+    // the adjoint's first operands, read across the exchange
+    bf16x8 ah = tr_operand<Kp, SWZ>(X, 0, 32 * (kh * KH), lane);
+    bf16x8 al = tr_operand<Kp, SWZ>(X + TE, 0, 32 * (kh * KH), lane);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 o = hx[cbw][kh ^ 1][q][lane];
+      H[4 * q] += o.x;
+      H[4 * q + 1] += o.y;
+      H[4 * q + 2] += o.z;
+      H[4 * q + 3] += o.w;
+    }
+    const int64_t n0 = t * 32;
+    float lpt = 0.f, srt = 0.f;
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      bf16x8 rh, rl;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int j = 8 * st + jj;
+        const int row = 8 * (j >> 2) + 4 * h + (j & 3);
+        const float yv = Y[row];
+        const float eta = H[j] + alpha;
+        float term, rr;
+        if (FAM == SX_LOGIT) {
+          logit_row_fast(eta, yv, &term, &rr);
+        } else {
+          rr = n0 + row < a.N ? yv - eta : 0.f;
+          term = rr * rr;
+        }
+        nf = fmaf(eta, 0.f, nf);
+        lpt += term;
+        srt += rr;
+        __bf16 hh, ll;
+        split(rr, &hh, &ll);
+        rh[jj] = hh;
+        rl[jj] = ll;
+      }
+#pragma unroll
+      for (int kb = 0; kb < KH; ++kb) {
+        bf16x8 nh = ah, nl = al;
+        const int nst = kb + 1 < KH ? st : st + 1, nkb = kb + 1 < KH ? kb + 1 : 0;
+        if (nst < 2) {
+          nh = tr_operand<Kp, SWZ>(X, 16 * nst, 32 * (kh * KH + nkb), lane);
+          nl = tr_operand<Kp, SWZ>(X + TE, 16 * nst, 32 * (kh * KH + nkb), lane);
+        }
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, ah, g[kb], 0, 0, 0);
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, al, g[kb], 0, 0, 0);
+        g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, ah, g[kb], 0, 0, 0);
+        ah = nh;
+        al = nl;
+      }
+    }
+    if (kh == 0) {
+      lp += static_cast<double>(lpt);
+      sr += static_cast<double>(srt);
+    }
+  };
+  if (t0 < t1) SX_STAGE(t0, xa, ya);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int64_t t = t0; t < t1; t += 2) {
+    if (t + 1 < t1) SX_STAGE(t + 1, xb, yb);
+    tile(xa, ya, t);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (t + 1 >= t1) break;
+    if (t + 2 < t1) SX_STAGE(t + 2, xa, ya);
+    tile(xb, yb, t + 1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+#undef SX_STAGE
+  const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
+#pragma unroll
+  for (int kb = 0; kb < KH; ++kb)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int ci = cbase + 8 * (j >> 2) + 4 * h + (j & 3);
+      a.gpart[(pbase + ci) * Kp + 32 * (kh * KH + kb) + li] = g[kb][j];
+    }
+  if (kh == 0) {
+    lp += __shfl_xor(lp, 32, 64);
+    sr += __shfl_xor(sr, 32, 64);
+    nf += __shfl_xor(nf, 32, 64);
+    const double bad = nf == nf ? 0.0 : 1.0;
+    if (h == 0) {
+      double* sp = a.spart + (pbase + chain) * 3;
+      sp[0] = lp;
+      sp[1] = sr;
+      sp[2] = bad;
+    }
+  }
+}
+
 // 256 < K <= 512 (KQ = Kp / 32 in {10, 12, 14, 16}): B hi/lo of 32 chains
 // over the full K fill 256 registers, so the gradient accumulators cannot
-// stay beside them; the evaluation is split in two launches.  This one is
-// glm_mfma32's forward and link (same workgroup, grid, X1 staging and
-// software pipeline): it writes the residual r of every row and chain as
-// the BF16 hi/lo MFMA operands the adjoint will use, lane by lane (tile t,
-// 32-chain group g, half st: the lane's bf16x8 at
-// R[((t * Cp / 32 + g) * 2 + st) * 64 + lane]), and the row terms per
-// split.  glm_mfma32_adj then forms G^T = R^T X per block of 256 k.
+// stay beside them; the evaluation is split in two launches, and B keeps
+// two BF16 terms here (a third would not fit either).  This one is
+// glm_mfma32's forward and link (same workgroup, grid and tile staging):
+// it writes the residual r of every row and chain as the BF16 hi/lo MFMA
+// operands the adjoint will use, lane by lane (tile t, 32-chain group g,
+// half st: the lane's bf16x8 at R[((t * Cp / 32 + g) * 2 + st) * 64 +
+// lane]), and the row terms per split.  glm_mfma32_adj then forms
+// G^T = R^T X per block of 256 k.
 template <int KQ, int FAM, int NW>
 __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
   static_assert(KQ > 8 && KQ <= 16 && KQ % 2 == 0, "glm_mfma32_fwd handles 256 < Kp <= 512, Kp % 64 == 0");
@@ -402,8 +587,11 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
   constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
   constexpr int TE = 32 * Kp;
   constexpr int BPA = Kp / 16;
-  __shared__ __attribute__((aligned(16))) __bf16 x1[2][2][TE];  // [buffer][hi, lo]
-  __shared__ float ys[2][32];
+  constexpr int NBW = 2 * BPA / NW;
+  __shared__ __attribute__((aligned(16))) __bf16 xa[2 * TE];
+  __shared__ __attribute__((aligned(16))) __bf16 xb[2 * TE];
+  __shared__ __attribute__((aligned(16))) float ya[32];
+  __shared__ __attribute__((aligned(16))) float yb[32];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -428,70 +616,41 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
   float nf = 0.f;  // sum of 0 * eta over the lane's rows
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
-  const __bf16* X1h = reinterpret_cast<const __bf16*>(a.X1h);
-  const __bf16* X1l = reinterpret_cast<const __bf16*>(a.X1l);
+  const __bf16* Xh = reinterpret_cast<const __bf16*>(a.Xh);
+  const __bf16* Xl = reinterpret_cast<const __bf16*>(a.Xl);
   bf16x8* Rh = reinterpret_cast<bf16x8*>(a.Rh);
   bf16x8* Rl = reinterpret_cast<bf16x8*>(a.Rl);
-  float ystage = 0.f;
-#define SX_STAGE_X1(T, BUF)                                                      \
+#define SX_STAGE(T, XB, YB)                                                      \
   {                                                                              \
     const int64_t base_ = (T) * 32 * Kp;                                         \
-    _Pragma("unroll") for (int i = 0; i < 4 * KQ / NW; ++i) {                    \
+    int ln_ = lane; /* recomputed per stage, not kept live */                    \
+    asm volatile("" : "+v"(ln_));                                                \
+    _Pragma("unroll") for (int i = 0; i < NBW; ++i) {                            \
       const int b_ = w + NW * i;                                                 \
-      const int p_ = (b_ % BPA) * 64 + lane;                                     \
+      const int bb_ = b_ % BPA;                                                  \
+      const int p_ = bb_ * 64 + ln_;                                             \
       const int r_ = p_ / CPR;                                                   \
       const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
-      __builtin_amdgcn_global_load_lds((b_ < BPA ? X1h : X1l) + base_ + 8 * src_, \
-          (lds_ptr)(&x1[BUF][b_ / BPA][(b_ % BPA) * 512]), 16, 0, 0);            \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + base_ + 8 * src_,  \
+          (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
+    if (w == 0 && lane < 8)                                                      \
+      __builtin_amdgcn_global_load_lds(a.yf + (T) * 32 + 4 * lane, (lds_ptr)(YB), \
+                                       16, 0, 0);                                \
   }
-#define SX_STAGE_Y(T)                                                            \
-  {                                                                              \
-    if (tid < 32) {                                                              \
-      const int64_t n_ = (T) * 32 + tid;                                         \
-      float v_ = 0.f;                                                            \
-      if (n_ < a.N) {                                                            \
-        if (FAM == SX_LOGIT)                                                     \
-          v_ = static_cast<const int8_t*>(a.y)[n_] ? 1.0f : -1.0f;               \
-        else                                                                     \
-          v_ = static_cast<float>(static_cast<const double*>(a.y)[n_]);          \
-      }                                                                          \
-      ystage = v_;                                                               \
-    }                                                                            \
-  }
-#define SX_STAGE_DONE(YBUF, WITH_Y)                                              \
-  {                                                                              \
-    __builtin_amdgcn_s_waitcnt(0);                                               \
-    if ((WITH_Y) && tid < 32) ys[YBUF][tid] = ystage;                            \
-  }
-#define SX_FORWARD(BUF, ACC)                                                     \
-  {                                                                              \
-    _Pragma("unroll") for (int j = 0; j < 16; ++j) ACC[j] = 0.f;                 \
-    _Pragma("unroll") for (int s = 0; s < S16; ++s) {                            \
-      const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));               \
-      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&x1[BUF][0][off]);      \
-      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&x1[BUF][1][off]);      \
-      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], ACC, 0, 0, 0);    \
-      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], ACC, 0, 0, 0);    \
-      ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], ACC, 0, 0, 0);    \
-    }                                                                            \
-  }
-  f32x16 hcur, hnext;
+  auto tile = [&](const __bf16* X, const float* Y, int64_t t) {
+    f32x16 H;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) hcur[j] = hnext[j] = 0.f;
-  if (t0 < t1) {
-    SX_STAGE_X1(t0, 0);
-    if (t0 + 1 < t1) SX_STAGE_X1(t0 + 1, 1);
-    SX_STAGE_Y(t0);
-    SX_STAGE_DONE(0, true);
-  }
-  __syncthreads();
-  if (t0 < t1) SX_FORWARD(0, hcur);
-  for (int64_t t = t0; t < t1; ++t) {
-    const int cur = static_cast<int>((t - t0) & 1), nxt = cur ^ 1;
-    if (t + 2 < t1) SX_STAGE_X1(t + 2, cur);
-    if (t + 1 < t1) SX_STAGE_Y(t + 1);
-    SX_FORWARD(nxt, hnext);
+    for (int j = 0; j < 16; ++j) H[j] = 0.f;
+#pragma unroll
+    for (int s = 0; s < S16; ++s) {
+      const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(X + off);
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(X + TE + off);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], H, 0, 0, 0);
+    }
     const int64_t n0 = t * 32;
     float lpt = 0.f, srt = 0.f;
 #pragma unroll
@@ -501,18 +660,15 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
       for (int jj = 0; jj < 8; ++jj) {
         const int j = 8 * st + jj;
         const int row = 8 * (j >> 2) + 4 * h + (j & 3);
-        const float yv = ys[cur][row];
-        const float eta = hcur[j] + alpha;
+        const float yv = Y[row];
+        const float eta = H[j] + alpha;
         float term, rr;
         if (FAM == SX_LOGIT) {
-          // rows past N carry sign 0: term = r = 0
           logit_row_fast(eta, yv, &term, &rr);
         } else {
           rr = n0 + row < a.N ? yv - eta : 0.f;
           term = rr * rr;
         }
-        // NaN once any eta is infinite or NaN; a row past N has eta =
-        // alpha, non-finite only if every row's is
         nf = fmaf(eta, 0.f, nf);
         lpt += term;
         srt += rr;
@@ -527,15 +683,22 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
     }
     lp += static_cast<double>(lpt);
     sr += static_cast<double>(srt);
-
-    SX_STAGE_DONE(nxt, t + 1 < t1);
+  };
+  if (t0 < t1) SX_STAGE(t0, xa, ya);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int64_t t = t0; t < t1; t += 2) {
+    if (t + 1 < t1) SX_STAGE(t + 1, xb, yb);
+    tile(xa, ya, t);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    hcur = hnext;
+    if (t + 1 >= t1) break;
+    if (t + 2 < t1) SX_STAGE(t + 2, xa, ya);
+    tile(xb, yb, t + 1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
-#undef SX_FORWARD
-#undef SX_STAGE_X1
-#undef SX_STAGE_Y
-#undef SX_STAGE_DONE
+#undef SX_STAGE
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
   lp += __shfl_xor(lp, 32, 64);
   sr += __shfl_xor(sr, 32, 64);
@@ -552,17 +715,21 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
 // The adjoint of a 256 < K <= 512 evaluation for one block of k (KQA x 32
 // columns from column 32 kq0): G^T (32 chains x 32 k) += R^T X per tile,
 // as in glm_mfma32 (step st takes the lane's residuals of rows 8 (j/4) +
-// 4 h + j%4, j = 8 st .. 8 st + 7, and the same rows of the 8-row-packed
-// tile), with R read from the forward launch's output and the tile's
-// column block of X2 staged through LDS one tile ahead.
+// 4 h + j%4, j = 8 st .. 8 st + 7, and the same rows of X through the
+// transposing LDS read), with R read from the forward launch's output and
+// the tile's column block of X staged through LDS one tile ahead.
 template <int KQA, int NW>
 __global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
   static_assert(KQA >= 1 && KQA <= 8, "blocks of at most 256 columns");
   static_assert((4 * KQA) % NW == 0, "staging blocks per wave");
   constexpr int KR = 32 * KQA;         // columns of the block
+  constexpr int CPRB = KR / 8;         // 16-byte chunks per block row
+  constexpr int SWZ = (CPRB % 16 == 0 ? 16 : CPRB % 8 == 0 ? 8 : 4) - 1;
   constexpr int TE = 32 * KR;          // elements of one tile array
-  constexpr int NB = 2 * KQA;          // 1 KB staging blocks per tile array
-  __shared__ __attribute__((aligned(16))) __bf16 x2[2][2][TE];  // [buffer][hi, lo]
+  constexpr int BPA = 2 * KQA;         // 1 KB staging blocks per tile array
+  constexpr int NBW = 2 * BPA / NW;
+  __shared__ __attribute__((aligned(16))) __bf16 xa[2 * TE];
+  __shared__ __attribute__((aligned(16))) __bf16 xb[2 * TE];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -583,81 +750,59 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
     for (int j = 0; j < 16; ++j) g[i][j] = 0.f;
   const int64_t t0 = static_cast<int64_t>(spl) * a.tiles_per_split;
   const int64_t t1 = t0 + a.tiles_per_split < a.n_tiles ? t0 + a.tiles_per_split : a.n_tiles;
-  const __bf16* X2h = reinterpret_cast<const __bf16*>(a.X2h);
-  const __bf16* X2l = reinterpret_cast<const __bf16*>(a.X2l);
+  const __bf16* Xh = reinterpret_cast<const __bf16*>(a.Xh);
+  const __bf16* Xl = reinterpret_cast<const __bf16*>(a.Xl);
   const bf16x8* Rh = reinterpret_cast<const bf16x8*>(a.Rh);
   const bf16x8* Rl = reinterpret_cast<const bf16x8*>(a.Rl);
-  // staging: chunk p (16 bytes: the 8 rows of one column) of the tile's
-  // column block is group p / KR, column k0 + p % KR; wave w moves blocks
-  // w, w + NW, ... of 64 chunks per array (hi, then lo)
-#define SX_STAGE_X2B(T, BUF)                                                     \
+  // staging: LDS chunk p of an array is row p / CPRB of the tile, chunk
+  // (p % CPRB) ^ (row & SWZ) of the column block
+#define SX_STAGE(T, XB)                                                          \
   {                                                                              \
-    _Pragma("unroll") for (int i = 0; i < 2 * NB / NW; ++i) {                    \
+    _Pragma("unroll") for (int i = 0; i < NBW; ++i) {                            \
       const int b_ = w + NW * i;                                                 \
-      const int bb_ = b_ % NB;                                                   \
+      const int bb_ = b_ % BPA;                                                  \
       const int p_ = bb_ * 64 + lane;                                            \
-      const int64_t src_ = ((4 * (T) + p_ / KR) * Kp + k0 + p_ % KR) * 8;        \
-      __builtin_amdgcn_global_load_lds((b_ < NB ? X2h : X2l) + src_,             \
-          (lds_ptr)(&x2[BUF][b_ / NB][bb_ * 512]), 16, 0, 0);                    \
+      const int r_ = p_ / CPRB;                                                  \
+      const int64_t src_ = ((T) * 32 + r_) * Kp + k0 + 8 * ((p_ % CPRB) ^ (r_ & SWZ)); \
+      __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + src_,              \
+          (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
   }
-  bf16x8 rh[2], rl[2], nh[2], nl[2];
-  if (t0 < t1) {
-    SX_STAGE_X2B(t0, 0);
+  auto tile = [&](const __bf16* X, int64_t t) {
+    bf16x8 rh[2], rl[2];
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
-      const int64_t ri = ((t0 * nG + g32) * 2 + st) * 64 + lane;
+      const int64_t ri = ((t * nG + g32) * 2 + st) * 64 + lane;
       rh[st] = Rh[ri];
       rl[st] = Rl[ri];
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-  }
-  __syncthreads();
-  for (int64_t t = t0; t < t1; ++t) {
-    const int cur = static_cast<int>((t - t0) & 1), nxt = cur ^ 1;
-    // x2[nxt] held tile t - 1, read before the last barrier
-    if (t + 1 < t1) {
-      SX_STAGE_X2B(t + 1, nxt);
-#pragma unroll
-      for (int st = 0; st < 2; ++st) {
-        const int64_t ri = (((t + 1) * nG + g32) * 2 + st) * 64 + lane;
-        nh[st] = Rh[ri];
-        nl[st] = Rl[ri];
-      }
     }
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
 #pragma unroll
       for (int kb = 0; kb < KQA; ++kb) {
-        const int k = 32 * kb + li;
-        const int o0 = ((2 * st) * KR + k) * 8 + 4 * h;
-        const int o1 = ((2 * st + 1) * KR + k) * 8 + 4 * h;
-        const bf16x4 ah0 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o0]);
-        const bf16x4 ah1 = *reinterpret_cast<const bf16x4*>(&x2[cur][0][o1]);
-        const bf16x4 al0 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o0]);
-        const bf16x4 al1 = *reinterpret_cast<const bf16x4*>(&x2[cur][1][o1]);
-        bf16x8 xh, xl;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          xh[j] = ah0[j];
-          xh[4 + j] = ah1[j];
-          xl[j] = al0[j];
-          xl[4 + j] = al1[j];
-        }
+        const bf16x8 xh = tr_operand<KR, SWZ>(X, 16 * st, 32 * kb, lane);
+        const bf16x8 xl = tr_operand<KR, SWZ>(X + TE, 16 * st, 32 * kb, lane);
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[st], xh, g[kb], 0, 0, 0);
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[st], xl, g[kb], 0, 0, 0);
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[st], xh, g[kb], 0, 0, 0);
       }
     }
+  };
+  if (t0 < t1) SX_STAGE(t0, xa);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int64_t t = t0; t < t1; t += 2) {
+    if (t + 1 < t1) SX_STAGE(t + 1, xb);
+    tile(xa, t);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-      rh[st] = nh[st];
-      rl[st] = nl[st];
-    }
+    if (t + 1 >= t1) break;
+    if (t + 2 < t1) SX_STAGE(t + 2, xa);
+    tile(xb, t + 1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
-#undef SX_STAGE_X2B
+#undef SX_STAGE
   const int64_t pbase = static_cast<int64_t>(spl) * a.Cp;
 #pragma unroll
   for (int kb = 0; kb < KQA; ++kb)
@@ -759,12 +904,29 @@ hipError_t launch_kq32_nw(int family, const mx_args& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// chain tile: a batch of at most 64 chains (Cp = 64) gets two waves per
-// workgroup, larger batches four.  (One-wave workgroups of 32 chains were
-// measured slower than two-wave ones of 64: 0.274 vs 0.257 ms at N = 1e6,
-// K = 100.)
+template <int KQ, int NCB>
+hipError_t launch_pair_ncb(int family, const mx_args& a, hipStream_t s) {
+  if (a.splits % 8 != 0 || a.Cp % (32 * NCB) != 0) return hipErrorInvalidValue;
+  dim3 grid(a.splits * (a.Cp / (32 * NCB)));
+  if (family == SX_LOGIT)
+    hipLaunchKernelGGL((glm_mfma32_pair<KQ, SX_LOGIT, NCB>), grid, dim3(128 * NCB), 0, s, a);
+  else
+    hipLaunchKernelGGL((glm_mfma32_pair<KQ, SX_NORMAL, NCB>), grid, dim3(128 * NCB), 0, s, a);
+  return hipGetLastError();
+}
+
+// Kp <= 256.  Variant 0: one wave per 32-chain block (two waves per
+// workgroup for a batch of at most 64 chains (Cp = 64), four otherwise);
+// variant 1 (KQ even): a pair of waves per chain block, split by k.
 template <int KQ>
 hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
+  if constexpr (KQ % 2 == 0) {
+    if (a.variant == 1) {
+      if (a.Cp == 64) return launch_pair_ncb<KQ, 2>(family, a, s);
+      return launch_pair_ncb<KQ, 4>(family, a, s);
+    }
+  }
+  if (a.variant != 0) return hipErrorInvalidValue;
   if (a.Cp == 64) return launch_kq32_nw<KQ, 2>(family, a, s);
   return launch_kq32_nw<KQ, 4>(family, a, s);
 }
@@ -794,15 +956,21 @@ hipError_t launch_wide(int family, const mx_args& a, hipStream_t s) {
 
 }  // namespace
 
-hipError_t launch_split_x(const double* Xrm, uint16_t* X1h, uint16_t* X1l, uint16_t* X2h,
-                          uint16_t* X2l, int64_t N, int64_t K, int64_t Np, int64_t Kp,
-                          hipStream_t s) {
+hipError_t launch_split_x(const double* Xrm, uint16_t* Xh, uint16_t* Xl, int64_t N, int64_t K,
+                          int64_t Np, int64_t Kp, hipStream_t s) {
   const int64_t total = Np * Kp;
   if (total == 0) return hipSuccess;
   hipLaunchKernelGGL(split_x, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
-                     Xrm, reinterpret_cast<__bf16*>(X1h), reinterpret_cast<__bf16*>(X1l),
-                     reinterpret_cast<__bf16*>(X2h), reinterpret_cast<__bf16*>(X2l), N, K, Kp,
+                     Xrm, reinterpret_cast<__bf16*>(Xh), reinterpret_cast<__bf16*>(Xl), N, K, Kp,
                      total);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_y(const void* y, float* yf, int family, int64_t N, int64_t Np,
+                         hipStream_t s) {
+  if (Np == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_y, dim3(static_cast<unsigned>((Np + 255) / 256)), dim3(256), 0, s, y, yf,
+                     family, N, Np);
   return hipGetLastError();
 }
 
@@ -811,6 +979,7 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, c
   hipLaunchKernelGGL(prep_chains, dim3(a.Cp), dim3(256), 0, s, theta, D, C, pa.K, a.Kp,
                      reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bh)),
                      reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bl)),
+                     reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bm)),
                      const_cast<float*>(a.alpha));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

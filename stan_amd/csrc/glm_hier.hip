@@ -34,47 +34,105 @@ constexpr double kHalfLog2PiH = 0.91893853320467274178032973640562;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// FP64: a wave per chunk, rows on lanes; chains on grid.y.  A chunk's sums
-// are formed by one wave in a fixed order, so each chain's result does not
-// depend on the launch shape or on the other chains.
+// FP64: a wave per chunk, rows on lanes; chains on grid.y.  A chunk's rows
+// are contiguous in X (row-major), so each block of up to 64 rows is one
+// contiguous run of rows * K doubles: the wave loads it with every load
+// instruction covering 64 consecutive doubles (512 bytes), the loads of the
+// next block in flight while this one is computed, and places it in the
+// wave's LDS at row stride K | 1, from where lane r reads row r.  A chunk's
+// sums are formed by one wave in a fixed order (lane r takes rows r, r + 64,
+// ...; columns in order), so each chain's result depends neither on the
+// launch shape nor on the other chains.
 template <int KMAX>
 __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
+  constexpr int BS = 64 * (KMAX | 1);  // LDS doubles of one block
+  __shared__ double sx[4][BS + 1];     // + a sink for positions past the block
   __shared__ double sw[4][KMAX + 1];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int K = a.K;
+  const int KS = K | 1;
   const int c = blockIdx.y;
   const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
   const int64_t W = static_cast<int64_t>(gridDim.x) * 4;
-  for (int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w; ch < a.n_chunks; ch += W) {
+  // (K = 0, intercepts only: nothing is staged; no division by zero)
+  const int Kd = K > 0 ? K : 1;
+  const int dr = 64 / Kd, dk = 64 % Kd;
+  const int row0 = lane / Kd, col0 = lane % Kd;
+  double stg[KMAX];
+  int8_t ystg = 0;
+  // loads of rows [base, base + 64) of chunk ch into registers; element
+  // indices past the block are clamped to its last element (an in-bounds
+  // load) and their values never stored
+#define SX_LOAD_BLOCK(CH, BASE)                                                  \
+  {                                                                              \
+    const int64_t e1_ = a.chunk_row0[(CH) + 1];                                  \
+    const int cnt_ = static_cast<int>(e1_ - (BASE) < 64 ? e1_ - (BASE) : 64);    \
+    const int lim_ = cnt_ * K;                                                   \
+    const double* src_ = a.X + (BASE) * K;                                       \
+    _Pragma("unroll") for (int j = 0; j < KMAX; ++j) {                           \
+      const int e_ = lane + 64 * j;                                              \
+      stg[j] = (j < K && lim_ > 0) ? src_[e_ < lim_ ? e_ : lim_ - 1] : 0.0;      \
+    }                                                                            \
+    ystg = a.y[(BASE) + (lane < cnt_ ? lane : cnt_ - 1)];                        \
+  }
+  int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (ch >= a.n_chunks) return;  // whole wave; no block barrier follows
+  int64_t base = a.chunk_row0[ch];
+  SX_LOAD_BLOCK(ch, base);
+  while (ch < a.n_chunks) {
     const int g = a.chunk_group[ch];
-    const int64_t r0 = a.chunk_row0[ch], r1 = a.chunk_row0[ch + 1];
+    const int64_t r1 = a.chunk_row0[ch + 1];
     const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
     if (lane <= K) {
       const double tau = exp(th[1 + K + lane]);
       sw[w][lane] = lane == 0 ? th[0] + tau * z[0] : th[lane] + tau * z[lane];
     }
-    // sw[w] is this wave's: wait for the LDS writes, then a wave barrier
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) acc[k] = 0.0;
     double s0 = 0.0, s1 = 0.0, bad = 0.0;
-    for (int64_t base = r0; base < r1; base += 64) {
-      const int64_t n = base + lane;
-      const bool valid = n < r1;
-      const double* x = a.X + n * K;
+    for (; base < r1; base += 64) {
+      const int cnt = static_cast<int>(r1 - base < 64 ? r1 - base : 64);
+      // LDS accesses of one wave complete in order: the previous block's
+      // reads are done before these writes land
+      __builtin_amdgcn_wave_barrier();
+      {
+        const int lim = cnt * K;
+        int row = row0, col = col0;
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+          if (j < K) sx[w][lane + 64 * j < lim ? row * KS + col : BS] = stg[j];
+          row += dr;
+          col += dk;
+          if (col >= Kd) {
+            col -= Kd;
+            ++row;
+          }
+        }
+      }
+      const int yv = ystg;
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      // the next block (of this chunk or the wave's next one) in flight
+      {
+        int64_t nch = ch, nbase = base + 64;
+        if (nbase >= r1) {
+          nch = ch + W;
+          nbase = nch < a.n_chunks ? a.chunk_row0[nch] : 0;
+        }
+        if (nch < a.n_chunks) SX_LOAD_BLOCK(nch, nbase);
+      }
+      const bool valid = lane < cnt;
       double e = sw[w][0];
       double xv[KMAX];
 #pragma unroll
       for (int k = 0; k < KMAX; ++k) {
-        xv[k] = (valid && k < K) ? x[k] : 0.0;
+        xv[k] = (valid && k < K) ? sx[w][lane * KS + k] : 0.0;
         e = fma(xv[k], k < K ? sw[w][k + 1] : 0.0, e);
       }
       double term = 0.0, r = 0.0;
       if (valid) {
-        const int yv = a.y[n];
         stanx::math::sx_logit_row(e, yv ? 1.0 : -1.0, &term, &r);
         if (!isfinite(e)) bad += 1.0;
       }
@@ -100,7 +158,10 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
       out[K + 2] = bad;
     }
     __builtin_amdgcn_wave_barrier();
+    ch += W;
+    if (ch < a.n_chunks) base = a.chunk_row0[ch];
   }
+#undef SX_LOAD_BLOCK
 }
 
 // The row term and residual of the matrix-core pass on the raw hardware

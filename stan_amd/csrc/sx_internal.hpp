@@ -68,20 +68,23 @@ hipError_t launch_tile_x_f64(const double* Xrm, double* Xt, int64_t N,
 // Arguments of the batched BF16 hi+lo MFMA pass.  BF16 arrays are stored
 // as uint16_t on the host side.
 struct mx_args {
-  const uint16_t *X1h, *X1l, *X2h, *X2l;  // k-packed and 8-row-packed X
+  const uint16_t *Xh, *Xl;  // k-packed X, Np x Kp
   const void* y;
-  const uint16_t *Bh, *Bl;  // Cp x Kp chain coefficients
+  const float* yf;          // Np: sign (logistic) or value (normal), 0 past N
+  const uint16_t *Bh, *Bl, *Bm;  // Cp x Kp chain coefficients (three terms)
   const float* alpha;       // Cp intercepts
   float* gpart;             // splits x Cp x Kp
   double* spart;            // splits x Cp x 3
   uint16_t *Rh, *Rl;        // Kp > 256: BF16 hi/lo residuals, Np x Cp each
   int64_t N, n_tiles, tiles_per_split;
   int Kp, Cp, splits;
+  int variant;  // Kp <= 256: 0 one wave per 32-chain block, 1 a k-split pair
 };
 
-hipError_t launch_split_x(const double* Xrm, uint16_t* X1h, uint16_t* X1l, uint16_t* X2h,
-                          uint16_t* X2l, int64_t N, int64_t K, int64_t Np, int64_t Kp,
-                          hipStream_t s);
+hipError_t launch_split_x(const double* Xrm, uint16_t* Xh, uint16_t* Xl, int64_t N, int64_t K,
+                          int64_t Np, int64_t Kp, hipStream_t s);
+hipError_t launch_prep_y(const void* y, float* yf, int family, int64_t N, int64_t Np,
+                         hipStream_t s);
 // prep (theta -> bf16 coefficients), MFMA pass, FP64 reduction + finalize
 // into pa.out (stride pa.out_stride) for C chains
 hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C,
@@ -110,7 +113,7 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s);
 // Device workspace of batched MFMA evaluations for up to cap chains.
 struct mx_workspace {
   int cap = 0, Cp = 0, splits = 0;
-  uint16_t *Bh = nullptr, *Bl = nullptr;
+  uint16_t *Bh = nullptr, *Bl = nullptr, *Bm = nullptr;
   uint16_t *Rh = nullptr, *Rl = nullptr;  // Kp > 256 only
   float* alpha = nullptr;
   float* gpart = nullptr;
@@ -157,7 +160,8 @@ struct sx_glm {
   int blocks = 0, threads = 256, max_blocks = 0;
   // BF16 hi+lo layouts for batched MFMA evaluation (precision SX_BF16X2)
   int64_t Np = 0, Kp = 0;
-  uint16_t *d_X1h = nullptr, *d_X1l = nullptr, *d_X2h = nullptr, *d_X2l = nullptr;
+  uint16_t *d_Xh = nullptr, *d_Xl = nullptr;  // k-packed BF16 hi and lo
+  float* d_yf = nullptr;                       // y per padded row
   sx::mx_workspace mx;
   double* d_btheta = nullptr;  // batched theta / out staging for the C-ABI batch call
   double* d_bout = nullptr;

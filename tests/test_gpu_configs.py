@@ -9,8 +9,9 @@
       256-chain BF16 sampler to completion (status 0, split R-hat, pooled
       means within 4 MCSE of an FP64 device run on the same data).
   C4  hierarchical logistic J=1000, K=20, 64 chains FP32: the batched FP32
-      gradient against the FP64 oracle; the 64-chain sampler to completion
-      (status 0, split R-hat of the global parameters).
+      gradient against the FP64 oracle; the 64-chain sampler (1000 warm-up
+      iterations, max_depth 10) to completion (status 0, split R-hat of the
+      global parameters).
   C5  the serving stream's largest shapes (K=468, N=8e5) alone and in a
       mix: bitwise the same draws.
 
@@ -174,7 +175,7 @@ def test_c4_fp32_gradient_and_sampler_at_full_size():
     d = W.hier_logistic()
     m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
                    precision=S.FP32)
-    ref = O.GlmData(O.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"])
+    ref = O.GlmData(O.HIER, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"])
     th = W.random_thetas(m.D, 64, seed=4004) * 0.5
     lp, g = m.log_prob_grad_batch(th)
     e_lp = e_g = 0.0
@@ -186,8 +187,20 @@ def test_c4_fp32_gradient_and_sampler_at_full_size():
     _say(f"C4 FP32 batched (N={d['X'].shape[0]}, D={m.D}): max rel lp err {e_lp:.2e}, "
          f"grad {e_g:.2e}")
     assert e_lp <= 1e-6 and e_g <= 1e-5
-    res = S.hmc_nuts_diag_e_adapt_batch(m, 64, grad_precision=S.FP32, random_seed=404,
-                                        chain=1, num_warmup=200, num_samples=200, max_depth=8)
+    # Stan's default warm-up (1000 iterations, max_depth 10): after 200
+    # warm-up iterations at max_depth 8, chains started far out in the
+    # 21,042-dimensional space were still on their way in (step sizes 100x
+    # those of converged chains, lp thousands below the mode)
+    b = S.NutsBatch(m, 64, grad_precision=S.FP32, random_seed=404, chain=1, num_warmup=1000,
+                    num_samples=200, max_depth=10)
+    done = 0
+    while done < 64:
+        steps, done = b.run(20000)
+        _say(f"C4 sampler: {steps} steps, {done} of 64 chains done")
+        if steps == 0:
+            break
+    res = b.results()
+    b.close()
     assert np.all(res["status"] == 0), res["status"]
     K = d["X"].shape[1]
     q = np.transpose(res["draws"][:, :, 7:], (1, 0, 2))
