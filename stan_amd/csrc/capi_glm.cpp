@@ -442,6 +442,8 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
                sx::launch_prep_y(m->d_y, m->d_yf, family, N, m->Np, m->stream) != hipSuccess) {
       set_error("BF16 split of X failed");
       rc = SX_EDEVICE;
+    } else if (do_tune && N * K > 0) {
+      rc = sx::mx_autotune(m);
     }
   }
   if (rc == SX_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = SX_EDEVICE;
@@ -510,32 +512,8 @@ namespace {
 int enqueue_pass(sx_glm* m, int propto, int jacobian, bool reduce) {
   if (m->family == SX_HIER_LOGIT)
     return sx::hier_enqueue(m, m->hw, m->d_theta, 1, m->d_out, propto, jacobian, false, m->stream);
-  sx::pass_args a;
-  a.X = m->d_X;
-  a.y = m->d_y;
-  a.theta = m->d_theta;
-  a.partials = m->d_partials;
-  a.out = m->d_out;
-  a.N = m->N;
-  a.T = m->geom.T;
-  a.K = static_cast<int>(m->K);
-  a.L = m->geom.L;
-  a.lgL = m->geom.lgL;
-  a.S = m->geom.S;
-  a.D = static_cast<int>(m->D);
-  a.E = static_cast<int>(m->K) + 3;
-  a.propto = propto;
-  a.jacobian = jacobian;
-  a.alpha_sd = m->prior.alpha_sd;
-  a.beta_sd = m->prior.beta_sd;
-  a.sigma_sd = m->prior.sigma_sd;
-  a.sigma_mu = m->prior.sigma_mu;
-  const double half_log_2pi = 0.91893853320467274178032973640562;
-  a.alpha_const = a.alpha_sd > 0 ? -std::log(a.alpha_sd) - half_log_2pi : 0.0;
-  a.beta_const = a.beta_sd > 0 ? -std::log(a.beta_sd) - half_log_2pi : 0.0;
-  a.sigma_const = a.sigma_sd > 0 ? -std::log(a.sigma_sd) - half_log_2pi : 0.0;
-  SX_HIP(sx::launch_pass_f64(m->family, m->geom, a, m->blocks, m->threads, m->stream, reduce));
-  return SX_OK;
+  return sx::plain_enqueue(m, m->d_theta, m->d_out, m->d_partials, propto, jacobian, m->stream,
+                           reduce);
 }
 
 int eval(sx_glm* m, const double* theta, int propto, int jacobian, double* lp,
@@ -599,6 +577,39 @@ void free_workspace(hier_workspace& w) {
 // chains padded to the chain tile of the MFMA kernel (32x32 tiles, waves
 // of 32 chains): 64 for batches of at most 64 chains (two waves per
 // workgroup), else a multiple of 128 (four waves)
+// The FP64 pass of a logistic or normal context: theta (device, D) ->
+// out (device, D + 2), per-block partials in `partials` (blocks x (K + 3)
+// doubles), on `stream`.
+int plain_enqueue(sx_glm* m, const double* d_theta, double* d_out, double* partials, int propto,
+                  int jacobian, hipStream_t stream, bool reduce) {
+  pass_args a;
+  a.X = m->d_X;
+  a.y = m->d_y;
+  a.theta = d_theta;
+  a.partials = partials;
+  a.out = d_out;
+  a.N = m->N;
+  a.T = m->geom.T;
+  a.K = static_cast<int>(m->K);
+  a.L = m->geom.L;
+  a.lgL = m->geom.lgL;
+  a.S = m->geom.S;
+  a.D = static_cast<int>(m->D);
+  a.E = static_cast<int>(m->K) + 3;
+  a.propto = propto;
+  a.jacobian = jacobian;
+  a.alpha_sd = m->prior.alpha_sd;
+  a.beta_sd = m->prior.beta_sd;
+  a.sigma_sd = m->prior.sigma_sd;
+  a.sigma_mu = m->prior.sigma_mu;
+  const double half_log_2pi = 0.91893853320467274178032973640562;
+  a.alpha_const = a.alpha_sd > 0 ? -std::log(a.alpha_sd) - half_log_2pi : 0.0;
+  a.beta_const = a.beta_sd > 0 ? -std::log(a.beta_sd) - half_log_2pi : 0.0;
+  a.sigma_const = a.sigma_sd > 0 ? -std::log(a.sigma_sd) - half_log_2pi : 0.0;
+  SX_HIP(launch_pass_f64(m->family, m->geom, a, m->blocks, m->threads, stream, reduce));
+  return SX_OK;
+}
+
 int mx_chain_pad(const sx_glm* m, int C) {
   (void)m;
   if (C <= 64) return 64;
@@ -612,7 +623,9 @@ int mx_prepare(sx_glm* m, mx_workspace& w, int C) {
   }
   const int Cp = mx_chain_pad(m, C);
   const int64_t n_tiles = m->Np / 32;
-  int splits = static_cast<int>(std::min<int64_t>(512, std::max<int64_t>(8, ((n_tiles + 7) / 8 + 7) / 8 * 8)));
+  const int64_t tps = m->mx_tps > 0 ? m->mx_tps : 8;
+  int splits = static_cast<int>(
+      std::min<int64_t>(512, std::max<int64_t>(8, ((n_tiles + tps - 1) / tps + 7) / 8 * 8)));
   if (w.cap >= Cp && w.splits == splits) return SX_OK;
   free_workspace(w);
   const size_t ck = static_cast<size_t>(Cp) * m->Kp;
@@ -656,8 +669,7 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   a.Cp = mx_chain_pad(m, C);
   a.splits = w.splits;
   a.tiles_per_split = (a.n_tiles + a.splits - 1) / a.splits;
-  a.variant = 0;
-  if (const char* e = std::getenv("STANX_MFMA_VARIANT")) a.variant = std::atoi(e);
+  a.variant = m->mx_variant;
   pass_args pa{};
   pa.theta = d_theta;
   pa.out = d_out;
@@ -679,6 +691,94 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
   pa.beta_const = pa.beta_sd > 0 ? -std::log(pa.beta_sd) - h : 0.0;
   pa.sigma_const = pa.sigma_sd > 0 ? -std::log(pa.sigma_sd) - h : 0.0;
   SX_HIP(launch_mfma_pass(m->family, d_theta, m->D, C, a, pa, stream));
+  return SX_OK;
+}
+
+// Launch shape of the batched MFMA pass, timed on the device at context
+// creation: the kernel variant (one wave per 32-chain block, or a pair of
+// waves split by k where Kp / 32 is even and at most 8) and the row split
+// (target tiles of 32 rows per split: 4, 8, 16 or 32), each timed as full
+// evaluations of 128 chains at theta = 0 (the kernels' work does not depend
+// on the values).  The split sets the order of the FP64 sums, so, as for
+// the FP64 pass, the first decision per (device, family, N, K) is kept for
+// the process (and in STANX_TUNING_CACHE, as family + 100).  The choice
+// does not depend on the batch size, so a chain's result stays independent
+// of the other chains in its batch.
+int mx_autotune(sx_glm* m) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  read_tune_file();
+  const std::string key = tune_key(m->device, 100 + m->family, m->N, m->K);
+  const auto hit = g_tuned.find(key);
+  if (hit != g_tuned.end()) {
+    m->mx_variant = hit->second.L;
+    m->mx_tps = hit->second.smax;
+    char line[160];
+    std::snprintf(line, sizeof line, "MFMA pass cached: variant=%d tiles/split=%d\n",
+                  m->mx_variant, m->mx_tps);
+    m->tuning_log += line;
+    return SX_OK;
+  }
+  const int C = 128;
+  const int64_t D = m->D;
+  double *d_th = nullptr, *d_out = nullptr;
+  if (hipMalloc(&d_th, sizeof(double) * C * D) != hipSuccess ||
+      hipMalloc(&d_out, sizeof(double) * C * (D + 2)) != hipSuccess) {
+    (void)hipFree(d_th);
+    set_error("MFMA autotune: out of device memory");
+    return SX_ENOMEM;
+  }
+  mx_workspace w;
+  std::string log;
+  int rc = hipMemsetAsync(d_th, 0, sizeof(double) * C * D, m->stream) == hipSuccess
+               ? SX_OK : SX_EDEVICE;
+  const int kq = static_cast<int>(m->Kp / 32);
+  int best_v = 0, best_t = 8;
+  double best_ms = 1e30;
+  for (int v = 0; v <= 1 && rc == SX_OK; ++v) {
+    if (v == 1 && (m->Kp > 256 || kq % 2 != 0)) break;
+    for (int tps : {4, 8, 16, 32}) {
+      m->mx_variant = v;
+      m->mx_tps = tps;
+      for (int i = 0; i < 2 && rc == SX_OK; ++i)
+        rc = mx_enqueue(m, w, d_th, C, d_out, 1, 1, m->stream);
+      if (rc) break;
+      if (hipEventRecord(m->ev0, m->stream) != hipSuccess) { rc = SX_EDEVICE; break; }
+      const int reps = 3;
+      for (int i = 0; i < reps && rc == SX_OK; ++i)
+        rc = mx_enqueue(m, w, d_th, C, d_out, 1, 1, m->stream);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(m->ev1, m->stream) != hipSuccess ||
+          hipEventSynchronize(m->ev1) != hipSuccess ||
+          hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess) {
+        rc = SX_EDEVICE;
+        break;
+      }
+      const double per = ms / reps;
+      char line[160];
+      std::snprintf(line, sizeof line, "MFMA pass variant=%d tiles/split=%d: %.4f ms (128 chains)\n",
+                    v, tps, per);
+      log += line;
+      if (per < best_ms) {
+        best_ms = per;
+        best_v = v;
+        best_t = tps;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(m->stream);
+  free_workspace(w);
+  (void)hipFree(d_th);
+  (void)hipFree(d_out);
+  m->mx_variant = best_v;
+  m->mx_tps = best_t;
+  if (rc) return rc;
+  remember_tuning(key, tuned{best_v, best_t, 0, 0});
+  char line[160];
+  std::snprintf(line, sizeof line, "MFMA pass chosen: variant=%d tiles/split=%d (%.4f ms)\n",
+                best_v, best_t, best_ms);
+  log += line;
+  m->tuning_log += log;
   return SX_OK;
 }
 

@@ -22,8 +22,9 @@
 // of one row per lane (one 16-byte LDS read); the adjoint contracts over
 // rows and takes 4 consecutive rows of one column per lane with gfx950's
 // transposing LDS read (ds_read_b64_tr_b16), from the same staged tile.
-// In LDS, 16-byte chunk c of tile row r sits at c ^ (r & SWZ), which
-// spreads both access patterns over the banks.
+// In LDS, 16-byte chunk c of tile row r sits at c ^ 2 (r mod 8) (a smaller
+// XOR block for short rows), which spreads both access patterns over the
+// banks.
 //
 // Workgroup = NW waves x 32 chains (NW = 4 for 128-chain tiles, 2 for
 // batches of at most 64 chains); each 32-row tile of X is staged once per
@@ -128,28 +129,64 @@ __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, 
   if (threadIdx.x == 0) alpha[c] = c < C ? static_cast<float>(theta[c * D]) : 0.0f;
 }
 
-// The adjoint's X operand from a staged tile (row stride RS elements, CPR
-// chunks of 8 per row, swizzle SWZ): for the lane's column c = c0 + li
-// (li = lane % 32, h = lane / 32), rows r0 + 4 h + (0..3) and
-// r0 + 8 + 4 h + (0..3), i.e. element j of the MFMA B operand is row
-// r0 + 8 (j / 4) + 4 h + j % 4 (the rows of the forward's accumulator
-// element 8 st + j when r0 = 16 st).  Two transposing reads: within each
-// group of 16 lanes, lane i reads 4 consecutive columns (8 bytes) of one
-// row, and lane i receives column i of the 4 rows the group read
-// (result element e = row e).
-template <int RS, int SWZ>
-__device__ __forceinline__ bf16x8 tr_operand(const __bf16* t, int r0, int c0, int lane) {
-  const int i = lane & 15, g = lane >> 4;
-  const int hh = g >> 1, ch = g & 1;
+// Staged tile layout in LDS: row r of a tile occupies ROWB bytes; its
+// 16-byte chunk c (8 consecutive columns) sits at chunk position
+// c ^ f(r), f(r) = 2 (r & (P/2 - 1)), P the largest power of two <= 16
+// dividing the chunks per row.  The XOR spreads both read patterns over the
+// banks: the forward's 16-byte reads (8 consecutive rows, one chunk each)
+// and the transposing reads (4 consecutive rows x 2 adjacent chunks per 16
+// lanes).  Because f(r) only changes bits 1..3 of the chunk index and
+// depends on r mod 8, every read address is a per-lane base, one XOR with
+// a per-lane value and an immediate.
+template <int ROWB, int P>
+struct tile_addr {
+  static_assert(P == 4 || P == 8 || P == 16, "XOR block of 4, 8 or 16 chunks");
+  static constexpr int GM = P / 2 - 1;
+  int fb, fx, tb, tx;
+  __device__ __forceinline__ explicit tile_addr(int lane) {
+    const int li = lane & 31, h = lane >> 5;
+    fb = li * ROWB + 16 * h;
+    fx = 32 * (li & GM);
+    // transposing reads: within each group of 16 lanes, lane i reads 8
+    // bytes of row 4 hh + i / 4 (plus 16 st + 8 q), columns c0 + 16 ch +
+    // 4 (i % 4) .. + 3
+    const int i = lane & 15, g = lane >> 4, hh = g >> 1, ch = g & 1;
+    const int rr = 4 * hh + (i >> 2);
+    const int f = 2 * (rr & GM);
+    const int cl = 2 * ch + ((i & 3) >> 1);
+    tb = rr * ROWB + 16 * (cl ^ (f & 3)) + 8 * (i & 1);
+    tx = 16 * (f & ~3);
+  }
+  // byte offset of the forward operand of k-step s (columns 16 s + 8 h ..)
+  __device__ __forceinline__ int fwd(int s) const {
+    return fb + ((32 * (s % (P / 2))) ^ fx) + 16 * P * (s / (P / 2));
+  }
+  // byte offset of the transposing read q of half st, column block kb
+  __device__ __forceinline__ int tr(int st, int q, int kb) const {
+    return tb + (16 * st + 8 * q) * ROWB + 16 * P * (kb / (P / 4)) +
+           ((64 * (kb % (P / 4))) ^ tx);
+  }
+};
+
+// chunk position of global chunk c of tile row r (and the reverse)
+template <int P>
+__device__ __forceinline__ int swz(int r, int c) { return c ^ (2 * (r & (P / 2 - 1))); }
+
+template <int CPR>
+constexpr int xor_block() { return CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4; }
+
+// The adjoint's X operand for the lane's column (c0 + lane % 32) and rows
+// r0 + 8 (j / 4) + 4 (lane / 32) + j % 4 (j < 8): element j of the MFMA B
+// operand.  Two transposing reads: lane i of a 16-lane group receives
+// column i of the 4 rows the group read (result element e = row e).
+__device__ __forceinline__ bf16x8 tr_operand(const char* base, int off0, int off1) {
+  const bf16x4 p0 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4)(base + off0));
+  const bf16x4 p1 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4)(base + off1));
   bf16x8 v;
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int r = r0 + 8 * q + 4 * hh + (i >> 2);
-    const int c = c0 + 16 * ch + 4 * (i & 3);
-    const int off = r * RS + 8 * ((c >> 3) ^ (r & SWZ)) + (c & 7);
-    const bf16x4 p = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4)(t + off));
-#pragma unroll
-    for (int e = 0; e < 4; ++e) v[4 * q + e] = p[e];
+  for (int e = 0; e < 4; ++e) {
+    v[e] = p0[e];
+    v[4 + e] = p1[e];
   }
   return v;
 }
@@ -172,29 +209,28 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
                                         float& nf) {
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;
-  constexpr int CPR = Kp / 8;
-  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
-  constexpr int TE = 32 * Kp;
-  // lane-derived address terms are recomputed per tile rather than kept
-  // live across the tile loop next to the coefficients and accumulators
+  constexpr int P = xor_block<Kp / 8>();
+  constexpr int TB = 64 * Kp;  // bytes of one tile array (hi or lo)
+  // the lane's address terms are formed per tile (a few registers) rather
+  // than every read address being kept live across the tile loop
   int ln = lane;
   asm volatile("" : "+v"(ln));
-  const int li = ln & 31, h = ln >> 5;
-  lane = ln;
+  const tile_addr<2 * Kp, P> ad(ln);
+  const int h = ln >> 5;
+  const char* xb = reinterpret_cast<const char*>(X);
   // forward: the operands of step s + 1 are read while step s's products
   // run (one step of prefetch)
-  auto fwd_off = [&](int s) { return 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ))); };
   f32x16 H;
 #pragma unroll
   for (int j = 0; j < 16; ++j) H[j] = 0.f;
-  bf16x8 fh = *reinterpret_cast<const bf16x8*>(X + fwd_off(0));
-  bf16x8 fl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(0));
+  bf16x8 fh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(0));
+  bf16x8 fl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(0));
 #pragma unroll
   for (int s = 0; s < S16; ++s) {
     bf16x8 nh = fh, nl = fl;
     if (s + 1 < S16) {
-      nh = *reinterpret_cast<const bf16x8*>(X + fwd_off(s + 1));
-      nl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(s + 1));
+      nh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(s + 1));
+      nl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(s + 1));
     }
     H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bh[s], H, 0, 0, 0);
     H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bl[s], H, 0, 0, 0);
@@ -204,8 +240,8 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
     fl = nl;
   }
   // adjoint operands of (st, kb) = (0, 0), read while the link runs
-  bf16x8 ah = tr_operand<Kp, SWZ>(X, 0, 0, lane);
-  bf16x8 al = tr_operand<Kp, SWZ>(X + TE, 0, 0, lane);
+  bf16x8 ah = tr_operand(xb, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
+  bf16x8 al = tr_operand(xb + TB, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
   const int64_t n0 = t * 32;
   float lpt = 0.f, srt = 0.f;
 #pragma unroll
@@ -242,8 +278,8 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
       bf16x8 nh = ah, nl = al;
       const int nst = kb + 1 < KQ ? st : st + 1, nkb = kb + 1 < KQ ? kb + 1 : 0;
       if (nst < 2) {
-        nh = tr_operand<Kp, SWZ>(X, 16 * nst, 32 * nkb, lane);
-        nl = tr_operand<Kp, SWZ>(X + TE, 16 * nst, 32 * nkb, lane);
+        nh = tr_operand(xb, ad.tr(nst, 0, nkb), ad.tr(nst, 1, nkb));
+        nl = tr_operand(xb + TB, ad.tr(nst, 0, nkb), ad.tr(nst, 1, nkb));
       }
       g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, ah, g[kb], 0, 0, 0);
       g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, al, g[kb], 0, 0, 0);
@@ -263,7 +299,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;          // forward k-steps of 16
   constexpr int CPR = Kp / 8;          // 16-byte chunks per row
-  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int P = xor_block<CPR>();
   constexpr int TE = 32 * Kp;          // elements of one tile array
   constexpr int BPA = Kp / 16;         // 1 KB blocks per tile array
   constexpr int NBW = 2 * BPA / NW;    // staging blocks per wave and tile (hi + lo)
@@ -309,7 +345,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   const __bf16* Xl = reinterpret_cast<const __bf16*>(a.Xl);
   // Staging of tile T into buffer XB / YB: wave w moves 1 KB blocks w,
   // w + NW, ... of the tile (hi blocks, then lo), lane l 16 bytes at block
-  // offset 16 l, reading chunk (p % CPR) ^ (r & SWZ) of row r = p / CPR
+  // offset 16 l, reading chunk swz(r, p % CPR) of row r = p / CPR
   // for LDS chunk p; 8 lanes of wave 0 move the tile's 32 y values.
 #define SX_STAGE(T, XB, YB)                                                      \
   {                                                                              \
@@ -321,7 +357,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
       const int bb_ = b_ % BPA;                                                  \
       const int p_ = bb_ * 64 + ln_;                                             \
       const int r_ = p_ / CPR;                                                   \
-      const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
+      const int src_ = r_ * CPR + swz<P>(r_, p_ % CPR);                           \
       __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + base_ + 8 * src_,  \
           (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
@@ -386,7 +422,7 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
   constexpr int KH = KQ / 2;           // 32-column blocks per half
   constexpr int Kp = 32 * KQ;
   constexpr int CPR = Kp / 8;
-  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int P = xor_block<CPR>();
   constexpr int TE = 32 * Kp;
   constexpr int BPA = Kp / 16;
   constexpr int NBW = 2 * BPA / NW;
@@ -440,7 +476,7 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
       const int bb_ = b_ % BPA;                                                  \
       const int p_ = bb_ * 64 + ln_;                                             \
       const int r_ = p_ / CPR;                                                   \
-      const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
+      const int src_ = r_ * CPR + swz<P>(r_, p_ % CPR);                           \
       __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + base_ + 8 * src_,  \
           (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
@@ -449,18 +485,22 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
                                        16, 0, 0);                                \
   }
   auto tile = [&](const __bf16* X, const float* Y, int64_t t) {
-    auto fwd_off = [&](int s) { return 8 * (li * CPR + ((2 * (s0 + s) + h) ^ (li & SWZ))); };
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const tile_addr<2 * Kp, P> ad(ln);
+    const char* xb = reinterpret_cast<const char*>(X);
+    constexpr int TB = 64 * Kp;
     f32x16 H;
 #pragma unroll
     for (int j = 0; j < 16; ++j) H[j] = 0.f;
-    bf16x8 fh = *reinterpret_cast<const bf16x8*>(X + fwd_off(0));
-    bf16x8 fl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(0));
+    bf16x8 fh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(s0));
+    bf16x8 fl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(s0));
 #pragma unroll
     for (int s = 0; s < KQ; ++s) {
       bf16x8 nh = fh, nl = fl;
       if (s + 1 < KQ) {
-        nh = *reinterpret_cast<const bf16x8*>(X + fwd_off(s + 1));
-        nl = *reinterpret_cast<const bf16x8*>(X + TE + fwd_off(s + 1));
+        nh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(s0 + s + 1));
+        nl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(s0 + s + 1));
       }
       H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bh[s], H, 0, 0, 0);
       H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bl[s], H, 0, 0, 0);
@@ -473,8 +513,8 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
     for (int q = 0; q < 4; ++q)
       hx[cbw][kh][q][lane] = make_float4(H[4 * q], H[4 * q + 1], H[4 * q + 2], H[4 * q + 3]);
     // the adjoint's first operands, read across the exchange
-    bf16x8 ah = tr_operand<Kp, SWZ>(X, 0, 32 * (kh * KH), lane);
-    bf16x8 al = tr_operand<Kp, SWZ>(X + TE, 0, 32 * (kh * KH), lane);
+    bf16x8 ah = tr_operand(xb, ad.tr(0, 0, kh * KH), ad.tr(0, 1, kh * KH));
+    bf16x8 al = tr_operand(xb + TB, ad.tr(0, 0, kh * KH), ad.tr(0, 1, kh * KH));
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -515,8 +555,8 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
         bf16x8 nh = ah, nl = al;
         const int nst = kb + 1 < KH ? st : st + 1, nkb = kb + 1 < KH ? kb + 1 : 0;
         if (nst < 2) {
-          nh = tr_operand<Kp, SWZ>(X, 16 * nst, 32 * (kh * KH + nkb), lane);
-          nl = tr_operand<Kp, SWZ>(X + TE, 16 * nst, 32 * (kh * KH + nkb), lane);
+          nh = tr_operand(xb, ad.tr(nst, 0, kh * KH + nkb), ad.tr(nst, 1, kh * KH + nkb));
+          nl = tr_operand(xb + TB, ad.tr(nst, 0, kh * KH + nkb), ad.tr(nst, 1, kh * KH + nkb));
         }
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, ah, g[kb], 0, 0, 0);
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, al, g[kb], 0, 0, 0);
@@ -584,7 +624,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
   constexpr int Kp = 32 * KQ;
   constexpr int S16 = 2 * KQ;
   constexpr int CPR = Kp / 8;
-  constexpr int SWZ = (CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4) - 1;
+  constexpr int P = xor_block<CPR>();
   constexpr int TE = 32 * Kp;
   constexpr int BPA = Kp / 16;
   constexpr int NBW = 2 * BPA / NW;
@@ -630,7 +670,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
       const int bb_ = b_ % BPA;                                                  \
       const int p_ = bb_ * 64 + ln_;                                             \
       const int r_ = p_ / CPR;                                                   \
-      const int src_ = r_ * CPR + ((p_ % CPR) ^ (r_ & SWZ));                     \
+      const int src_ = r_ * CPR + swz<P>(r_, p_ % CPR);                           \
       __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + base_ + 8 * src_,  \
           (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
@@ -639,17 +679,28 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
                                        16, 0, 0);                                \
   }
   auto tile = [&](const __bf16* X, const float* Y, int64_t t) {
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const tile_addr<2 * Kp, P> ad(ln);
+    const char* xb = reinterpret_cast<const char*>(X);
+    constexpr int TB = 64 * Kp;
     f32x16 H;
 #pragma unroll
     for (int j = 0; j < 16; ++j) H[j] = 0.f;
+    bf16x8 fh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(0));
+    bf16x8 fl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(0));
 #pragma unroll
     for (int s = 0; s < S16; ++s) {
-      const int off = 8 * (li * CPR + ((2 * s + h) ^ (li & SWZ)));
-      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(X + off);
-      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(X + TE + off);
-      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bh[s], H, 0, 0, 0);
-      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bl[s], H, 0, 0, 0);
-      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bh[s], H, 0, 0, 0);
+      bf16x8 nh = fh, nl = fl;
+      if (s + 1 < S16) {
+        nh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(s + 1));
+        nl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(s + 1));
+      }
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bh[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bl[s], H, 0, 0, 0);
+      H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, bh[s], H, 0, 0, 0);
+      fh = nh;
+      fl = nl;
     }
     const int64_t n0 = t * 32;
     float lpt = 0.f, srt = 0.f;
@@ -724,7 +775,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
   static_assert((4 * KQA) % NW == 0, "staging blocks per wave");
   constexpr int KR = 32 * KQA;         // columns of the block
   constexpr int CPRB = KR / 8;         // 16-byte chunks per block row
-  constexpr int SWZ = (CPRB % 16 == 0 ? 16 : CPRB % 8 == 0 ? 8 : 4) - 1;
+  constexpr int P = xor_block<CPRB>();
   constexpr int TE = 32 * KR;          // elements of one tile array
   constexpr int BPA = 2 * KQA;         // 1 KB staging blocks per tile array
   constexpr int NBW = 2 * BPA / NW;
@@ -755,7 +806,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
   const bf16x8* Rh = reinterpret_cast<const bf16x8*>(a.Rh);
   const bf16x8* Rl = reinterpret_cast<const bf16x8*>(a.Rl);
   // staging: LDS chunk p of an array is row p / CPRB of the tile, chunk
-  // (p % CPRB) ^ (row & SWZ) of the column block
+  // swz(row, p % CPRB) of the column block
 #define SX_STAGE(T, XB)                                                          \
   {                                                                              \
     _Pragma("unroll") for (int i = 0; i < NBW; ++i) {                            \
@@ -763,12 +814,17 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
       const int bb_ = b_ % BPA;                                                  \
       const int p_ = bb_ * 64 + lane;                                            \
       const int r_ = p_ / CPRB;                                                  \
-      const int64_t src_ = ((T) * 32 + r_) * Kp + k0 + 8 * ((p_ % CPRB) ^ (r_ & SWZ)); \
+      const int64_t src_ = ((T) * 32 + r_) * Kp + k0 + 8 * swz<P>(r_, p_ % CPRB);  \
       __builtin_amdgcn_global_load_lds((b_ < BPA ? Xh : Xl) + src_,              \
           (lds_ptr)((XB) + (b_ / BPA) * TE + bb_ * 512), 16, 0, 0);              \
     }                                                                            \
   }
   auto tile = [&](const __bf16* X, int64_t t) {
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const tile_addr<2 * KR, P> ad(ln);
+    const char* xb = reinterpret_cast<const char*>(X);
+    constexpr int TB = 64 * KR;
     bf16x8 rh[2], rl[2];
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
@@ -780,8 +836,8 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_adj(mx_args a, int kq0) {
     for (int st = 0; st < 2; ++st) {
 #pragma unroll
       for (int kb = 0; kb < KQA; ++kb) {
-        const bf16x8 xh = tr_operand<KR, SWZ>(X, 16 * st, 32 * kb, lane);
-        const bf16x8 xl = tr_operand<KR, SWZ>(X + TE, 16 * st, 32 * kb, lane);
+        const bf16x8 xh = tr_operand(xb, ad.tr(st, 0, kb), ad.tr(st, 1, kb));
+        const bf16x8 xl = tr_operand(xb + TB, ad.tr(st, 0, kb), ad.tr(st, 1, kb));
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[st], xh, g[kb], 0, 0, 0);
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[st], xl, g[kb], 0, 0, 0);
         g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[st], xh, g[kb], 0, 0, 0);

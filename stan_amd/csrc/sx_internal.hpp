@@ -140,6 +140,10 @@ namespace sx {
 int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double* d_out,
                int propto, int jacobian, hipStream_t stream);
 int hier_prepare(sx_glm* m, hier_workspace& w, int C);
+int mx_autotune(sx_glm* m);
+int plain_enqueue(sx_glm* m, const double* d_theta, double* d_out, double* partials, int propto,
+                  int jacobian, hipStream_t stream, bool reduce);
+
 int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, double* d_out,
                  int propto, int jacobian, bool fp32, hipStream_t stream);
 }
@@ -163,6 +167,9 @@ struct sx_glm {
   uint16_t *d_Xh = nullptr, *d_Xl = nullptr;  // k-packed BF16 hi and lo
   float* d_yf = nullptr;                       // y per padded row
   sx::mx_workspace mx;
+  // launch shape of the MFMA pass (fixed per context; chosen by the
+  // autotuner): kernel variant and target tiles of 32 rows per row split
+  int mx_variant = 0, mx_tps = 8;
   double* d_btheta = nullptr;  // batched theta / out staging for the C-ABI batch call
   double* d_bout = nullptr;
   int64_t bcap = 0;
