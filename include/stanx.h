@@ -183,6 +183,24 @@ int sx_glm_time_logp_grad(sx_glm* m, const double* theta, int64_t C,
                           int iters, int warmup, double* ms_per_eval,
                           double* ms_kernel);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  One log
+ * density + gradient evaluation at theta, already on the device (D
+ * doubles), into out on the device (D + 2 doubles: lp, gradient, then a
+ * status word that is nonzero when the density cannot be evaluated, the
+ * SX_EDOMAIN case of sx_glm_logp_grad), enqueued on `stream` (a
+ * hipStream_t; NULL: the context's stream) without waiting.  The same
+ * kernels as sx_glm_logp_grad, with their partial sums in a per-GPU scratch
+ * array of SX_NOALLOC_SCRATCH_BYTES that the library holds until the
+ * process ends: no device memory is allocated or freed, no copy or memset
+ * is issued.  A context whose partial sums need more (FP64 pass: blocks x
+ * (K + 3) doubles; hierarchical: (K + 3) per row chunk + (2K + 5) per 8
+ * groups) gets SX_EINVAL.  Calls that overlap on one GPU share the array
+ * and race; the array keeps the last call's contents.                   */
+#define SX_NOALLOC_SCRATCH_BYTES (8 * 1024 * 1024)
+int sx_glm_logp_grad_noalloc(sx_glm* m, const double* d_theta, int propto,
+                             int jacobian, double* d_out, void* stream);
+
 /* Tuning knobs of the current kernel configuration (for reports):
  * lanes per row, slots per lane, blocks, threads per block.           */
 int sx_glm_kernel_config(const sx_glm* m, int* lanes_per_row,

@@ -32,7 +32,7 @@ EXPORTED = [
     "sx_hmc_nuts_dense_e_adapt", "sx_hmc_nuts_dense_e_adapt_cb", "sx_hmc_nuts_dense_e_adapt_csv",
     "sx_hmc_nuts_dense_e", "sx_hmc_nuts_dense_e_cb", "sx_hmc_nuts_dense_e_adapt_cb_csv",
     "sx_hmc_nuts_diag_e_adapt_files", "sx_hmc_nuts_diag_e_adapt_cb_files",
-    "sx_hmc_nuts_diag_e_files", "sx_device_copy_bandwidth",
+    "sx_hmc_nuts_diag_e_files", "sx_device_copy_bandwidth", "sx_glm_logp_grad_noalloc",
 ]
 
 _lib = None
@@ -123,6 +123,7 @@ def lib():
         L.sx_glm_dims.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
         L.sx_last_error.restype = C.c_char_p
         L.sx_glm_logp_grad.argtypes = [vp, vp, C.c_int, C.c_int, dp, vp]
+        L.sx_glm_logp_grad_noalloc.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp]
         L.sx_glm_logp.argtypes = [vp, vp, C.c_int, C.c_int, dp]
         L.sx_glm_logp_grad_batch.argtypes = [vp, vp, i64, C.c_int, C.c_int, vp, vp]
         L.sx_glm_time_logp_grad.argtypes = [vp, vp, i64, C.c_int, C.c_int, dp, dp]

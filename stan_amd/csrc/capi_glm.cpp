@@ -342,6 +342,9 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
   int cus = 0;
   int rc = device_cus(&cus);
   if (rc) return rc;
+  // the allocation-free entry point's scratch address on this device (its
+  // failure only disables that entry point, which then reports it)
+  (void)sx::noalloc_setup(device);
   sx_glm* m = new sx_glm();
   m->family = family;
   m->precision = precision;

@@ -143,7 +143,9 @@ int hier_prepare(sx_glm* m, hier_workspace& w, int C);
 int mx_autotune(sx_glm* m);
 int plain_enqueue(sx_glm* m, const double* d_theta, double* d_out, double* partials, int propto,
                   int jacobian, hipStream_t stream, bool reduce);
-
+// the per-device scratch array of the allocation-free entry point: its
+// address on `device`, looked up once (called at context creation)
+int noalloc_setup(int device);
 int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, double* d_out,
                  int propto, int jacobian, bool fp32, hipStream_t stream);
 }
