@@ -114,7 +114,11 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y,
  *                  -1: choose
  *   blocks         workgroups per launch
  *   autotune       1: time the candidate configurations on the device at
- *                  create and keep the fastest (default for N*K >= 2^22)
+ *                  create and keep the fastest (default for N*K >= 2^22):
+ *                  the FP64 pass's layout and grid, the MFMA pass's kernel
+ *                  variant and row split, the hierarchical chunk size
+ *   hier_chunk     SX_HIER_LOGIT: rows per chunk at most (a multiple of 32,
+ *                  <= 4096; 0: autotuned, else 256)
  * No counterpart in the reference (its CPU GLM has no tunables).         */
 typedef struct sx_tuning {
   int lanes_per_row;
@@ -122,6 +126,7 @@ typedef struct sx_tuning {
   int prefetch;
   int blocks;
   int autotune;
+  int hier_chunk;
 } sx_tuning;
 
 /* sx_glm_create with explicit tuning (NULL = defaults). */

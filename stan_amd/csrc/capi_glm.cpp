@@ -206,25 +206,28 @@ int autotune(sx_glm* m, const double* d_rowmajor, int cus) {
   return SX_OK;
 }
 
-// Hierarchical model: cut every group into chunks of at most kHierChunk
-// rows (a chunk never straddles a group), upload the chunk tables, an FP32
-// copy of X for batched evaluations (SX_FP32), and the single-chain
-// partials.
-int hier_setup(sx_glm* m, const double* d_X, const double* X, const int64_t* off) {
+// Hierarchical model: cut every group into chunks of at most `chunk` rows
+// (a chunk never straddles a group) and upload the chunk tables.
+int hier_chunks(sx_glm* m, const int64_t* off, int chunk) {
   std::vector<int> cg;
   std::vector<int64_t> cr0, gc0;
   gc0.reserve(static_cast<size_t>(m->J) + 1);
   for (int64_t j = 0; j < m->J; ++j) {
     gc0.push_back(static_cast<int64_t>(cg.size()));
-    for (int64_t r = off[j]; r < off[j + 1]; r += sx::kHierChunk) {
+    for (int64_t r = off[j]; r < off[j + 1]; r += chunk) {
       cg.push_back(static_cast<int>(j));
       cr0.push_back(r);
     }
   }
   gc0.push_back(static_cast<int64_t>(cg.size()));
   cr0.push_back(m->N);
+  (void)hipFree(m->d_chunk_group);
+  (void)hipFree(m->d_chunk_row0);
+  (void)hipFree(m->d_group_chunk0);
+  m->d_chunk_group = nullptr;
+  m->d_chunk_row0 = m->d_group_chunk0 = nullptr;
   m->n_chunks = static_cast<int64_t>(cg.size());
-  (void)d_X;
+  m->hier_chunk = chunk;
   SX_HIP(hipMalloc(&m->d_chunk_group, sizeof(int) * std::max<size_t>(1, cg.size())));
   SX_HIP(hipMalloc(&m->d_chunk_row0, sizeof(int64_t) * cr0.size()));
   SX_HIP(hipMalloc(&m->d_group_chunk0, sizeof(int64_t) * gc0.size()));
@@ -232,13 +235,107 @@ int hier_setup(sx_glm* m, const double* d_X, const double* X, const int64_t* off
     SX_HIP(hipMemcpy(m->d_chunk_group, cg.data(), sizeof(int) * cg.size(), hipMemcpyHostToDevice));
   SX_HIP(hipMemcpy(m->d_chunk_row0, cr0.data(), sizeof(int64_t) * cr0.size(), hipMemcpyHostToDevice));
   SX_HIP(hipMemcpy(m->d_group_chunk0, gc0.data(), sizeof(int64_t) * gc0.size(), hipMemcpyHostToDevice));
-  if (m->precision == SX_FP32 && m->N * m->K > 0) {
-    const size_t n = static_cast<size_t>(m->N * m->K);
-    std::vector<float> x32(n);
-    for (size_t i = 0; i < n; ++i) x32[i] = static_cast<float>(X[i]);
-    SX_HIP(hipMalloc(&m->d_X32, n * sizeof(float)));
-    SX_HIP(hipMemcpy(m->d_X32, x32.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  return SX_OK;
+}
+
+// Chunk size of the hierarchical model, timed on the device at context
+// creation: 128, 256, 512 or 1024 rows, each timed as full evaluations at
+// theta = 0 (the kernels' work does not depend on the values) -- one chain
+// in FP64 for an FP64 context, a batch of 64 chains for an FP32 one.  The
+// chunk sets the order of the FP64 sums, so the first decision per
+// (device, precision, N, J, K) is kept for the process and in
+// STANX_TUNING_CACHE (family field 200 + [FP32], K field 64 J + K).
+int hier_autotune(sx_glm* m, const int64_t* off) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  read_tune_file();
+  const bool fp32 = m->precision == SX_FP32;
+  const std::string key = tune_key(m->device, 200 + (fp32 ? 1 : 0), m->N, 64 * m->J + m->K);
+  const auto hit = g_tuned.find(key);
+  if (hit != g_tuned.end()) {
+    char line[160];
+    std::snprintf(line, sizeof line, "hierarchical chunk cached: %d rows\n", hit->second.L);
+    m->tuning_log += line;
+    return hier_chunks(m, off, hit->second.L);
   }
+  const int C = fp32 ? 64 : 1;
+  const int64_t D = m->D;
+  double *d_th = nullptr, *d_out = nullptr;
+  if (hipMalloc(&d_th, sizeof(double) * C * D) != hipSuccess ||
+      hipMalloc(&d_out, sizeof(double) * C * (D + 2)) != hipSuccess) {
+    (void)hipFree(d_th);
+    set_error("hierarchical autotune: out of device memory");
+    return SX_ENOMEM;
+  }
+  int rc = hipMemsetAsync(d_th, 0, sizeof(double) * C * D, m->stream) == hipSuccess
+               ? SX_OK : SX_EDEVICE;
+  std::string log;
+  int best_c = sx::kHierChunk;
+  double best_ms = 1e30;
+  for (int chunk : {128, 256, 512, 1024}) {
+    if (rc) break;
+    rc = hier_chunks(m, off, chunk);
+    if (rc) break;
+    sx::hier_workspace w;
+    for (int i = 0; i < 2 && rc == SX_OK; ++i)
+      rc = sx::hier_enqueue(m, w, d_th, C, d_out, 1, 1, fp32, m->stream);
+    float ms = 0.f;
+    const int reps = 5;
+    if (rc == SX_OK && hipEventRecord(m->ev0, m->stream) != hipSuccess) rc = SX_EDEVICE;
+    for (int i = 0; i < reps && rc == SX_OK; ++i)
+      rc = sx::hier_enqueue(m, w, d_th, C, d_out, 1, 1, fp32, m->stream);
+    if (rc == SX_OK && (hipEventRecord(m->ev1, m->stream) != hipSuccess ||
+                        hipEventSynchronize(m->ev1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess))
+      rc = SX_EDEVICE;
+    (void)hipStreamSynchronize(m->stream);
+    sx::free_workspace(w);
+    if (rc) break;
+    const double per = ms / reps;
+    char line[160];
+    std::snprintf(line, sizeof line, "hierarchical chunk=%d rows (%lld chunks): %.4f ms (%d chain%s)\n",
+                  chunk, static_cast<long long>(m->n_chunks), per, C, C > 1 ? "s" : "");
+    log += line;
+    if (per < best_ms) {
+      best_ms = per;
+      best_c = chunk;
+    }
+  }
+  (void)hipFree(d_th);
+  (void)hipFree(d_out);
+  m->tuning_log += log;
+  if (rc) return rc;
+  remember_tuning(key, tuned{best_c, 0, 0, 0});
+  char line[160];
+  std::snprintf(line, sizeof line, "hierarchical chunk chosen: %d rows (%.4f ms)\n", best_c, best_ms);
+  m->tuning_log += line;
+  return hier_chunks(m, off, best_c);
+}
+
+// Hierarchical model: the chunk tables (the caller's chunk size, else the
+// autotuned one, else kHierChunk), an FP32 copy of X for batched evaluations (SX_FP32), and the
+// single-chain partials.  The FP32 copy is sign-folded: row n holds
+// (2 y_n - 1) x_n.
+int hier_setup(sx_glm* m, const double* d_X, const double* X, const int32_t* y,
+               const int64_t* off, bool do_tune, int chunk) {
+  if (m->precision == SX_FP32 && m->N * m->K > 0) {
+    // 32 K + 64 zero floats past the last row: the FP32 pass stages whole
+    // 32-row tiles (64-float loads per lane) past a chunk's end
+    const size_t n = static_cast<size_t>(m->N * m->K);
+    const size_t K = static_cast<size_t>(m->K);
+    const size_t total = n + 32 * K + 64;
+    std::vector<float> x32(total, 0.0f);
+    for (size_t i = 0; i < n; ++i) {
+      const float v = static_cast<float>(X[i]);
+      x32[i] = y[i / K] ? v : -v;
+    }
+    SX_HIP(hipMalloc(&m->d_X32, total * sizeof(float)));
+    SX_HIP(hipMemcpy(m->d_X32, x32.data(), total * sizeof(float), hipMemcpyHostToDevice));
+  }
+  m->d_X = const_cast<double*>(d_X);  // the FP64 pass reads X row-major
+  const int rc = chunk > 0 ? hier_chunks(m, off, chunk)
+                 : do_tune && m->N * m->K > 0 ? hier_autotune(m, off)
+                                              : hier_chunks(m, off, sx::kHierChunk);
+  if (rc) return rc;
   return sx::hier_prepare(m, m->hw, 1);
 }
 
@@ -300,12 +397,13 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     set_error("sx_glm_create: invalid sizes or NULL data");
     return SX_EINVAL;
   }
-  sx_tuning tune{0, 0, -1, 0, -1};
+  sx_tuning tune{0, 0, -1, 0, -1, 0};
   if (tuning) tune = *tuning;
   if (tune.lanes_per_row < 0 || tune.lanes_per_row > 64 ||
       (tune.lanes_per_row & (tune.lanes_per_row - 1)) != 0 ||
       tune.slots_cap < 0 || tune.slots_cap > 32 || tune.slots_cap % 4 != 0 ||
-      tune.prefetch < -1 || tune.prefetch > 1 || tune.blocks < 0) {
+      tune.prefetch < -1 || tune.prefetch > 1 || tune.blocks < 0 || tune.hier_chunk < 0 ||
+      tune.hier_chunk > 4096 || tune.hier_chunk % 32 != 0) {
     set_error("sx_glm_create: invalid tuning");
     return SX_EINVAL;
   }
@@ -402,19 +500,20 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
       return fail(SX_EDEVICE);
     }
   }
+  const bool do_tune =
+      tune.autotune == 1 || (tune.autotune == -1 && tuning == nullptr && N * K >= (1 << 22));
   if (hier) {
-    rc = hier_setup(m, staging, X, group_offsets);
+    rc = hier_setup(m, staging, X, static_cast<const int32_t*>(y), group_offsets, do_tune,
+                    tune.hier_chunk);
     if (rc == SX_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = SX_EDEVICE;
     if (rc) {
+      m->d_X = nullptr;
       if (staging) (void)hipFree(staging);
       return fail(rc);
     }
-    m->d_X = staging;  // the FP64 pass reads X row-major
     *out = m;
     return SX_OK;
   }
-  const bool do_tune =
-      tune.autotune == 1 || (tune.autotune == -1 && tuning == nullptr && N * K >= (1 << 22));
   if (do_tune && N * K > 0) {
     rc = autotune(m, staging, cus);
   } else {

@@ -110,8 +110,8 @@ int64_t footprint_of(const sx_job_spec& s) {
   bytes += 8 * N * K;                                             // staging (transient)
   bytes += s.family == SX_NORMAL ? 8 * N : N;                     // y
   if (s.family == SX_HIER_LOGIT) {
-    bytes += 8 * N * K + (prec == SX_FP32 ? 4 * N * K : 0);
-    const int64_t chunks = ceil_div(N, sx::kHierChunk) + s.J;
+    bytes += 8 * N * K + (prec == SX_FP32 ? 4 * (N * K + 32 * K + 64) : 0);
+    const int64_t chunks = ceil_div(N, sx::kHierChunkMin) + s.J;  // any autotuned chunk size
     bytes += chunks * 16 + (s.J + 1) * 8;
     bytes += 8 * ((K + 3) * std::max<int64_t>(1, chunks) + (2 * K + 5) * s.J) * (C + 1);
   } else {
@@ -199,7 +199,7 @@ int start_job(sx_server* srv, job* j) {
   const sx_job_spec& s = j->spec;
   const int prec = gradient_precision(s);
   const int ctx_prec = prec == SX_BF16X2 ? SX_BF16X2 : prec == SX_FP32 ? SX_FP32 : SX_FP64;
-  sx_tuning tune{0, 0, 0, 0, 0};  // fixed launch shapes: results independent of load
+  sx_tuning tune{0, 0, 0, 0, 0, 0};  // fixed launch shapes: results independent of load
   const void* y = s.family == SX_NORMAL ? static_cast<const void*>(j->yd.data())
                                         : static_cast<const void*>(j->yi.data());
   int rc = sx_glm_create_ex(&j->m, s.family, j->X.data(), y, s.N, s.K,

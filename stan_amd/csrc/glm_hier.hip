@@ -164,69 +164,70 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
 #undef SX_LOAD_BLOCK
 }
 
-// The row term and residual of the matrix-core pass on the raw hardware
-// transcendentals: the exp2 argument is <= 0 and the log2 argument lies in
-// [1, 2], where v_exp_f32 and v_log_f32 need none of the range handling of
-// the library calls (an exp2 result below 2^-126 may flush to zero; it
-// enters u = 1 + t and r = s t / u only at that size).  s = 0 (a padding
-// row) gives term = r = 0 through the factor |s|, without a select.
-__device__ __forceinline__ void logit_row_lean(float e, float s, float* term, float* r) {
-  constexpr float kLog2e = 1.44269504088896341f, kLn2 = 0.693147180559945309f;
-  const float se = s * e;
-  const float t = __builtin_amdgcn_exp2f(-fabsf(se) * kLog2e);
-  const float u = 1.0f + t;
-  const float inv = __builtin_amdgcn_rcpf(u);
-  *term = -fabsf(s) * fmaf(__builtin_amdgcn_logf(u), kLn2, fmaxf(-se, 0.0f));
-  *r = s * (se >= 0.0f ? t * inv : inv);
-}
-
 // FP32 on the matrix cores: a wave per chunk and up to 64 chains (NH
-// halves of 32).  Within a group every chain's coefficients w (K) and
-// intercept b are fixed, so a 32-row tile of the chunk gives two small
-// products per half: the forward H (32 rows x 32 chains) = X W and the
-// adjoint S^T (32 chains x 32 k) += R^T X, both on
-// v_mfma_f32_32x32x2_f32 (FP32 products, FP32 accumulation), with the
-// logistic link on the VALU between them.  Operands (lane l, h = l / 32,
-// i = l % 32): A[i][h], B[h][i]; result d[j] = D[8 (j/4) + 4 h + j%4][i].
-//   forward  step s: A = X[row i][2s + h] (LDS), B = w[2s + h] of chain i
-//            (registers, zero past K)
-//   adjoint  step s: A = r of chain i at row 8 (s/4) + 4 h + s%4 (the
-//            lane's own forward element s), B = X[that row][col i]
+// halves of 32).  X is stored sign-folded, x' = s x with s = 2y - 1, and
+// each staged row carries s itself as column K, so one product per tile
+// and half gives the base-2 margin of every (row, chain):
+//   q = log2(e) s eta = sum_k x'_k w'_k + s b',  w' = log2(e) w, b' = log2(e) b
+// (w, b: the chain's coefficients and intercept in the chunk's group,
+// scaled by log2(e) in FP64 and rounded once).  With L = log2(1 + 2^-|q|)
+// the row's log density is -ln2 (L + max(-q, 0)) and its scaled residual
+// r' = sigma(-s eta) = 2^-(L + max(q, 0)), so the link costs three
+// hardware transcendentals (v_exp_f32, v_log_f32, v_exp_f32) and a few
+// adds.  The adjoint product sum_rows r' [x' | s] then yields the
+// gradient sums S_k = sum r x_k (columns k < K, since r = s r' and s^2 =
+// 1) and S_0 = sum r (column K; for K = 32, past the product's 32
+// columns, a VALU sum of r' s).  Both products are
+// v_mfma_f32_32x32x2_f32 (FP32 products and accumulation).  Operands
+// (lane l, h = l / 32, i = l % 32): A[i][h], B[h][i]; result d[j] =
+// D[8 (j/4) + 4 h + j%4][i].
+//   forward  step s: A = [x'|s][row i][2s + h] (LDS), B = w'[2s + h] of
+//            chain i (registers; b' at column K, zero past it)
+//   adjoint  step s: A = r' of chain i at row 8 (s/4) + 4 h + s%4 (the
+//            lane's own forward element s), B = [x'|s][that row][col i]
+//            (columns past K read whatever follows in LDS, always finite,
+//            and are never stored)
 // The tile (32 rows x K, contiguous in X) is loaded coalesced into
 // registers one tile ahead and placed in the wave's LDS at row stride
-// K | 1 (odd: the forward's 32 rows fall in distinct banks).  Rows past
-// the chunk have x = 0 and sign 0 and add nothing.
-template <int KMAX, int NH>
+// (K + 1) | 1 (odd: the forward's 32 rows fall in distinct banks).  Rows
+// past the chunk have x' = 0 and s = 0, so q = 0 exactly: their r' adds
+// nothing to the adjoint, and their log-density value L(0), computed once
+// by the same instruction, is subtracted per padding row.
+template <int K, int NH>
 __global__ __launch_bounds__(256) void hier_mfma_f32(hier_args a) {
-  static_assert(KMAX % 4 == 0 && KMAX <= 32, "K <= 32 in buckets of 4");
+  static_assert(K >= 0 && K <= 32, "K <= 32");
   static_assert(NH == 1 || NH == 2, "one or two halves of 32 chains");
-  constexpr int NJ = KMAX / 2;           // staged floats per lane (32 K / 64), forward steps
-  constexpr int TS = 32 * 33 + 1;        // LDS floats of a tile (stride <= 33)
-  __shared__ float sx[4][TS + 1];        // + a sink for staging lanes past the tile
-  __shared__ float sy[4][32];
+  constexpr int KS = (K + 1) | 1;          // LDS row stride
+  constexpr int NJ = (K + 2) / 2;          // forward steps: columns 0 .. 2 NJ - 1 >= K
+  constexpr int NS = (32 * K + 63) / 64;   // staged floats per lane
+  constexpr int TS = 31 * KS + 34;         // covers every forward and adjoint read
+  constexpr float kLog2e = 1.44269504088896341f;
+  __shared__ float sx[4][TS + 1];          // + a sink for staging lanes past the tile
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 31, h = lane >> 5;
-  const int K = a.K;
-  const int KS = K | 1;
   const int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w;
   if (ch >= a.n_chunks) return;  // whole wave; no block barrier follows
   const int g = a.chunk_group[ch];
   const int64_t r0 = a.chunk_row0[ch], r1 = a.chunk_row0[ch + 1];
   const int rows = static_cast<int>(r1 - r0);
   const int cb = blockIdx.y * (32 * NH);
-  // chain coefficients: half hf, chain cb + 32 hf + i, k = 2 s + h
-  float wk[NH][NJ], b[NH];
+  // chain coefficients: half hf, chain cb + 32 hf + i, column 2 s + h
+  float wk[NH][NJ];
 #pragma unroll
   for (int hf = 0; hf < NH; ++hf) {
     const int c = cb + 32 * hf + li;
     const double* th = a.theta + static_cast<int64_t>(c < a.chains ? c : 0) * a.theta_stride;
     const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
-    b[hf] = static_cast<float>(th[0] + exp(th[1 + K]) * z[0]);
 #pragma unroll
     for (int s = 0; s < NJ; ++s) {
       const int k = 2 * s + h;
-      wk[hf][s] = k < K ? static_cast<float>(th[1 + k] + exp(th[2 + K + k]) * z[1 + k]) : 0.f;
+      double v = 0.0;
+      if (k < K)
+        v = th[1 + k] + exp(th[2 + K + k]) * z[1 + k];
+      else if (k == K)
+        v = th[0] + exp(th[1 + K]) * z[0];
+      wk[hf][s] = static_cast<float>(v * 1.4426950408889634074);
     }
   }
   f32x16 sacc[NH];
@@ -234,130 +235,140 @@ __global__ __launch_bounds__(256) void hier_mfma_f32(hier_args a) {
   for (int hf = 0; hf < NH; ++hf)
 #pragma unroll
     for (int j = 0; j < 16; ++j) sacc[hf][j] = 0.f;
-  // nf: sum of 0 * eta, NaN once any eta is infinite or NaN.  A padding
-  // row's eta (0 * w + b) is non-finite only if the chain's w or b is, and
-  // then so is every real row's
-  float s1[NH], s0p[NH], nf[NH];
+  // s0: the lane's rows' log-density sums in base 2 (L + max(-q, 0)).
+  // They also carry the non-finite flag: |q| - q is NaN or +inf exactly
+  // when q is, so a non-finite q makes the sum non-finite.
   double s0[NH];
+  // K = 32: the sign column lies past the adjoint's 32 output columns;
+  // sum r' s (= sum r) on the VALU instead, per lane over its rows
+  float s1v[NH];
 #pragma unroll
   for (int hf = 0; hf < NH; ++hf) {
-    s1[hf] = 0.f;
-    nf[hf] = 0.f;
     s0[hf] = 0.0;
+    s1v[hf] = 0.f;
   }
-  // positions past row 31 or col K - 1 are read (times a zero) but never
+  const float lpad = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-0.0f));
+  // positions past row 31 or col K are read (times a zero) but never
   // written: zero them once
-  for (int e = lane; e < TS; e += 64) sx[w][e] = 0.f;
-  float stg[NJ];
-  float ystg = 0.f;
-  int lim = 0;
-  // (K = 0, intercepts only: nothing is staged; no division by zero)
-  const int Kd = K > 0 ? K : 1;
-  const int dr = 64 / Kd, dk = 64 % Kd;
-  const int row0 = lane / Kd, col0 = lane % Kd;
-  // branch-free staging: element indices past the tile's last row are
-  // clamped to it (an in-bounds load) and the value replaced by zero
-#define SX_LOAD_TILE(T0)                                                         \
-  {                                                                              \
-    const int cnt_ = rows - (T0) < 32 ? rows - (T0) : 32;                        \
-    lim = cnt_ * K;                                                              \
-    if (lim > 0) {                                                               \
-      const float* src_ = a.X32 + (r0 + (T0)) * K;                               \
-      _Pragma("unroll") for (int j = 0; j < NJ; ++j) {                           \
-        const int e_ = lane + 64 * j;                                            \
-        const float v_ = src_[e_ < lim ? e_ : lim - 1];                          \
-        stg[j] = e_ < lim ? v_ : 0.f;                                            \
-      }                                                                          \
-    }                                                                            \
-    const int yi_ = lane < cnt_ ? lane : cnt_ - 1;                               \
-    const float yv_ = a.y[r0 + (T0) + yi_] ? 1.0f : -1.0f;                       \
-    ystg = lane < cnt_ ? yv_ : 0.f;                                              \
+  for (int e = lane; e <= TS; e += 64) sx[w][e] = 0.f;
+  // LDS position of each staged float (row e / K, column e % K of the
+  // tile); the sink past the tile for e >= 32 K
+  constexpr int Kd = K > 0 ? K : 1;
+  int dst[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int e = lane + 64 * j;
+    dst[j] = e < 32 * K ? (e / Kd) * KS + e % Kd : TS;
+  }
+  // staging registers: the next tile's floats, its row count and the raw
+  // y byte of the lane's row.  Loads stay in flight through a whole tile;
+  // nothing reads them before the next tile's LDS writes.  A tile always
+  // loads 64 NS consecutive floats: X carries 32 K + 64 zero floats past
+  // its last row, so the reads past a chunk's end stay in bounds, and the
+  // values past the tile's rows are replaced by zero when written.
+  float stg[NS > 0 ? NS : 1];
+  int ybyte = 0, scnt = 0;
+#define SX_LOAD_TILE(T0)                                                          \
+  {                                                                               \
+    scnt = rows - (T0) < 32 ? rows - (T0) : 32;                                   \
+    const float* src_ = a.X32 + (r0 + (T0)) * K + lane;                           \
+    _Pragma("unroll") for (int j = 0; j < NS; ++j) stg[j] = src_[64 * j];         \
+    ybyte = a.y[r0 + (T0) + (lane < scnt ? lane : scnt - 1)];                     \
   }
   SX_LOAD_TILE(0);
   for (int t0 = 0; t0 < rows; t0 += 32) {
+    const int cnt = scnt;
     // LDS accesses of one wave complete in order: the previous tile's reads
-    // are done before these writes land.  All 32 x K positions are written
-    // (zeros past the chunk).
+    // are done before these writes land.  All 32 x (K + 1) positions are
+    // written (zeros past the chunk).
     __builtin_amdgcn_wave_barrier();
-    {
-      int row = row0, col = col0;
+    if (cnt == 32) {
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        sx[w][lane + 64 * j < 32 * K ? row * KS + col : TS] = stg[j];
-        row += dr;
-        col += dk;
-        if (col >= K) {
-          col -= K;
-          ++row;
-        }
-      }
+      for (int j = 0; j < NS; ++j) sx[w][dst[j]] = stg[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) sx[w][dst[j]] = lane + 64 * j < cnt * K ? stg[j] : 0.f;
     }
-    if (lane < 32) sy[w][lane] = ystg;
+    if (lane < 32) sx[w][lane * KS + K] = lane < cnt ? (ybyte ? 1.0f : -1.0f) : 0.f;
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     if (t0 + 32 < rows) SX_LOAD_TILE(t0 + 32);  // in flight during this tile
+    float xa[NJ];
+#pragma unroll
+    for (int s = 0; s < NJ; ++s) xa[s] = sx[w][li * KS + 2 * s + h];
     f32x16 hacc[NH];
 #pragma unroll
     for (int hf = 0; hf < NH; ++hf)
 #pragma unroll
       for (int j = 0; j < 16; ++j) hacc[hf][j] = 0.f;
-    // KMAX / 2 steps (columns K .. KMAX - 1 meet w = 0; what they read is
-    // the next row's x or the zeroed tail of the buffer, finite either way)
 #pragma unroll
-    for (int s = 0; s < NJ; ++s) {
-      const float xa = sx[w][li * KS + 2 * s + h];
+    for (int s = 0; s < NJ; ++s)
 #pragma unroll
       for (int hf = 0; hf < NH; ++hf)
-        hacc[hf] = __builtin_amdgcn_mfma_f32_32x32x2f32(xa, wk[hf][s], hacc[hf], 0, 0, 0);
+        hacc[hf] = __builtin_amdgcn_mfma_f32_32x32x2f32(xa[s], wk[hf][s], hacc[hf], 0, 0, 0);
+    float xb[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) xb[s] = sx[w][(8 * (s >> 2) + 4 * h + (s & 3)) * KS + li];
+    // padding rows among the lane's 16 (rows 8 b + 4 h + 0..3)
+    int npad = 0;
+    if (cnt < 32) {
+#pragma unroll
+      for (int b4 = 0; b4 < 4; ++b4) {
+        const int p = 8 * b4 + 4 * h + 4 - cnt;
+        npad += p < 0 ? 0 : (p > 4 ? 4 : p);
+      }
     }
+    float sg[16];
+    if constexpr (K == 32) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) sg[j] = sx[w][(8 * (j >> 2) + 4 * h + (j & 3)) * KS + K];
+    }
+    // per (row, chain): L = log2(1 + 2^-|q|); log density (base 2, negated)
+    // L + max(-q, 0) = L + (|q| - q) / 2; r' = 2^-(L + (q + |q|) / 2).
+    // Both halves are exact.
 #pragma unroll
     for (int hf = 0; hf < NH; ++hf) {
-      s0p[hf] = 0.f;
+      float al = 0.f, am = 0.f;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const int row = 8 * (j >> 2) + 4 * h + (j & 3);
-        const float sg = sy[w][row];
-        const float e = hacc[hf][j] + b[hf];
-        float term, rr;
-        logit_row_lean(e, sg, &term, &rr);
-        nf[hf] = fmaf(e, 0.f, nf[hf]);
-        s0p[hf] += term;
-        s1[hf] += rr;
-        hacc[hf][j] = rr;
+        const float q = hacc[hf][j];
+        const float L = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-fabsf(q)));
+        al += L;
+        am += fabsf(q) - q;
+        hacc[hf][j] = __builtin_amdgcn_exp2f(-fmaf(0.5f, q + fabsf(q), L));
+        if constexpr (K == 32) s1v[hf] = fmaf(hacc[hf][j], sg[j], s1v[hf]);
       }
-      s0[hf] += static_cast<double>(s0p[hf]);
+      s0[hf] += static_cast<double>(fmaf(0.5f, am, al)) - static_cast<double>(npad) * lpad;
     }
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const int row = 8 * (s >> 2) + 4 * h + (s & 3);
-      const float xv = sx[w][row * KS + li];
-      const float xb = li < K ? xv : 0.f;
+    for (int s = 0; s < 16; ++s)
 #pragma unroll
       for (int hf = 0; hf < NH; ++hf)
-        sacc[hf] = __builtin_amdgcn_mfma_f32_32x32x2f32(hacc[hf][s], xb, sacc[hf], 0, 0, 0);
-    }
+        sacc[hf] = __builtin_amdgcn_mfma_f32_32x32x2f32(hacc[hf][s], xb[s], sacc[hf], 0, 0, 0);
   }
 #undef SX_LOAD_TILE
-  // sacc[hf][j] = S[chain cb + 32 hf + 8 (j/4) + 4 h + j%4][k = i]; the row
-  // terms of chain i are split over the two halves of the wave's rows
+  // sacc[hf][j] = column i of chain cb + 32 hf + 8 (j/4) + 4 h + j%4:
+  // sum r x_i for i < K (slot 1 + i), sum r for i = K (slot 0).  The
+  // log-density sums of chain i are split over the two halves of the
+  // wave's rows.
 #pragma unroll
   for (int hf = 0; hf < NH; ++hf) {
-    if (li < K) {
+    if (li <= K) {
+      const int slot = li < K ? 1 + li : 0;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int c = cb + 32 * hf + 8 * (j >> 2) + 4 * h + (j & 3);
-        if (c < a.chains) a.cpart[(ch * a.chains + c) * a.E + 1 + li] = sacc[hf][j];
+        if (c < a.chains) a.cpart[(ch * a.chains + c) * a.E + slot] = sacc[hf][j];
       }
     }
-    const float s1t = s1[hf] + __shfl_xor(s1[hf], 32, 64);
     const double s0t = s0[hf] + __shfl_xor(s0[hf], 32, 64);
-    const float nft = nf[hf] + __shfl_xor(nf[hf], 32, 64);
-    const int badt = nft == nft ? 0 : 1;
+    const int badt = isfinite(s0t) ? 0 : 1;
+    const float s1t = s1v[hf] + __shfl_xor(s1v[hf], 32, 64);
     const int c = cb + 32 * hf + li;
     if (h == 0 && c < a.chains) {
       double* out = a.cpart + (ch * a.chains + c) * a.E;
-      out[0] = s1t;
-      out[K + 1] = s0t;
+      if constexpr (K == 32) out[0] = s1t;
+      out[K + 1] = -0.69314718055994530942 * s0t;
       out[K + 2] = static_cast<double>(badt);
     }
   }
@@ -544,16 +555,15 @@ __global__ __launch_bounds__(1024) void hier_final(hier_args a) {
 
 template <int NH>
 hipError_t launch_hier_mfma(const hier_args& a, dim3 grid, hipStream_t s) {
-  switch ((a.K + 3) / 4) {
-    case 0:
-    case 1:
-    case 2: hipLaunchKernelGGL((hier_mfma_f32<8, NH>), grid, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((hier_mfma_f32<12, NH>), grid, dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((hier_mfma_f32<16, NH>), grid, dim3(256), 0, s, a); break;
-    case 5: hipLaunchKernelGGL((hier_mfma_f32<20, NH>), grid, dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL((hier_mfma_f32<24, NH>), grid, dim3(256), 0, s, a); break;
-    case 7: hipLaunchKernelGGL((hier_mfma_f32<28, NH>), grid, dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((hier_mfma_f32<32, NH>), grid, dim3(256), 0, s, a); break;
+  switch (a.K) {
+#define SX_CASE(KK) \
+  case KK: hipLaunchKernelGGL((hier_mfma_f32<KK, NH>), grid, dim3(256), 0, s, a); break;
+    SX_CASE(0) SX_CASE(1) SX_CASE(2) SX_CASE(3) SX_CASE(4) SX_CASE(5) SX_CASE(6) SX_CASE(7)
+    SX_CASE(8) SX_CASE(9) SX_CASE(10) SX_CASE(11) SX_CASE(12) SX_CASE(13) SX_CASE(14)
+    SX_CASE(15) SX_CASE(16) SX_CASE(17) SX_CASE(18) SX_CASE(19) SX_CASE(20) SX_CASE(21)
+    SX_CASE(22) SX_CASE(23) SX_CASE(24) SX_CASE(25) SX_CASE(26) SX_CASE(27) SX_CASE(28)
+    SX_CASE(29) SX_CASE(30) SX_CASE(31) SX_CASE(32)
+#undef SX_CASE
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

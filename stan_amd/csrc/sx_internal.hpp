@@ -91,8 +91,10 @@ hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C,
                             const mx_args& a, const pass_args& pa, hipStream_t s);
 
 // Hierarchical logistic model: group-aligned row chunks of at most
-// kHierChunk rows; partials per chunk and per group.
+// `chunk` rows (kHierChunk by default; the autotuner picks one of
+// 128 .. 1024 per data shape); partials per chunk and per group.
 constexpr int kHierChunk = 256;
+constexpr int kHierChunkMin = 128;
 struct hier_args {
   const double* X;          // N x K row-major (FP64 pass)
   const float* X32;         // N x K row-major (FP32 batched pass)
@@ -180,6 +182,7 @@ struct sx_glm {
   int* d_chunk_group = nullptr;
   int64_t *d_chunk_row0 = nullptr, *d_group_chunk0 = nullptr;
   int64_t n_chunks = 0;
+  int hier_chunk = 0;  // rows per chunk at most
   sx::hier_workspace hw;
   std::string tuning_log;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

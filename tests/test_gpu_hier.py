@@ -6,8 +6,9 @@ Bars:
   of the compensated oracle, relative (normwise for the gradient); a chain's
   batched result is bitwise its single evaluation.
   FP32 batched (X in FP32; FP32 matrix-core products and FP32 sums within
-  a chunk of at most 256 rows, FP32 row terms per 32-row tile added in
-  FP64; FP64 sums over chunks and groups), random test points
+  a chunk of at most 256 rows by default (128 .. 1024 when autotuned),
+  FP32 row terms per 32-row tile added in FP64; FP64 sums over chunks and
+  groups), random test points
   theta ~ N(0, 0.5^2):
   relative error of the log density <= LP_TOL_FP32 and normwise relative
   error of the gradient <= GRAD_TOL_FP32; the measured values are printed.
@@ -113,6 +114,63 @@ def test_hier_fp32_batch_against_oracle(J, K, size, C):
     st, lp_r, g_r = ref.lpgrad(th[0], mode=1)
     assert abs(lp64 - lp_r) <= TOL * abs(lp_r)
     m.close()
+
+
+@pytest.mark.parametrize("chunk", [32, 128, 512, 1024])
+def test_hier_chunk_sizes_against_oracle(chunk):
+    """Every chunk size the autotuner may pick (and the smallest allowed)
+    gives the oracle's values: FP64 single chain to 1e-12, FP32 batch to
+    its bars.  Groups of up to a few thousand rows span many chunks."""
+    d = _data(20, 6, 1500, seed=chunk)
+    _say(f"hier chunk={chunk} N={d['X'].shape[0]}")
+    ref = O.GlmData(O.HIER, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"])
+    th = W.random_thetas(2 * 6 + 2 + 20 * 7, 40, seed=chunk + 1) * 0.5
+    for precision in (S.FP64, S.FP32):
+        m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                       precision=precision, tuning=dict(hier_chunk=chunk))
+        if precision == S.FP64:
+            for c in range(3):
+                lp, g = m.log_prob_grad(th[c])
+                st, lp_r, g_r = ref.lpgrad(th[c], mode=1)
+                assert st == 0
+                assert abs(lp - lp_r) <= TOL * max(1.0, abs(lp_r)), (lp, lp_r)
+                assert np.linalg.norm(g - g_r) <= TOL * max(1.0, np.linalg.norm(g_r))
+        else:
+            lp, g = m.log_prob_grad_batch(th)
+            worst_lp = worst_g = 0.0
+            for c in range(th.shape[0]):
+                st, lp_r, g_r = ref.lpgrad(th[c], mode=1)
+                worst_lp = _nanmax(worst_lp, abs(lp[c] - lp_r) / abs(lp_r))
+                worst_g = _nanmax(worst_g, np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r))
+            _say(f"  fp32: max rel lp err {worst_lp:.2e}, max normwise grad err {worst_g:.2e}")
+            assert worst_lp < LP_TOL_FP32 and worst_g < GRAD_TOL_FP32
+        m.close()
+
+
+def test_hier_autotuned_contexts_are_bitwise_equal():
+    """The chunk size sets the order of the FP64 sums; the autotuner's first
+    decision per device and shape is reused, so two autotuned contexts of
+    one data set give bitwise-equal results (FP64 single chain and FP32
+    batch)."""
+    d = _data(150, 8, 4000, seed=77)
+    N = d["X"].shape[0]
+    assert N * 8 >= 1 << 22
+    _say(f"hier autotune N={N}")
+    for precision in (S.FP64, S.FP32):
+        ms = [S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                         precision=precision, tuning=dict(autotune=1)) for _ in range(2)]
+        log0, log1 = ms[0].tuning_log(), ms[1].tuning_log()
+        _say(log0.strip())
+        assert "hierarchical chunk" in log0 and "hierarchical chunk cached" in log1, (log0, log1)
+        th = W.random_thetas(ms[0].D, 8, seed=5) * 0.5
+        if precision == S.FP64:
+            a, b = ms[0].log_prob_grad(th[0]), ms[1].log_prob_grad(th[0])
+        else:
+            a, b = ms[0].log_prob_grad_batch(th), ms[1].log_prob_grad_batch(th)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        for m in ms:
+            m.close()
 
 
 def test_hier_fp32_chain_independent_of_batch():
