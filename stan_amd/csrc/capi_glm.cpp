@@ -486,15 +486,19 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     return fail(SX_ENOMEM);
   m->max_blocks = max_blocks;
 
-  // row-major staging copy on the device, tiled into the kernel layout
+  // row-major staging copy on the device, tiled into the kernel layout (the
+  // hierarchical model keeps it, with 64 K zero doubles past the last row:
+  // its FP64 pass loads whole 64-row blocks past a chunk's end)
   double* staging = nullptr;
   if (N * K > 0) {
-    if (hipMalloc(&staging, static_cast<size_t>(N * K) * sizeof(double)) != hipSuccess) {
+    const size_t pad = hier ? static_cast<size_t>(64 * K) : 0;
+    if (hipMalloc(&staging, (static_cast<size_t>(N * K) + pad) * sizeof(double)) != hipSuccess) {
       set_error("hipMalloc(staging X) failed");
       return fail(SX_ENOMEM);
     }
     if (hipMemcpy(staging, X, static_cast<size_t>(N * K) * sizeof(double),
-                  hipMemcpyHostToDevice) != hipSuccess) {
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        (pad > 0 && hipMemset(staging + N * K, 0, pad * sizeof(double)) != hipSuccess)) {
       (void)hipFree(staging);
       set_error("X upload failed");
       return fail(SX_EDEVICE);

@@ -43,41 +43,40 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // sums are formed by one wave in a fixed order (lane r takes rows r, r + 64,
 // ...; columns in order), so each chain's result depends neither on the
 // launch shape nor on the other chains.
-template <int KMAX>
+template <int K>
 __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
-  constexpr int BS = 64 * (KMAX | 1);  // LDS doubles of one block
-  __shared__ double sx[4][BS + 1];     // + a sink for positions past the block
-  __shared__ double sw[4][KMAX + 1];
+  static_assert(K >= 0 && K <= 32, "K <= 32");
+  constexpr int KS = K | 1;
+  constexpr int BS = 64 * KS;           // LDS doubles of one block
+  constexpr int Kd = K > 0 ? K : 1;
+  constexpr int DR = 64 / Kd, DK = 64 % Kd;
+  __shared__ double sx[4][BS + 1];      // + a sink for positions past the block
+  __shared__ double sw[4][K + 1];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int K = a.K;
-  const int KS = K | 1;
   const int c = blockIdx.y;
   const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
   const int64_t W = static_cast<int64_t>(gridDim.x) * 4;
-  // (K = 0, intercepts only: nothing is staged; no division by zero)
-  const int Kd = K > 0 ? K : 1;
-  const int dr = 64 / Kd, dk = 64 % Kd;
   const int row0 = lane / Kd, col0 = lane % Kd;
-  double stg[KMAX];
+  double stg[K > 0 ? K : 1];
   int8_t ystg = 0;
-  // loads of rows [base, base + 64) of chunk ch into registers; element
-  // indices past the block are clamped to its last element (an in-bounds
-  // load) and their values never stored
+  // loads of rows [base, base + 64) of chunk ch into registers: always 64 K
+  // consecutive doubles (X carries 64 K zero doubles past its last row, so
+  // a block past a chunk's end stays in bounds); the values past the
+  // chunk's rows are never stored
 #define SX_LOAD_BLOCK(CH, BASE)                                                  \
   {                                                                              \
     const int64_t e1_ = a.chunk_row0[(CH) + 1];                                  \
     const int cnt_ = static_cast<int>(e1_ - (BASE) < 64 ? e1_ - (BASE) : 64);    \
-    const int lim_ = cnt_ * K;                                                   \
-    const double* src_ = a.X + (BASE) * K;                                       \
-    _Pragma("unroll") for (int j = 0; j < KMAX; ++j) {                           \
-      const int e_ = lane + 64 * j;                                              \
-      stg[j] = (j < K && lim_ > 0) ? src_[e_ < lim_ ? e_ : lim_ - 1] : 0.0;      \
-    }                                                                            \
+    const double* src_ = a.X + (BASE) * K + lane;                                \
+    _Pragma("unroll") for (int j = 0; j < K; ++j) stg[j] = src_[64 * j];         \
     ystg = a.y[(BASE) + (lane < cnt_ ? lane : cnt_ - 1)];                        \
   }
   int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w;
   if (ch >= a.n_chunks) return;  // whole wave; no block barrier follows
+  // positions a block leaves unwritten are read by the rows past a chunk's
+  // end (their residual is 0): they must hold finite values from the start
+  for (int e = lane; e <= BS; e += 64) sx[w][e] = 0.0;
   int64_t base = a.chunk_row0[ch];
   SX_LOAD_BLOCK(ch, base);
   while (ch < a.n_chunks) {
@@ -88,9 +87,9 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
       const double tau = exp(th[1 + K + lane]);
       sw[w][lane] = lane == 0 ? th[0] + tau * z[0] : th[lane] + tau * z[lane];
     }
-    double acc[KMAX];
+    double acc[K > 0 ? K : 1];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) acc[k] = 0.0;
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
     double s0 = 0.0, s1 = 0.0, bad = 0.0;
     for (; base < r1; base += 64) {
       const int cnt = static_cast<int>(r1 - base < 64 ? r1 - base : 64);
@@ -101,10 +100,10 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
         const int lim = cnt * K;
         int row = row0, col = col0;
 #pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-          if (j < K) sx[w][lane + 64 * j < lim ? row * KS + col : BS] = stg[j];
-          row += dr;
-          col += dk;
+        for (int j = 0; j < K; ++j) {
+          sx[w][lane + 64 * j < lim ? row * KS + col : BS] = stg[j];
+          row += DR;
+          col += DK;
           if (col >= Kd) {
             col -= Kd;
             ++row;
@@ -124,13 +123,11 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
         if (nch < a.n_chunks) SX_LOAD_BLOCK(nch, nbase);
       }
       const bool valid = lane < cnt;
+      // lane r: row r of the block (columns in order); rows past the chunk
+      // read earlier blocks' rows or zeros (finite) and get r = 0
       double e = sw[w][0];
-      double xv[KMAX];
 #pragma unroll
-      for (int k = 0; k < KMAX; ++k) {
-        xv[k] = (valid && k < K) ? sx[w][lane * KS + k] : 0.0;
-        e = fma(xv[k], k < K ? sw[w][k + 1] : 0.0, e);
-      }
+      for (int k = 0; k < K; ++k) e = fma(sx[w][lane * KS + k], sw[w][k + 1], e);
       double term = 0.0, r = 0.0;
       if (valid) {
         stanx::math::sx_logit_row(e, yv ? 1.0 : -1.0, &term, &r);
@@ -138,18 +135,18 @@ __global__ __launch_bounds__(256) void hier_pass_f64(hier_args a) {
       }
       s0 += term;
       s1 += r;
+      // read the row again rather than hold it through the link
+      asm volatile("" ::: "memory");
 #pragma unroll
-      for (int k = 0; k < KMAX; ++k) acc[k] = fma(xv[k], r, acc[k]);
+      for (int k = 0; k < K; ++k) acc[k] = fma(sx[w][lane * KS + k], r, acc[k]);
     }
     double* out = a.cpart + (ch * a.chains + c) * a.E;
     s1 = wave_sum(s1);
     if (lane == 0) out[0] = s1;
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-      if (k < K) {
-        const double v = wave_sum(acc[k]);
-        if (lane == 0) out[1 + k] = v;
-      }
+    for (int k = 0; k < K; ++k) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) out[1 + k] = v;
     }
     s0 = wave_sum(s0);
     bad = wave_sum(bad);
@@ -577,16 +574,17 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s) {
     if (!fp32) {
       const int64_t want = (a.n_chunks + 3) / 4;
       const dim3 grid(static_cast<unsigned>(want < 2048 ? want : 2048), static_cast<unsigned>(a.chains));
-      if (a.K <= 4)
-        hipLaunchKernelGGL((hier_pass_f64<4>), grid, dim3(256), 0, s, a);
-      else if (a.K <= 8)
-        hipLaunchKernelGGL((hier_pass_f64<8>), grid, dim3(256), 0, s, a);
-      else if (a.K <= 16)
-        hipLaunchKernelGGL((hier_pass_f64<16>), grid, dim3(256), 0, s, a);
-      else if (a.K <= 24)
-        hipLaunchKernelGGL((hier_pass_f64<24>), grid, dim3(256), 0, s, a);
-      else
-        hipLaunchKernelGGL((hier_pass_f64<32>), grid, dim3(256), 0, s, a);
+      switch (a.K) {
+#define SX_CASE(KK) \
+  case KK: hipLaunchKernelGGL((hier_pass_f64<KK>), grid, dim3(256), 0, s, a); break;
+        SX_CASE(0) SX_CASE(1) SX_CASE(2) SX_CASE(3) SX_CASE(4) SX_CASE(5) SX_CASE(6) SX_CASE(7)
+        SX_CASE(8) SX_CASE(9) SX_CASE(10) SX_CASE(11) SX_CASE(12) SX_CASE(13) SX_CASE(14)
+        SX_CASE(15) SX_CASE(16) SX_CASE(17) SX_CASE(18) SX_CASE(19) SX_CASE(20) SX_CASE(21)
+        SX_CASE(22) SX_CASE(23) SX_CASE(24) SX_CASE(25) SX_CASE(26) SX_CASE(27) SX_CASE(28)
+        SX_CASE(29) SX_CASE(30) SX_CASE(31) SX_CASE(32)
+#undef SX_CASE
+        default: return hipErrorInvalidValue;
+      }
     } else {
       const dim3 grid(static_cast<unsigned>((a.n_chunks + 3) / 4), static_cast<unsigned>((a.chains + 31) / 32));
       const hipError_t e = launch_hier_mfma<1>(a, grid, s);
