@@ -61,23 +61,52 @@ __device__ __forceinline__ void split(float x, __bf16* hi, __bf16* lo) {
   *lo = to_bf16(x - static_cast<float>(h));
 }
 
-// FP32 Bernoulli-logit row on the raw hardware transcendentals: term =
-// -(max(-s eta, 0) + log(1 + t)), t = exp(-|s eta|), r = s inv_logit(-s eta).
-// The exp2 argument is <= 0 and the log2 argument lies in [1, 2], where
-// v_exp_f32 and v_log_f32 need none of the range handling of the library
-// calls (an exp2 result below 2^-126 may flush to zero; it enters u = 1 + t
-// and r = s t / u only at that size).  Absolute error of term ~1e-7,
-// relative error of r a few ulp: below the BF16 hi+lo representation error
-// of the products that feed eta.  s = 0 (a row past N) gives term = r = 0
-// through the factor |s|, without a select.
-__device__ __forceinline__ void logit_row_fast(float eta, float s, float* term, float* r) {
-  constexpr float kLog2e = 1.44269504088896341f, kLn2 = 0.693147180559945309f;
-  const float se = s * eta;
-  const float t = __builtin_amdgcn_exp2f(-fabsf(se) * kLog2e);
-  const float u = 1.0f + t;
-  const float inv = __builtin_amdgcn_rcpf(u);
-  *term = -fabsf(s) * fmaf(__builtin_amdgcn_logf(u), kLn2, fmaxf(-se, 0.0f));
-  *r = s * (se >= 0.0f ? t * inv : inv);
+// One (row, chain) of the link, FP32.  Logistic: the chain's coefficients
+// and intercept arrive scaled by log2(e) (prep_chains), so e2 = log2(e) eta
+// and, with the row's sign s = 2y - 1 and q = s e2,
+//   L = log2(1 + 2^-|q|),  log density = -ln2 (L + (|q| - q) / 2),
+//   r = s inv_logit(-s eta) = s 2^-(L + (q + |q|) / 2)
+// on three hardware transcendentals (v_exp_f32, v_log_f32, v_exp_f32; the
+// exp2 arguments are <= 0 and the log2 argument lies in [1, 2], where the
+// instructions need none of the library calls' range handling).  |s| L is
+// added to l2 and |q| - q to m2 (both halves of max(+-q, 0) are exact), so a
+// row past N (s = 0) adds nothing, and a non-finite q makes m2 non-finite
+// (|q| - q is NaN or +inf exactly when q is): the caller flags a non-finite
+// log density.  Normal: r = y - eta (0 past N), l2 += r^2, and the
+// non-finite flag nf is a sum of 0 * eta.
+template <int FAM>
+__device__ __forceinline__ float link_row(float h, float alpha, float yv, bool live, float& l2,
+                                          float& m2, float& nf) {
+  if (FAM == SX_LOGIT) {
+    const float q = yv * (h + alpha);
+    const float L = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-fabsf(q)));
+    l2 = fmaf(fabsf(yv), L, l2);
+    m2 += fabsf(q) - q;
+    return yv * __builtin_amdgcn_exp2f(-fmaf(0.5f, q + fabsf(q), L));
+  } else {
+    const float eta = h + alpha;
+    const float rr = live ? yv - eta : 0.f;
+    nf = fmaf(eta, 0.f, nf);
+    l2 = fmaf(rr, rr, l2);
+    return rr;
+  }
+}
+
+// A tile's row sums as the log-density contribution the reductions expect:
+// the logistic log density (natural units); the normal sum of squares.
+template <int FAM>
+__device__ __forceinline__ double tile_lp(float l2, float m2) {
+  if (FAM == SX_LOGIT)
+    return -0.69314718055994530942 * (static_cast<double>(l2) + 0.5 * static_cast<double>(m2));
+  return static_cast<double>(l2);
+}
+
+// The pass's status word: 1 when a row's linear predictor (or the log
+// density it gives) is not finite.
+template <int FAM>
+__device__ __forceinline__ double bad_flag(double lp, float nf) {
+  if (FAM == SX_LOGIT) return isfinite(lp) ? 0.0 : 1.0;
+  return nf == nf ? 0.0 : 1.0;
 }
 
 // Row-major FP64 X -> k-packed BF16 hi and lo (Np x Kp, zero padded).
@@ -111,12 +140,13 @@ __global__ void prep_y(const void* __restrict__ y, float* yf, int family, int64_
 }
 
 // theta (C x D, FP64) -> B as three BF16 terms (Cp x Kp each: the FP32
-// value of beta exactly), alpha (Cp)
+// value of scale x beta exactly), alpha (Cp, times scale); scale is log2(e)
+// for the logistic link (base 2), 1 for the normal one
 __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, int K, int Kp,
-                            __bf16* Bh, __bf16* Bl, __bf16* Bm, float* alpha) {
+                            double scale, __bf16* Bh, __bf16* Bl, __bf16* Bm, float* alpha) {
   const int c = blockIdx.x;
   for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
-    const float b = (c < C && k < K) ? static_cast<float>(theta[c * D + 1 + k]) : 0.0f;
+    const float b = (c < C && k < K) ? static_cast<float>(scale * theta[c * D + 1 + k]) : 0.0f;
     const __bf16 h = to_bf16(b);
     const float r1 = b - static_cast<float>(h);  // exact: at most 16 significant bits
     const __bf16 l = to_bf16(r1);
@@ -126,7 +156,7 @@ __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, 
     Bl[o] = l;
     Bm[o] = m;
   }
-  if (threadIdx.x == 0) alpha[c] = c < C ? static_cast<float>(theta[c * D]) : 0.0f;
+  if (threadIdx.x == 0) alpha[c] = c < C ? static_cast<float>(scale * theta[c * D]) : 0.0f;
 }
 
 // Staged tile layout in LDS: row r of a tile occupies ROWB bytes; its
@@ -243,7 +273,7 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
   bf16x8 ah = tr_operand(xb, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
   bf16x8 al = tr_operand(xb + TB, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
   const int64_t n0 = t * 32;
-  float lpt = 0.f, srt = 0.f;
+  float l2 = 0.f, m2 = 0.f, srt = 0.f;
 #pragma unroll
   for (int st = 0; st < 2; ++st) {
     bf16x8 rh, rl;
@@ -252,19 +282,7 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
       const int j = 8 * st + jj;
       const int row = 8 * (j >> 2) + 4 * h + (j & 3);
       const float yv = Y[row];
-      const float eta = H[j] + alpha;
-      float term, rr;
-      if (FAM == SX_LOGIT) {
-        // rows past N carry sign 0: term = r = 0
-        logit_row_fast(eta, yv, &term, &rr);
-      } else {
-        rr = n0 + row < a.N ? yv - eta : 0.f;
-        term = rr * rr;
-      }
-      // NaN once any eta is infinite or NaN; a row past N has eta =
-      // alpha, non-finite only if every row's is
-      nf = fmaf(eta, 0.f, nf);
-      lpt += term;
+      const float rr = link_row<FAM>(H[j], alpha, yv, n0 + row < a.N, l2, m2, nf);
       srt += rr;
       __bf16 hh, ll;
       split(rr, &hh, &ll);
@@ -288,7 +306,7 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
       al = nl;
     }
   }
-  lp += static_cast<double>(lpt);
+  lp += tile_lp<FAM>(l2, m2);
   sr += static_cast<double>(srt);
 }
 
@@ -394,7 +412,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   lp += __shfl_xor(lp, 32, 64);
   sr += __shfl_xor(sr, 32, 64);
   nf += __shfl_xor(nf, 32, 64);
-  const double bad = nf == nf ? 0.0 : 1.0;
+  const double bad = bad_flag<FAM>(lp, nf);
   if (h == 0) {
     double* sp = a.spart + (pbase + chain) * 3;
     sp[0] = lp;
@@ -525,7 +543,7 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
       H[4 * q + 3] += o.w;
     }
     const int64_t n0 = t * 32;
-    float lpt = 0.f, srt = 0.f;
+    float l2 = 0.f, m2 = 0.f, srt = 0.f;
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
       bf16x8 rh, rl;
@@ -534,16 +552,7 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
         const int j = 8 * st + jj;
         const int row = 8 * (j >> 2) + 4 * h + (j & 3);
         const float yv = Y[row];
-        const float eta = H[j] + alpha;
-        float term, rr;
-        if (FAM == SX_LOGIT) {
-          logit_row_fast(eta, yv, &term, &rr);
-        } else {
-          rr = n0 + row < a.N ? yv - eta : 0.f;
-          term = rr * rr;
-        }
-        nf = fmaf(eta, 0.f, nf);
-        lpt += term;
+        const float rr = link_row<FAM>(H[j], alpha, yv, n0 + row < a.N, l2, m2, nf);
         srt += rr;
         __bf16 hh, ll;
         split(rr, &hh, &ll);
@@ -566,7 +575,7 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
       }
     }
     if (kh == 0) {
-      lp += static_cast<double>(lpt);
+      lp += tile_lp<FAM>(l2, m2);
       sr += static_cast<double>(srt);
     }
   };
@@ -597,7 +606,7 @@ __global__ __launch_bounds__(128 * NCB) void glm_mfma32_pair(mx_args a) {
     lp += __shfl_xor(lp, 32, 64);
     sr += __shfl_xor(sr, 32, 64);
     nf += __shfl_xor(nf, 32, 64);
-    const double bad = nf == nf ? 0.0 : 1.0;
+    const double bad = bad_flag<FAM>(lp, nf);
     if (h == 0) {
       double* sp = a.spart + (pbase + chain) * 3;
       sp[0] = lp;
@@ -703,7 +712,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
       fl = nl;
     }
     const int64_t n0 = t * 32;
-    float lpt = 0.f, srt = 0.f;
+    float l2 = 0.f, m2 = 0.f, srt = 0.f;
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
       bf16x8 rh, rl;
@@ -712,16 +721,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
         const int j = 8 * st + jj;
         const int row = 8 * (j >> 2) + 4 * h + (j & 3);
         const float yv = Y[row];
-        const float eta = H[j] + alpha;
-        float term, rr;
-        if (FAM == SX_LOGIT) {
-          logit_row_fast(eta, yv, &term, &rr);
-        } else {
-          rr = n0 + row < a.N ? yv - eta : 0.f;
-          term = rr * rr;
-        }
-        nf = fmaf(eta, 0.f, nf);
-        lpt += term;
+        const float rr = link_row<FAM>(H[j], alpha, yv, n0 + row < a.N, l2, m2, nf);
         srt += rr;
         __bf16 hh, ll;
         split(rr, &hh, &ll);
@@ -732,7 +732,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
       Rh[ri] = rh;
       Rl[ri] = rl;
     }
-    lp += static_cast<double>(lpt);
+    lp += tile_lp<FAM>(l2, m2);
     sr += static_cast<double>(srt);
   };
   if (t0 < t1) SX_STAGE(t0, xa, ya);
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32_fwd(mx_args a) {
   lp += __shfl_xor(lp, 32, 64);
   sr += __shfl_xor(sr, 32, 64);
   nf += __shfl_xor(nf, 32, 64);
-  const double bad = nf == nf ? 0.0 : 1.0;
+  const double bad = bad_flag<FAM>(lp, nf);
   if (h == 0) {
     double* sp = a.spart + (pbase + chain) * 3;
     sp[0] = lp;
@@ -1033,6 +1033,7 @@ hipError_t launch_prep_y(const void* y, float* yf, int family, int64_t N, int64_
 hipError_t launch_mfma_pass(int family, const double* theta, int64_t D, int C, const mx_args& a,
                             const pass_args& pa, hipStream_t s) {
   hipLaunchKernelGGL(prep_chains, dim3(a.Cp), dim3(256), 0, s, theta, D, C, pa.K, a.Kp,
+                     family == SX_LOGIT ? 1.4426950408889634074 : 1.0,
                      reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bh)),
                      reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bl)),
                      reinterpret_cast<__bf16*>(const_cast<uint16_t*>(a.Bm)),
