@@ -381,6 +381,7 @@ __global__ __launch_bounds__(256) void hier_mfma_f32(hier_args a) {
 // coalesced.  sum z^2 is a fixed-order shuffle reduction over the chain's
 // 32 threads.
 constexpr int kGroupsPerBlock = 8;
+template <int NR>  // entries per thread: 1 for K + 3 <= 32, else 2
 __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
   const int c = blockIdx.y * 8 + (threadIdx.x >> 5);
   const int e = threadIdx.x & 31;
@@ -390,21 +391,50 @@ __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
   double* out = a.out + static_cast<int64_t>(c) * a.out_stride;
   // entries e and e + 32 (K <= 32: at most two per thread)
   const double tau0 = e <= K ? exp(th[1 + K + e]) : 0.0;
-  const double tau1 = e + 32 <= K ? exp(th[1 + K + e + 32]) : 0.0;
+  const double tau1 = NR > 1 && e + 32 <= K ? exp(th[1 + K + e + 32]) : 0.0;
   const int g0 = blockIdx.x * kGroupsPerBlock;
   const int g1 = g0 + kGroupsPerBlock < a.J ? g0 + kGroupsPerBlock : a.J;
-  double sv[2] = {0.0, 0.0}, szs[2] = {0.0, 0.0}, szz = 0.0;  // sums over the block's groups
+  double sv[NR], szs[NR], szz = 0.0;  // sums over the block's groups
+#pragma unroll
+  for (int r = 0; r < NR; ++r) sv[r] = szs[r] = 0.0;
+  // the first (up to) eight chunk partials and the z entries of a group,
+  // loaded one group ahead so that the next group's loads are in flight
+  // while this one is summed
+  double xn[NR][8], zn[NR];
+  auto load_head = [&](int g) {
+    const int64_t c0 = a.group_chunk0[g], c1 = a.group_chunk0[g + 1];
+    const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int ee = e + 32 * r;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        xn[r][u] = (ee < K + 3 && c0 + u < c1) ? a.cpart[((c0 + u) * a.chains + c) * a.E + ee] : 0.0;
+      zn[r] = ee <= K ? z[ee] : 0.0;
+    }
+  };
+  if (g0 < g1) load_head(g0);
   for (int g = g0; g < g1; ++g) {
     const int64_t ch0 = a.group_chunk0[g], ch1 = a.group_chunk0[g + 1];
-    const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+    double xc[NR][8], zc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      zc[r] = zn[r];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xc[r][u] = xn[r][u];
+    }
+    if (g + 1 < g1) load_head(g + 1);
     double zz = 0.0;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < NR; ++r) {
       const int ee = e + 32 * r;
       if (ee >= K + 3) continue;
-      // chunks in order; eight loads in flight at a time
+      // chunks in order: the preloaded head, then eight loads in flight
       double v = 0.0;
-      int64_t ch = ch0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (ch0 + u < ch1) v += xc[r][u];
+      int64_t ch = ch0 + 8;
       for (; ch + 8 <= ch1; ch += 8) {
         double x[8];
 #pragma unroll
@@ -416,7 +446,7 @@ __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
       sv[r] += v;
       if (ee <= K) {
         const double tau = r == 0 ? tau0 : tau1;
-        const double zv = z[ee];
+        const double zv = zc[r];
         out[1 + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1) + ee] = tau * v - zv;
         szs[r] += tau * zv * v;
         zz += zv * zv;
@@ -429,7 +459,7 @@ __global__ __launch_bounds__(256) void hier_groups(hier_args a) {
   const int nblk = (a.J + kGroupsPerBlock - 1) / kGroupsPerBlock;
   double* gs = a.gpart + (static_cast<int64_t>(c) * nblk + blockIdx.x) * a.G;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < NR; ++r) {
     const int ee = e + 32 * r;
     if (ee >= K + 3) continue;
     if (ee <= K) {
@@ -593,8 +623,11 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(hier_groups, dim3((a.J + kGroupsPerBlock - 1) / kGroupsPerBlock, (a.chains + 7) / 8),
-                     dim3(256), 0, s, a);
+  const dim3 ggrid((a.J + kGroupsPerBlock - 1) / kGroupsPerBlock, (a.chains + 7) / 8);
+  if (a.K + 3 <= 32)
+    hipLaunchKernelGGL(hier_groups<1>, ggrid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(hier_groups<2>, ggrid, dim3(256), 0, s, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(hier_final, dim3(a.chains), dim3(1024), 0, s, a);
