@@ -392,6 +392,9 @@ def kernels_section(steps, sampler_iters=30, only=None):
         d = gen()
         m = S.GlmModel(fam, d["X"], d["y"], offsets=d.get("offsets"), prior=d["prior"],
                        precision=prec)
+        for line in m.tuning_log().splitlines():
+            if "chosen" in line or "cached" in line:
+                log(f"  {line}")
         th = W.random_thetas(m.D, C, seed=7) * 0.5
         ms, _ = m.time_log_prob_grad(th, iters=steps, warmup=3, chains=C)
         out[name] = dict(ms_per_batched_eval=ms, chains=C)

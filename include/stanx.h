@@ -119,6 +119,9 @@ int sx_glm_create(sx_glm** out, int family, const double* X, const void* y,
  *                  variant and row split, the hierarchical chunk size
  *   hier_chunk     SX_HIER_LOGIT: rows per chunk at most (a multiple of 32,
  *                  <= 4096; 0: autotuned, else 256)
+ *   hier_kernel    SX_HIER_LOGIT with SX_FP32: 1 = FP32 MFMA products,
+ *                  2 = products of exact three-term BF16 splits (K <= 31;
+ *                  above, 1), 0 = autotuned, else 1
  * No counterpart in the reference (its CPU GLM has no tunables).         */
 typedef struct sx_tuning {
   int lanes_per_row;
@@ -127,6 +130,7 @@ typedef struct sx_tuning {
   int blocks;
   int autotune;
   int hier_chunk;
+  int hier_kernel;
 } sx_tuning;
 
 /* sx_glm_create with explicit tuning (NULL = defaults). */

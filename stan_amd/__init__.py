@@ -56,7 +56,8 @@ class Prior(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lanes_per_row", C.c_int), ("slots_cap", C.c_int), ("prefetch", C.c_int),
-                ("blocks", C.c_int), ("autotune", C.c_int), ("hier_chunk", C.c_int)]
+                ("blocks", C.c_int), ("autotune", C.c_int), ("hier_chunk", C.c_int),
+                ("hier_kernel", C.c_int)]
 
 
 class NutsConfig(C.Structure):
@@ -239,7 +240,8 @@ class GlmModel:
         self.prior = Prior(*prior)
         tn = None if tuning is None else Tuning(**{**dict(lanes_per_row=0, slots_cap=0,
                                                              prefetch=-1, blocks=0,
-                                                             autotune=0, hier_chunk=0),
+                                                             autotune=0, hier_chunk=0,
+                                                             hier_kernel=0),
                                                         **tuning})
         _check(lib().sx_glm_create_ex(C.byref(self._h), family, _ptr(X), _ptr(y), N, K,
                                       _ptr(off), J, precision, C.byref(self.prior), device,

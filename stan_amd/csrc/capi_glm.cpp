@@ -238,22 +238,28 @@ int hier_chunks(sx_glm* m, const int64_t* off, int chunk) {
   return SX_OK;
 }
 
-// Chunk size of the hierarchical model, timed on the device at context
-// creation: 128, 256, 512 or 1024 rows, each timed as full evaluations at
-// theta = 0 (the kernels' work does not depend on the values) -- one chain
-// in FP64 for an FP64 context, a batch of 64 chains for an FP32 one.  The
-// chunk sets the order of the FP64 sums, so the first decision per
-// (device, precision, N, J, K) is kept for the process and in
-// STANX_TUNING_CACHE (family field 200 + [FP32], K field 64 J + K).
-int hier_autotune(sx_glm* m, const int64_t* off) {
+// Chunk size and FP32 product kernel of the hierarchical model, timed on
+// the device at context creation: 128, 256, 512 or 1024 rows, and for an
+// FP32 context with K <= 31 the FP32 MFMA products or the BF16 three-term
+// split products, each timed as full evaluations at theta = 0 (the
+// kernels' work does not depend on the values) -- one chain in FP64 for an
+// FP64 context, a batch of 64 chains for an FP32 one.  Both choices set
+// the order (or rounding) of sums, so the first decision per (device,
+// precision, N, J, K, forced kernel) is kept for the process and in
+// STANX_TUNING_CACHE (family field 200 + [FP32] + 10 x forced kernel, K
+// field 64 J + K).
+int hier_autotune(sx_glm* m, const int64_t* off, int kernel) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   read_tune_file();
   const bool fp32 = m->precision == SX_FP32;
-  const std::string key = tune_key(m->device, 200 + (fp32 ? 1 : 0), m->N, 64 * m->J + m->K);
+  const std::string key =
+      tune_key(m->device, 200 + (fp32 ? 1 : 0) + 10 * kernel, m->N, 64 * m->J + m->K);
   const auto hit = g_tuned.find(key);
   if (hit != g_tuned.end()) {
+    m->hier_x6 = hit->second.smax;
     char line[160];
-    std::snprintf(line, sizeof line, "hierarchical chunk cached: %d rows\n", hit->second.L);
+    std::snprintf(line, sizeof line, "hierarchical chunk cached: %d rows, %s products\n",
+                  hit->second.L, m->hier_x6 ? "BF16 split" : "FP32");
     m->tuning_log += line;
     return hier_chunks(m, off, hit->second.L);
   }
@@ -269,10 +275,16 @@ int hier_autotune(sx_glm* m, const int64_t* off) {
   int rc = hipMemsetAsync(d_th, 0, sizeof(double) * C * D, m->stream) == hipSuccess
                ? SX_OK : SX_EDEVICE;
   std::string log;
-  int best_c = sx::kHierChunk;
+  int best_c = sx::kHierChunk, best_k = 0;
   double best_ms = 1e30;
-  for (int chunk : {128, 256, 512, 1024}) {
+  const bool can_x6 = fp32 && m->K <= 31;
+  const int k_lo = kernel == 2 && can_x6 ? 1 : 0;
+  const int k_hi = kernel == 1 || !can_x6 ? 0 : 1;
+  for (int cand = 0; cand < 4 * (k_hi - k_lo + 1); ++cand) {
+    const int chunk = 128 << (cand / (k_hi - k_lo + 1));
+    const int kx6 = k_lo + cand % (k_hi - k_lo + 1);
     if (rc) break;
+    m->hier_x6 = kx6;
     rc = hier_chunks(m, off, chunk);
     if (rc) break;
     sx::hier_workspace w;
@@ -292,21 +304,26 @@ int hier_autotune(sx_glm* m, const int64_t* off) {
     if (rc) break;
     const double per = ms / reps;
     char line[160];
-    std::snprintf(line, sizeof line, "hierarchical chunk=%d rows (%lld chunks): %.4f ms (%d chain%s)\n",
-                  chunk, static_cast<long long>(m->n_chunks), per, C, C > 1 ? "s" : "");
+    std::snprintf(line, sizeof line,
+                  "hierarchical chunk=%d rows (%lld chunks)%s: %.4f ms (%d chain%s)\n", chunk,
+                  static_cast<long long>(m->n_chunks), fp32 ? (kx6 ? ", BF16 split products" : ", FP32 products") : "",
+                  per, C, C > 1 ? "s" : "");
     log += line;
     if (per < best_ms) {
       best_ms = per;
       best_c = chunk;
+      best_k = kx6;
     }
   }
+  m->hier_x6 = best_k;
   (void)hipFree(d_th);
   (void)hipFree(d_out);
   m->tuning_log += log;
   if (rc) return rc;
-  remember_tuning(key, tuned{best_c, 0, 0, 0});
+  remember_tuning(key, tuned{best_c, best_k, 0, 0});
   char line[160];
-  std::snprintf(line, sizeof line, "hierarchical chunk chosen: %d rows (%.4f ms)\n", best_c, best_ms);
+  std::snprintf(line, sizeof line, "hierarchical chunk chosen: %d rows, %s products (%.4f ms)\n",
+                best_c, best_k ? "BF16 split" : "FP32", best_ms);
   m->tuning_log += line;
   return hier_chunks(m, off, best_c);
 }
@@ -316,7 +333,7 @@ int hier_autotune(sx_glm* m, const int64_t* off) {
 // single-chain partials.  The FP32 copy is sign-folded: row n holds
 // (2 y_n - 1) x_n.
 int hier_setup(sx_glm* m, const double* d_X, const double* X, const int32_t* y,
-               const int64_t* off, bool do_tune, int chunk) {
+               const int64_t* off, bool do_tune, int chunk, int kernel) {
   if (m->precision == SX_FP32 && m->N * m->K > 0) {
     // 32 K + 64 zero floats past the last row: the FP32 pass stages whole
     // 32-row tiles (64-float loads per lane) past a chunk's end
@@ -332,8 +349,10 @@ int hier_setup(sx_glm* m, const double* d_X, const double* X, const int32_t* y,
     SX_HIP(hipMemcpy(m->d_X32, x32.data(), total * sizeof(float), hipMemcpyHostToDevice));
   }
   m->d_X = const_cast<double*>(d_X);  // the FP64 pass reads X row-major
+  // FP32 products unless asked for (or timed) otherwise
+  m->hier_x6 = kernel == 2 && m->precision == SX_FP32 && m->K <= 31 ? 1 : 0;
   const int rc = chunk > 0 ? hier_chunks(m, off, chunk)
-                 : do_tune && m->N * m->K > 0 ? hier_autotune(m, off)
+                 : do_tune && m->N * m->K > 0 ? hier_autotune(m, off, kernel)
                                               : hier_chunks(m, off, sx::kHierChunk);
   if (rc) return rc;
   return sx::hier_prepare(m, m->hw, 1);
@@ -397,13 +416,14 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
     set_error("sx_glm_create: invalid sizes or NULL data");
     return SX_EINVAL;
   }
-  sx_tuning tune{0, 0, -1, 0, -1, 0};
+  sx_tuning tune{0, 0, -1, 0, -1, 0, 0};
   if (tuning) tune = *tuning;
   if (tune.lanes_per_row < 0 || tune.lanes_per_row > 64 ||
       (tune.lanes_per_row & (tune.lanes_per_row - 1)) != 0 ||
       tune.slots_cap < 0 || tune.slots_cap > 32 || tune.slots_cap % 4 != 0 ||
       tune.prefetch < -1 || tune.prefetch > 1 || tune.blocks < 0 || tune.hier_chunk < 0 ||
-      tune.hier_chunk > 4096 || tune.hier_chunk % 32 != 0) {
+      tune.hier_chunk > 4096 || tune.hier_chunk % 32 != 0 || tune.hier_kernel < 0 ||
+      tune.hier_kernel > 2) {
     set_error("sx_glm_create: invalid tuning");
     return SX_EINVAL;
   }
@@ -508,7 +528,7 @@ int sx_glm_create_ex(sx_glm** out, int family, const double* X, const void* y, i
       tune.autotune == 1 || (tune.autotune == -1 && tuning == nullptr && N * K >= (1 << 22));
   if (hier) {
     rc = hier_setup(m, staging, X, static_cast<const int32_t*>(y), group_offsets, do_tune,
-                    tune.hier_chunk);
+                    tune.hier_chunk, tune.hier_kernel);
     if (rc == SX_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = SX_EDEVICE;
     if (rc) {
       m->d_X = nullptr;
@@ -930,6 +950,7 @@ int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, dou
   a.chains = C;
   a.propto = propto;
   a.jacobian = jacobian;
+  a.x6 = m->hier_x6;
   a.alpha_sd = m->prior.alpha_sd;
   a.beta_sd = m->prior.beta_sd;
   a.tau_sd = m->prior.tau_sd;

@@ -199,7 +199,7 @@ int start_job(sx_server* srv, job* j) {
   const sx_job_spec& s = j->spec;
   const int prec = gradient_precision(s);
   const int ctx_prec = prec == SX_BF16X2 ? SX_BF16X2 : prec == SX_FP32 ? SX_FP32 : SX_FP64;
-  sx_tuning tune{0, 0, 0, 0, 0, 0};  // fixed launch shapes: results independent of load
+  sx_tuning tune{0, 0, 0, 0, 0, 0, 0};  // fixed launch shapes: results independent of load
   const void* y = s.family == SX_NORMAL ? static_cast<const void*>(j->yd.data())
                                         : static_cast<const void*>(j->yi.data());
   int rc = sx_glm_create_ex(&j->m, s.family, j->X.data(), y, s.N, s.K,

@@ -24,6 +24,7 @@
 
 #include "../include/stanx/sx_math.hpp"
 #include "glm_finalize.hpp"
+#include "mfma_tile.hpp"
 #include "sx_internal.hpp"
 
 namespace sx {
@@ -31,8 +32,6 @@ namespace sx {
 namespace {
 
 constexpr double kHalfLog2PiH = 0.91893853320467274178032973640562;
-
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // FP64: a wave per chunk, rows on lanes; chains on grid.y.  A chunk's rows
 // are contiguous in X (row-major), so each block of up to 64 rows is one
@@ -371,6 +370,214 @@ __global__ __launch_bounds__(256) void hier_mfma_f32(hier_args a) {
   }
 }
 
+// x (FP32) = h + m + l exactly in three BF16 terms, returned as FP32 bit
+// patterns whose low halves are zero: h keeps the leading 8 significant
+// bits (truncated), m the next 8 of the exact residual x - h, l the exact
+// rest (at most 8 bits).  |m| < 2^-7 |h|, |l| < 2^-7 |m|.
+__device__ __forceinline__ void split3(float x, uint32_t& h, uint32_t& m, uint32_t& l) {
+  h = __float_as_uint(x) & 0xffff0000u;
+  const float r1 = x - __uint_as_float(h);
+  m = __float_as_uint(r1) & 0xffff0000u;
+  l = __float_as_uint(r1 - __uint_as_float(m));
+}
+
+// two split terms (high halves) packed as BF16 elements 2i, 2i + 1
+__device__ __forceinline__ uint32_t pack_hi(uint32_t lo_elem, uint32_t hi_elem) {
+  return (hi_elem & 0xffff0000u) | (lo_elem >> 16);
+}
+
+// The same pass with FP32 products formed on the BF16 matrix cores
+// (K <= 31).  Every FP32 operand is split exactly into three BF16 terms,
+// x' = xh + xm + xl (X, staged from the FP32 copy), w' = wh + wm + wl (the
+// chain coefficients), r' = rh + rm + rl (the residuals), and each product
+// keeps the six cross terms down to 2^-14 of the leading one:
+//   forward  Q += Xh Wh + Xh Wm + Xm Wh + Xh Wl + Xl Wh + Xm Wm
+//   adjoint  S += Rh Xh + Rh Xm + Rm Xh + Rh Xl + Rl Xh + Rm Xm
+// (the three dropped terms are below 2^-20 of the product), with FP32
+// accumulation on v_mfma_f32_32x32x16_bf16.  The BF16
+// matrix cores issue beside the VALU link of other waves, which the FP32
+// 32x32x2 instruction does not.  Tile layout: 32 rows x 32 columns (x'
+// in columns < K, the sign s in column K, zeros past it) per term, in the
+// wave's LDS with the 16-byte-chunk XOR permutation of the batched GLM
+// pass; the forward operand (8 consecutive columns of a row) is one
+// 16-byte read per term, the adjoint's (8 rows of the lane's column,
+// matching the rows of the lane's own residuals) two transposing reads.
+// Link, padding rows and outputs as in hier_mfma_f32.
+template <int K>
+__global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
+  static_assert(K >= 0 && K <= 31, "K + 1 <= 32 columns");
+  constexpr int NS = (32 * K + 63) / 64;   // staged floats per lane
+  constexpr int KST = (K + 16) / 16;       // forward k-steps of 16 (columns 0 .. K)
+  constexpr int ROWB = 64;                 // bytes of a tile row (32 BF16)
+  constexpr int TB = 32 * ROWB;            // bytes of one term's tile
+  constexpr int WB = 3 * TB;               // bytes per wave
+  __shared__ __attribute__((aligned(16))) char lds[4][WB];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = lane & 31, h = lane >> 5;
+  const int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (ch >= a.n_chunks) return;  // whole wave; no block barrier follows
+  const int g = a.chunk_group[ch];
+  const int64_t r0 = a.chunk_row0[ch], r1 = a.chunk_row0[ch + 1];
+  const int rows = static_cast<int>(r1 - r0);
+  const int cb = blockIdx.y * 32;
+  char* const base = lds[w];
+  // chain coefficients (w' = log2(e) w, b' = log2(e) b at column K), the
+  // B operand of k-step s: columns 16 s + 8 h .. + 7 of chain cb + i
+  bf16x8 bwh[KST], bwm[KST], bwl[KST];
+  {
+    const int c = cb + li;
+    const double* th = a.theta + static_cast<int64_t>(c < a.chains ? c : 0) * a.theta_stride;
+    const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+#pragma unroll
+    for (int s = 0; s < KST; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = 16 * s + 8 * h + j;
+        double v = 0.0;
+        if (k < K)
+          v = th[1 + k] + exp(th[2 + K + k]) * z[1 + k];
+        else if (k == K)
+          v = th[0] + exp(th[1 + K]) * z[0];
+        uint32_t hh, mm, ll;
+        split3(static_cast<float>(v * 1.4426950408889634074), hh, mm, ll);
+        bwh[s][j] = __builtin_bit_cast(__bf16, static_cast<uint16_t>(hh >> 16));
+        bwm[s][j] = __builtin_bit_cast(__bf16, static_cast<uint16_t>(mm >> 16));
+        bwl[s][j] = __builtin_bit_cast(__bf16, static_cast<uint16_t>(ll >> 16));
+      }
+  }
+  f32x16 sacc;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) sacc[j] = 0.f;
+  double s0 = 0.0;
+  const float lpad = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-0.0f));
+  // all three terms zero: columns past K and rows past the chunk stay zero
+  for (int e = lane; e < WB / 16; e += 64)
+    *reinterpret_cast<uint4*>(base + 16 * e) = make_uint4(0u, 0u, 0u, 0u);
+  // LDS byte offset of each staged float (row e / K, column e % K of the
+  // tile, its 16-byte chunk permuted); elements e >= 32 K (only in the
+  // last staging register, for odd K) are not written
+  constexpr int Kd = K > 0 ? K : 1;
+  int dst[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int e = lane + 64 * j;
+    const int r = e / Kd, cc = e % Kd;
+    dst[j] = r * ROWB + 16 * swz<4>(r, cc >> 3) + 2 * (cc & 7);
+  }
+  // the sign column's position in row `lane` (lanes < 32)
+  const int sgn = li * ROWB + 16 * swz<4>(li, K >> 3) + 2 * (K & 7);
+  float stg[NS > 0 ? NS : 1];
+  int ybyte = 0, scnt = 0;
+#define SX_LOAD_TILE(T0)                                                          \
+  {                                                                               \
+    scnt = rows - (T0) < 32 ? rows - (T0) : 32;                                   \
+    const float* src_ = a.X32 + (r0 + (T0)) * K + lane;                           \
+    _Pragma("unroll") for (int j = 0; j < NS; ++j) stg[j] = src_[64 * j];         \
+    ybyte = a.y[r0 + (T0) + (lane < scnt ? lane : scnt - 1)];                     \
+  }
+  SX_LOAD_TILE(0);
+  int ln = lane;
+  asm volatile("" : "+v"(ln));
+  const tile_addr<ROWB, 4> ad(ln);
+  for (int t0 = 0; t0 < rows; t0 += 32) {
+    const int cnt = scnt;
+    // LDS accesses of one wave complete in order: the previous tile's reads
+    // are done before these writes land
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      if (64 * (j + 1) <= 32 * K || lane + 64 * j < 32 * K) {
+        const float v = lane + 64 * j < cnt * K ? stg[j] : 0.f;
+        uint32_t hh, mm, ll;
+        split3(v, hh, mm, ll);
+        *reinterpret_cast<uint16_t*>(base + dst[j]) = static_cast<uint16_t>(hh >> 16);
+        *reinterpret_cast<uint16_t*>(base + TB + dst[j]) = static_cast<uint16_t>(mm >> 16);
+        *reinterpret_cast<uint16_t*>(base + 2 * TB + dst[j]) = static_cast<uint16_t>(ll >> 16);
+      }
+    }
+    if (lane < 32)
+      *reinterpret_cast<__bf16*>(base + sgn) = to_bf16(lane < cnt ? (ybyte ? 1.0f : -1.0f) : 0.f);
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (t0 + 32 < rows) SX_LOAD_TILE(t0 + 32);  // in flight during this tile
+    f32x16 hacc;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) hacc[j] = 0.f;
+#pragma unroll
+    for (int s = 0; s < KST; ++s) {
+      const bf16x8 xh = *reinterpret_cast<const bf16x8*>(base + ad.fwd(s));
+      const bf16x8 xm = *reinterpret_cast<const bf16x8*>(base + TB + ad.fwd(s));
+      const bf16x8 xl = *reinterpret_cast<const bf16x8*>(base + 2 * TB + ad.fwd(s));
+      hacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bwh[s], hacc, 0, 0, 0);
+      hacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bwm[s], hacc, 0, 0, 0);
+      hacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xm, bwh[s], hacc, 0, 0, 0);
+      hacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, bwl[s], hacc, 0, 0, 0);
+      hacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, bwh[s], hacc, 0, 0, 0);
+      hacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xm, bwm[s], hacc, 0, 0, 0);
+    }
+    int npad = 0;
+    if (cnt < 32) {
+#pragma unroll
+      for (int b4 = 0; b4 < 4; ++b4) {
+        const int p = 8 * b4 + 4 * h + 4 - cnt;
+        npad += p < 0 ? 0 : (p > 4 ? 4 : p);
+      }
+    }
+    float al = 0.f, am = 0.f;
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      uint32_t ph[8], pm[8], pl[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const float q = hacc[8 * st + jj];
+        const float L = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-fabsf(q)));
+        al += L;
+        am += fabsf(q) - q;
+        split3(__builtin_amdgcn_exp2f(-fmaf(0.5f, q + fabsf(q), L)), ph[jj], pm[jj], pl[jj]);
+      }
+      uint32_t uh[4], um[4], ul[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        uh[i] = pack_hi(ph[2 * i], ph[2 * i + 1]);
+        um[i] = pack_hi(pm[2 * i], pm[2 * i + 1]);
+        ul[i] = pack_hi(pl[2 * i], pl[2 * i + 1]);
+      }
+      const bf16x8 rh = __builtin_bit_cast(bf16x8, uh);
+      const bf16x8 rm = __builtin_bit_cast(bf16x8, um);
+      const bf16x8 rl = __builtin_bit_cast(bf16x8, ul);
+      const bf16x8 th = tr_operand(base, ad.tr(st, 0, 0), ad.tr(st, 1, 0));
+      const bf16x8 tm = tr_operand(base + TB, ad.tr(st, 0, 0), ad.tr(st, 1, 0));
+      const bf16x8 tl = tr_operand(base + 2 * TB, ad.tr(st, 0, 0), ad.tr(st, 1, 0));
+      sacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, th, sacc, 0, 0, 0);
+      sacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, tm, sacc, 0, 0, 0);
+      sacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rm, th, sacc, 0, 0, 0);
+      sacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, tl, sacc, 0, 0, 0);
+      sacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, th, sacc, 0, 0, 0);
+      sacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rm, tm, sacc, 0, 0, 0);
+    }
+    s0 += static_cast<double>(fmaf(0.5f, am, al)) - static_cast<double>(npad) * lpad;
+  }
+#undef SX_LOAD_TILE
+  // sacc[j] = column i of chain cb + 8 (j/4) + 4 h + j%4: sum r x_i for
+  // i < K (slot 1 + i), sum r for i = K (slot 0)
+  if (li <= K) {
+    const int slot = li < K ? 1 + li : 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int c = cb + 8 * (j >> 2) + 4 * h + (j & 3);
+      if (c < a.chains) a.cpart[(ch * a.chains + c) * a.E + slot] = sacc[j];
+    }
+  }
+  const double s0t = s0 + __shfl_xor(s0, 32, 64);
+  const int c = cb + li;
+  if (h == 0 && c < a.chains) {
+    double* out = a.cpart + (ch * a.chains + c) * a.E;
+    out[K + 1] = -0.69314718055994530942 * s0t;
+    out[K + 2] = isfinite(s0t) ? 0.0 : 1.0;
+  }
+}
+
 // Per group and chain: sum the group's chunks in order; z gradients;
 // per-group contributions [S_0..S_K, z.S terms (K+1), lp, bad, sum z^2],
 // summed over the block's groups in group order: the block's row of
@@ -580,6 +787,21 @@ __global__ __launch_bounds__(1024) void hier_final(hier_args a) {
   out[a.D + 1] = bad > 0 ? 1.0 : 0.0;
 }
 
+hipError_t launch_hier_x6(const hier_args& a, dim3 grid, hipStream_t s) {
+  switch (a.K) {
+#define SX_CASE(KK) \
+  case KK: hipLaunchKernelGGL((hier_mfma_x6<KK>), grid, dim3(256), 0, s, a); break;
+    SX_CASE(0) SX_CASE(1) SX_CASE(2) SX_CASE(3) SX_CASE(4) SX_CASE(5) SX_CASE(6) SX_CASE(7)
+    SX_CASE(8) SX_CASE(9) SX_CASE(10) SX_CASE(11) SX_CASE(12) SX_CASE(13) SX_CASE(14)
+    SX_CASE(15) SX_CASE(16) SX_CASE(17) SX_CASE(18) SX_CASE(19) SX_CASE(20) SX_CASE(21)
+    SX_CASE(22) SX_CASE(23) SX_CASE(24) SX_CASE(25) SX_CASE(26) SX_CASE(27) SX_CASE(28)
+    SX_CASE(29) SX_CASE(30) SX_CASE(31)
+#undef SX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int NH>
 hipError_t launch_hier_mfma(const hier_args& a, dim3 grid, hipStream_t s) {
   switch (a.K) {
@@ -617,7 +839,8 @@ hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s) {
       }
     } else {
       const dim3 grid(static_cast<unsigned>((a.n_chunks + 3) / 4), static_cast<unsigned>((a.chains + 31) / 32));
-      const hipError_t e = launch_hier_mfma<1>(a, grid, s);
+      const hipError_t e =
+          a.x6 && a.K <= 31 ? launch_hier_x6(a, grid, s) : launch_hier_mfma<1>(a, grid, s);
       if (e != hipSuccess) return e;
     }
     const hipError_t e = hipGetLastError();

@@ -108,6 +108,7 @@ struct hier_args {
   const int64_t* group_chunk0;  // J + 1
   int64_t n_chunks, theta_stride, out_stride;
   int J, K, D, E, G, chains, propto, jacobian;
+  int x6;  // FP32 batches: products on BF16 MFMA in three-term splits (K <= 31)
   double alpha_sd, beta_sd, tau_sd, alpha_const, beta_const, tau_const;
 };
 hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s);
@@ -183,6 +184,7 @@ struct sx_glm {
   int64_t *d_chunk_row0 = nullptr, *d_group_chunk0 = nullptr;
   int64_t n_chunks = 0;
   int hier_chunk = 0;  // rows per chunk at most
+  int hier_x6 = 0;     // FP32 batches: products in BF16 three-term splits
   sx::hier_workspace hw;
   std::string tuning_log;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -173,6 +173,46 @@ def test_hier_autotuned_contexts_are_bitwise_equal():
             m.close()
 
 
+@pytest.mark.parametrize("J,K,size,C", [(5, 3, 60, 1), (40, 4, 300, 7), (50, 20, 400, 64),
+                                        (12, 31, 500, 130), (3, 0, 40, 65), (9, 15, 700, 33),
+                                        (8, 16, 300, 32)])
+def test_hier_fp32_split_products_against_oracle(J, K, size, C):
+    """FP32 batches with the products formed on BF16 MFMA from exact
+    three-term splits: the same bars as the FP32 MFMA products."""
+    d = _data(J, K, size, seed=J + K + C + 1)
+    m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                   precision=S.FP32, tuning=dict(hier_kernel=2))
+    ref = O.GlmData(O.HIER, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"])
+    th = W.random_thetas(m.D, C, seed=C + K + 1) * 0.5
+    lp, g = m.log_prob_grad_batch(th)
+    worst_lp = worst_g = 0.0
+    for c in range(C):
+        st, lp_r, g_r = ref.lpgrad(th[c], mode=1)
+        assert st == 0
+        worst_lp = _nanmax(worst_lp, abs(lp[c] - lp_r) / abs(lp_r))
+        worst_g = _nanmax(worst_g, np.linalg.norm(g[c] - g_r) / np.linalg.norm(g_r))
+    _say(f"hier split products J={J} K={K} C={C}: max rel lp err {worst_lp:.2e}, "
+         f"max normwise grad err {worst_g:.2e}")
+    assert worst_lp < LP_TOL_FP32 and worst_g < GRAD_TOL_FP32
+    # a chain's result does not depend on the batch
+    sel = np.arange(C)[::3]
+    lp_s, g_s = m.log_prob_grad_batch(th[sel])
+    np.testing.assert_array_equal(lp_s, lp[sel])
+    np.testing.assert_array_equal(g_s, g[sel])
+    m.close()
+
+
+def test_hier_split_products_domain_error_per_chain():
+    d = _data(6, 3, 50, seed=9)
+    m = S.GlmModel(S.HIER_LOGIT, d["X"], d["y"], offsets=d["offsets"], prior=d["prior"],
+                   precision=S.FP32, tuning=dict(hier_kernel=2))
+    th = W.random_thetas(m.D, 4, seed=2) * 0.5
+    th[2, 1 + 3] = 800.0  # tau_0 = exp(800) overflows: eta is not finite
+    lp, g = m.log_prob_grad_batch(th)
+    assert np.isneginf(lp[2]) and np.all(np.isfinite(lp[[0, 1, 3]]))
+    m.close()
+
+
 def test_hier_fp32_chain_independent_of_batch():
     d = _data(25, 5, 150, seed=5)
     m, _ = _models(d, precision=S.FP32)
