@@ -821,8 +821,10 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
 }
 
 // Launch shape of the batched MFMA pass, timed on the device at context
-// creation: the kernel variant (one wave per 32-chain block, or a pair of
-// waves split by k where Kp / 32 is even and at most 8) and the row split
+// creation: the kernel variant (one wave per 32-chain block; a pair of
+// waves split by k where Kp / 32 is even and at most 8; or, for Kp <= 128,
+// one wave per block without operand prefetch at two waves per SIMD, the
+// same products as the first) and the row split
 // (target tiles of 32 rows per split: 4, 8, 16 or 32), each timed as full
 // evaluations of 128 chains at theta = 0 (the kernels' work does not depend
 // on the values).  The split sets the order of the FP64 sums, so, as for
@@ -860,8 +862,9 @@ int mx_autotune(sx_glm* m) {
   const int kq = static_cast<int>(m->Kp / 32);
   int best_v = 0, best_t = 8;
   double best_ms = 1e30;
-  for (int v = 0; v <= 1 && rc == SX_OK; ++v) {
-    if (v == 1 && (m->Kp > 256 || kq % 2 != 0)) break;
+  for (int v = 0; v <= 2 && rc == SX_OK; ++v) {
+    if (v == 1 && (m->Kp > 256 || kq % 2 != 0)) continue;
+    if (v == 2 && kq > 4) continue;
     for (int tps : {4, 8, 16, 32}) {
       m->mx_variant = v;
       m->mx_tps = tps;

@@ -163,7 +163,7 @@ __global__ void prep_chains(const double* __restrict__ theta, int64_t D, int C, 
 // v_mfma_f32_32x32x16_bf16 operands (lane l, j < 8): A[l % 32][8 (l / 32)
 // + j], B[8 (l / 32) + j][l % 32]; result d[j] = D[8 (j / 4) + 4 (l / 32) +
 // j % 4][l % 32].
-template <int KQ, int FAM, int NW>
+template <int KQ, int FAM, int NW, bool PF>
 __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const float* Y,
                                         int64_t t, int lane, const bf16x8 (&bh)[2 * KQ],
                                         const bf16x8 (&bl)[2 * KQ], const bf16x8 (&bm)[2 * KQ],
@@ -180,8 +180,10 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
   const tile_addr<2 * Kp, P> ad(ln);
   const int h = ln >> 5;
   const char* xb = reinterpret_cast<const char*>(X);
-  // forward: the operands of step s + 1 are read while step s's products
-  // run (one step of prefetch)
+  // forward: with PF the operands of step s + 1 are read while step s's
+  // products run (one step of prefetch); without, each step reads its own
+  // (fewer registers: the kernel then runs two waves per SIMD, which hide
+  // each other's LDS latency).  The products are the same either way.
   f32x16 H;
 #pragma unroll
   for (int j = 0; j < 16; ++j) H[j] = 0.f;
@@ -190,7 +192,11 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
 #pragma unroll
   for (int s = 0; s < S16; ++s) {
     bf16x8 nh = fh, nl = fl;
-    if (s + 1 < S16) {
+    if (!PF && s > 0) {
+      fh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(s));
+      fl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(s));
+    }
+    if (PF && s + 1 < S16) {
       nh = *reinterpret_cast<const bf16x8*>(xb + ad.fwd(s + 1));
       nl = *reinterpret_cast<const bf16x8*>(xb + TB + ad.fwd(s + 1));
     }
@@ -198,12 +204,17 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
     H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bl[s], H, 0, 0, 0);
     H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, bh[s], H, 0, 0, 0);
     H = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, bm[s], H, 0, 0, 0);
-    fh = nh;
-    fl = nl;
+    if (PF) {
+      fh = nh;
+      fl = nl;
+    }
   }
-  // adjoint operands of (st, kb) = (0, 0), read while the link runs
-  bf16x8 ah = tr_operand(xb, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
-  bf16x8 al = tr_operand(xb + TB, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
+  // adjoint operands of (st, kb) = (0, 0), read while the link runs (PF)
+  bf16x8 ah, al;
+  if (PF) {
+    ah = tr_operand(xb, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
+    al = tr_operand(xb + TB, ad.tr(0, 0, 0), ad.tr(0, 1, 0));
+  }
   const int64_t n0 = t * 32;
   float l2 = 0.f, m2 = 0.f, srt = 0.f;
 #pragma unroll
@@ -221,29 +232,35 @@ __device__ __forceinline__ void mx_tile(const mx_args& a, const __bf16* X, const
       rh[jj] = hh;
       rl[jj] = ll;
     }
-    // one (st, kb) of prefetch: the operands of the next column block (or
-    // of the next half's first) are read while this block's products run
+    // (PF) one (st, kb) of prefetch: the operands of the next column block
+    // (or of the next half's first) are read while this block's products run
 #pragma unroll
     for (int kb = 0; kb < KQ; ++kb) {
+      if (!PF) {
+        ah = tr_operand(xb, ad.tr(st, 0, kb), ad.tr(st, 1, kb));
+        al = tr_operand(xb + TB, ad.tr(st, 0, kb), ad.tr(st, 1, kb));
+      }
       bf16x8 nh = ah, nl = al;
       const int nst = kb + 1 < KQ ? st : st + 1, nkb = kb + 1 < KQ ? kb + 1 : 0;
-      if (nst < 2) {
+      if (PF && nst < 2) {
         nh = tr_operand(xb, ad.tr(nst, 0, nkb), ad.tr(nst, 1, nkb));
         nl = tr_operand(xb + TB, ad.tr(nst, 0, nkb), ad.tr(nst, 1, nkb));
       }
       g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, ah, g[kb], 0, 0, 0);
       g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, al, g[kb], 0, 0, 0);
       g[kb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, ah, g[kb], 0, 0, 0);
-      ah = nh;
-      al = nl;
+      if (PF) {
+        ah = nh;
+        al = nl;
+      }
     }
   }
   lp += tile_lp<FAM>(l2, m2);
   sr += static_cast<double>(srt);
 }
 
-template <int KQ, int FAM, int NW>
-__global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
+template <int KQ, int FAM, int NW, bool PF>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(PF ? 1 : 2))) void glm_mfma32(mx_args a) {
   static_assert(KQ >= 1 && KQ <= 8, "glm_mfma32 handles 32 <= Kp <= 256");
   static_assert(NW == 2 || NW == 4, "2 or 4 waves per workgroup");
   constexpr int Kp = 32 * KQ;
@@ -321,12 +338,12 @@ __global__ __launch_bounds__(64 * NW) void glm_mfma32(mx_args a) {
   for (int64_t t = t0; t < t1; t += 2) {
     // xb held tile t - 1, read before the last barrier
     if (t + 1 < t1) SX_STAGE(t + 1, xb, yb);
-    mx_tile<KQ, FAM, NW>(a, xa, ya, t, lane, bh, bl, bm, alpha, g, lp, sr, nf);
+    mx_tile<KQ, FAM, NW, PF>(a, xa, ya, t, lane, bh, bl, bm, alpha, g, lp, sr, nf);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (t + 1 >= t1) break;
     if (t + 2 < t1) SX_STAGE(t + 2, xa, ya);
-    mx_tile<KQ, FAM, NW>(a, xb, yb, t + 1, lane, bh, bl, bm, alpha, g, lp, sr, nf);
+    mx_tile<KQ, FAM, NW, PF>(a, xb, yb, t + 1, lane, bh, bl, bm, alpha, g, lp, sr, nf);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
@@ -881,14 +898,14 @@ __global__ __launch_bounds__(1024) void mx_reduce(mx_args a, pass_args pa) {
   if (tid < 64) finalize<FAM>(pa, st);
 }
 
-template <int KQ, int NW>
+template <int KQ, int NW, bool PF = true>
 hipError_t launch_kq32_nw(int family, const mx_args& a, hipStream_t s) {
   if (a.splits % 8 != 0 || a.Cp % (32 * NW) != 0) return hipErrorInvalidValue;
   dim3 grid(a.splits * (a.Cp / (32 * NW)));
   if (family == SX_LOGIT)
-    hipLaunchKernelGGL((glm_mfma32<KQ, SX_LOGIT, NW>), grid, dim3(64 * NW), 0, s, a);
+    hipLaunchKernelGGL((glm_mfma32<KQ, SX_LOGIT, NW, PF>), grid, dim3(64 * NW), 0, s, a);
   else
-    hipLaunchKernelGGL((glm_mfma32<KQ, SX_NORMAL, NW>), grid, dim3(64 * NW), 0, s, a);
+    hipLaunchKernelGGL((glm_mfma32<KQ, SX_NORMAL, NW, PF>), grid, dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
@@ -905,7 +922,9 @@ hipError_t launch_pair_ncb(int family, const mx_args& a, hipStream_t s) {
 
 // Kp <= 256.  Variant 0: one wave per 32-chain block (two waves per
 // workgroup for a batch of at most 64 chains (Cp = 64), four otherwise);
-// variant 1 (KQ even): a pair of waves per chain block, split by k.
+// variant 1 (KQ even): a pair of waves per chain block, split by k;
+// variant 2 (KQ <= 4): variant 0 without operand prefetch, two waves per
+// SIMD (for Cp = 64 it is variant 0: the same products, bit for bit).
 template <int KQ>
 hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
   if constexpr (KQ % 2 == 0) {
@@ -914,7 +933,10 @@ hipError_t launch_kq32(int family, const mx_args& a, hipStream_t s) {
       return launch_pair_ncb<KQ, 4>(family, a, s);
     }
   }
-  if (a.variant != 0) return hipErrorInvalidValue;
+  if constexpr (KQ <= 4) {
+    if (a.variant == 2 && a.Cp != 64) return launch_kq32_nw<KQ, 4, false>(family, a, s);
+  }
+  if (a.variant != 0 && !(a.variant == 2 && KQ <= 4)) return hipErrorInvalidValue;
   if (a.Cp == 64) return launch_kq32_nw<KQ, 2>(family, a, s);
   return launch_kq32_nw<KQ, 4>(family, a, s);
 }
