@@ -825,7 +825,9 @@ int mx_enqueue(sx_glm* m, mx_workspace& w, const double* d_theta, int C, double*
 // waves split by k where Kp / 32 is even and at most 8; or, for Kp <= 128,
 // one wave per block without operand prefetch at two waves per SIMD, the
 // same products as the first) and the row split
-// (target tiles of 32 rows per split: 4, 8, 16 or 32), each timed as full
+// (target tiles of 32 rows per split: 4 to 128; at most 512 splits, so
+// large N reach fewer splits, and fewer gradient partials to sum, only with
+// the larger targets), each timed as full
 // evaluations of 128 chains at theta = 0 (the kernels' work does not depend
 // on the values).  The split sets the order of the FP64 sums, so, as for
 // the FP64 pass, the first decision per (device, family, N, K) is kept for
@@ -865,7 +867,7 @@ int mx_autotune(sx_glm* m) {
   for (int v = 0; v <= 2 && rc == SX_OK; ++v) {
     if (v == 1 && (m->Kp > 256 || kq % 2 != 0)) continue;
     if (v == 2 && kq > 4) continue;
-    for (int tps : {4, 8, 16, 32}) {
+    for (int tps : {4, 8, 16, 32, 64, 128}) {
       m->mx_variant = v;
       m->mx_tps = tps;
       for (int i = 0; i < 2 && rc == SX_OK; ++i)
