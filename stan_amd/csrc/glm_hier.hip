@@ -463,10 +463,10 @@ __global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
   for (int j = 0; j < NS; ++j) {
     const int e = lane + 64 * j;
     const int r = e / Kd, cc = e % Kd;
-    dst[j] = r * ROWB + 16 * swz<4>(r, cc >> 3) + 2 * (cc & 7);
+    dst[j] = elem_offset<ROWB, 4>(r, cc);
   }
   // the sign column's position in row `lane` (lanes < 32)
-  const int sgn = li * ROWB + 16 * swz<4>(li, K >> 3) + 2 * (K & 7);
+  const int sgn = elem_offset<ROWB, 4>(li, K);
   float stg[NS > 0 ? NS : 1];
   int ybyte = 0, scnt = 0;
 #define SX_LOAD_TILE(T0)                                                          \

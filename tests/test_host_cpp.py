@@ -27,3 +27,15 @@ def test_lean_link_math_accuracy(sx_math_check):
     assert vals["rcp_ulps"] <= 1.0, out
     assert vals["term_rel"] <= 1e-15, out
     assert vals["r_rel"] <= 1e-15, out
+
+
+def test_staged_tile_layout_matches_mfma_operands(tmp_path):
+    """The LDS tile layout of the BF16 MFMA passes (staging, element
+    writes, forward reads and transposing adjoint reads of
+    stan_amd/csrc/tile_layout.hpp), compiled for the host, delivers every
+    operand element the MFMA expects, for every padded K up to 256."""
+    exe = str(tmp_path / "tile_layout_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe,
+                    os.path.join(HERE, "cpp", "tile_layout_check.cpp")], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
