@@ -51,9 +51,11 @@ enum : int {
   RHO, RHOF, RHOB, TMP,
   IQ, IP, IG,              // z_init of the step-size heuristic
   MINV, WM, WM2,           // metric and Welford accumulators
+  NP,                      // momentum of the next leapfrog's begin half (leaf pass)
   LVL0                     // per level l >= 1: PIE, RI, ZPF, PFBG, RF
 };
 static_assert(MINV == kSlotMinv, "metric slot");
+static_assert(LVL0 + 5 * kMaxDepth == kNumSlots, "slot count");
 __device__ __forceinline__ int PIE(int l) { return LVL0 + 5 * (l - 1); }
 __device__ __forceinline__ int RI(int l) { return LVL0 + 5 * (l - 1) + 1; }
 __device__ __forceinline__ int ZPF(int l) { return LVL0 + 5 * (l - 1) + 2; }
@@ -472,15 +474,33 @@ __device__ dest dest_of(const ctx& k, int level) {
 // RF(ls) = 0 (no merge below ls reads or writes either slot, and they are
 // unused if the subtree ends there).  The kinetic energy, which the final
 // half's start also needs, is returned in T.
+//
+// With ls > 0 (spec) the tree, if it goes on, goes on from this point in
+// the same direction (the final half of level ls starts here), so the same
+// pass also takes the next leapfrog's begin half: p' = p - e/2 g into NP and
+// q' = q + e minv p' into theta, the gradient request position; the final
+// half's start then requests that gradient without another pass, and the
+// next leaf reads its point as q = theta, p = NP (znext).  The arithmetic is
+// that of the separate pass.  A spec leaf's own p and q (kept in the
+// subtree-end and proposal slots) are not stored at ZP, ZQ: the point is
+// the current one again only if the tree stops, and then its trajectory
+// ends are not read again (the transition ends).  A leaf without spec
+// stores the whole point (q, p, g) at ZQ, ZP, ZG.
 __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls, double& T) {
   chain_state* s = k.s;
   double acc[1] = {0.0};
   {
     const double* o = k.a->out + static_cast<int64_t>(k.c) * k.a->out_stride;
+    const bool zn = s->znext != 0;
+    const bool spec = ls > 0;
+    double* th = k.a->theta + static_cast<int64_t>(k.c) * k.a->D;
     double* g = k.slot(ZG);
     double* p = k.slot(ZP);
+    double* zq = k.slot(ZQ);
+    double* np = k.slot(NP);
+    const double* pin = zn ? np : p;
+    const double* qin = zn ? th : zq;
     const double* m = k.slot(MINV);
-    const double* q = k.slot(ZQ);
     double* prop = k.slot(prop_slot(d.zprop));
     double* r = k.slot(d.rho);
     double* pb = k.slot(d.pbeg);
@@ -489,22 +509,32 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
     double* rf = k.slot(ls > 0 ? RF(ls) : TMP);
     const double he = 0.5 * lf_eps;
     struct lf { double g, p, m, q, r; };
-    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{ok ? o[1 + i] : g[i], p[i], m[i], q[i], r[i]}; },
+    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{ok ? o[1 + i] : g[i], pin[i], m[i], qin[i], r[i]}; },
                      [&](int i, const lf& v, double (&ac)[1]) {
                        const double gn = -v.g;
                        const double pn = v.p - he * gn;
                        g[i] = gn;
-                       p[i] = pn;
                        ac[0] += pn * (v.m * pn);
                        prop[i] = v.q;
                        r[i] = v.r + pn;
                        pb[i] = pn;
                        pe[i] = pn;
-                       if (ls > 0) {
+                       if (spec) {
                          zpf[i] = v.q;
                          rf[i] = 0.0;
+                         const double p2 = pn - he * gn;
+                         np[i] = p2;
+                         th[i] = v.q + lf_eps * (v.m * p2);
+                       } else {
+                         p[i] = pn;
+                         if (zn) zq[i] = v.q;
                        }
                      });
+    __syncthreads();
+    if (k.lane == 0) {
+      s->znext = 0;
+      s->spec = spec ? 1 : 0;
+    }
   }
   block_sum_n<1>(k, acc);
   T = 0.5 * acc[0];
@@ -527,8 +557,11 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
   return !s->divergent;
 }
 
-// merge of the two halves of level l into d (base_nuts.hpp:321-351)
-__device__ bool merge(const ctx& k, int l, const dest& d) {
+// merge of the two halves of level l into d (base_nuts.hpp:321-351).
+// With clear > 0 the final half of level l + 1 starts next (its start was
+// taken by the leaf's pass): the same pass resets rho_init of levels
+// 1 .. clear (= l; this merge is the last to read them) for its subtree.
+__device__ bool merge(const ctx& k, int l, const dest& d, int clear) {
   chain_state* s = k.s;
   const double lsw_sub = log_sum_exp(s->lsw_init[l], s->lsw_final[l]);
   double& lw = lsw_ref(k, d.lsw);
@@ -564,8 +597,9 @@ __device__ bool merge(const ctx& k, int l, const dest& d) {
     const double* pie = k.slot(PIE(l));
     const double* psrc = k.slot(prop_slot(l));
     double* pdst = k.slot(prop_slot(d.zprop));
-    double* t = k.slot(TMP);
     double* r = k.slot(d.rho);
+    double* ri1 = k.slot(RI(1));  // RI(j) = RI(1) + 5 (j - 1) slots
+    const int64_t rstep = 5 * static_cast<int64_t>(k.a->Dpad);
     struct mg { double ri, rf, r, m, pend, pbeg, pfbg, pie, prop; };
     vpass_acc<6, mg>(
         k, acc,
@@ -576,8 +610,8 @@ __device__ bool merge(const ctx& k, int l, const dest& d) {
         [&](int i, const mg& v, double (&ac)[6]) {
           if (take) pdst[i] = v.prop;
           const double tv = v.ri + v.rf;
-          t[i] = tv;
           r[i] = v.r + tv;
+          for (int j = 0; j < clear; ++j) ri1[i + j * rstep] = 0.0;
           ac[0] += (v.m * v.pend) * tv;
           ac[1] += (v.m * v.pbeg) * tv;
           ac[2] += (v.m * v.pfbg) * (v.ri + v.pfbg);
@@ -787,9 +821,15 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   // point again, unread).  The bound keeps the launch from waiting on the
   // one chain whose leaf closes a deep subtree or a transition while the
   // others take one leapfrog each; the arithmetic and the random stream
-  // are those of an unbounded run.
-  for (int turn = 0; turn < a.phase_budget && !s.pending && s.phase != PH_DONE && !s.xfail; ++turn) {
+  // are those of an unbounded run.  A final half's start whose leapfrog
+  // the leaf's pass already took (no vector pass) is taken whatever the
+  // count and does not add to it.
+  for (int turn = 0; !s.pending && s.phase != PH_DONE && !s.xfail;) {
     const int ph = s.phase;
+    const bool free_turn = ph == PH_MERGE && s.mvalid && s.mlevel <= s.tree_depth &&
+                           s.stage[s.mlevel] == 0 && s.spec;
+    if (!free_turn && turn >= a.phase_budget) break;
+    turn += free_turn ? 0 : 1;
     __syncthreads();
     switch (ph) {
       case PH_INIT_TRY: {
@@ -1079,10 +1119,14 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
               s.lsw_final[l] = -kInf;
             }
             __syncthreads();
-            // one pass: every lower level's rho_init = 0 (its frame reset) and
-            // the leapfrog's begin half with its gradient request
+            // the leaf's pass took the begin half of the next leapfrog (NQ,
+            // NP, theta) and the last merge below l reset the lower levels'
+            // rho_init: request the gradient there.  Otherwise one pass:
+            // every lower level's rho_init = 0 (its frame reset) and the
+            // leapfrog's begin half with its gradient request
             // (leapfrog_request)
-            {
+            const bool spec = s.spec != 0;
+            if (!spec) {
               const double eps = (s.fwd ? 1.0 : -1.0) * s.eps;
               double* p = k.slot(ZP);
               double* q = k.slot(ZQ);
@@ -1109,13 +1153,17 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
                 s.stage[mm] = 0;
                 s.lsw_init[mm] = -kInf;
               }
+              s.znext = spec ? 1 : 0;
+              s.spec = 0;
               s.phase = PH_LEAF;
               s.pending = 1;
               s.n_grad++;
             }
             break;
           }
-          const bool v = merge(k, l, dest_of(k, l));
+          // the last merge before the final half of level l + 1 starts
+          const int clear = l + 1 <= s.tree_depth && s.stage[l + 1] == 0 ? l : 0;
+          const bool v = merge(k, l, dest_of(k, l), clear);
           __syncthreads();
           if (lane == 0) {
             s.mvalid = v ? 1 : 0;
@@ -1125,20 +1173,9 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         }
         // the subtree of depth tree_depth is complete (or invalid)
         if (!valid) {
-          // the trajectory's end on this side = z (one pass)
-          {
-            const double* zq = k.slot(ZQ);
-            const double* zp = k.slot(ZP);
-            const double* zg = k.slot(ZG);
-            double* eq = k.slot(s.fwd ? FQ : BQ);
-            struct z3 { double q, p, g; };
-            vpass<z3>(k, [&](int i) { return z3{zq[i], zp[i], zg[i]}; },
-                      [&](int i, const z3& v) {
-                        eq[i] = v.q;
-                        eq[i + a.Dpad] = v.p;
-                        eq[i + 2 * a.Dpad] = v.g;
-                      });
-          }
+          // the transition ends here: the reference's last z_fwd / z_bck
+          // assignment is not read again, so the trajectory end's vectors
+          // are left as they are (the next transition sets both ends)
           __syncthreads();
           if (lane == 0) {
             if (s.fwd)

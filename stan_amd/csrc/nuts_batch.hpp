@@ -12,7 +12,7 @@ namespace sx {
 namespace nb {
 
 constexpr int kMaxDepth = 12;  // frames per chain (max_depth <= kMaxDepth)
-constexpr int kNumSlots = 23 + 5 * kMaxDepth;
+constexpr int kNumSlots = 24 + 5 * kMaxDepth;
 constexpr int kSlotMinv = 20;  // vector slot of the diagonal inverse metric
 
 enum phase : int {
@@ -36,6 +36,8 @@ struct chain_state {
   int depth, tree_depth, n_leapfrog, divergent, fwd;
   int ss_ret, ss_iter, ss_dir, last_ok;
   int mlevel, mvalid;  // PH_MERGE: next level of the ladder above a leaf, still valid
+  int spec;   // the last leaf's pass also took the next leapfrog's begin half
+  int znext;  // the current point is that begin half (q at theta, p at NP)
   int stage[kMaxDepth + 1];
   unsigned w_counter;  // warm-up transitions seen by the metric windows
   int w_cursor;        // next close of the window plan not yet passed
