@@ -48,7 +48,7 @@ enum : int {
   BQ, BP, BG,              // backward end
   SQ, PQ,                  // sample, top-level proposal (positions)
   PFB, PBF,                // p at the backward end of the forward subtree / forward end of the backward subtree
-  RHO, RHOF, RHOB, TMP,
+  RHO, RHOF, RHOB,
   IQ, IP, IG,              // z_init of the step-size heuristic
   MINV, WM, WM2,           // metric and Welford accumulators
   NP,                      // momentum of the next leapfrog's begin half (leaf pass)
@@ -468,24 +468,31 @@ __device__ dest dest_of(const ctx& k, int level) {
 
 // base case of build_tree, fused with the leapfrog's second half.  One
 // pass: g = -grad (or the flipped stale g after a domain error), p -= e/2 g,
-// kinetic energy, proposal position, rho += p, subtree ends, and, for the
-// first level ls whose initial half this leaf completes if the merges
-// below it hold (ls = 0: none), that level's final-half start: ZPF(ls) = q,
-// RF(ls) = 0 (no merge below ls reads or writes either slot, and they are
-// unused if the subtree ends there).  The kinetic energy, which the final
-// half's start also needs, is returned in T.
+// kinetic energy, proposal position, rho, subtree ends.  The kinetic energy
+// is returned in T.
 //
-// With ls > 0 (spec) the tree, if it goes on, goes on from this point in
-// the same direction (the final half of level ls starts here), so the same
-// pass also takes the next leapfrog's begin half: p' = p - e/2 g into NP and
-// q' = q + e minv p' into theta, the gradient request position; the final
-// half's start then requests that gradient without another pass, and the
-// next leaf reads its point as q = theta, p = NP (znext).  The arithmetic is
-// that of the separate pass.  A spec leaf's own p and q (kept in the
-// subtree-end and proposal slots) are not stored at ZP, ZQ: the point is
-// the current one again only if the tree stops, and then its trajectory
-// ends are not read again (the transition ends).  A leaf without spec
-// stores the whole point (q, p, g) at ZQ, ZP, ZG.
+// Every rho of the tree (rho_init, rho_final of each level, rho_fwd /
+// rho_bck of the top-level subtree) starts as a zero vector and receives
+// exactly one "rho += x" (base_nuts.hpp: the base case adds p, a merged
+// subtree adds rho_init + rho_final, each into its caller's rho), so the
+// slots are never read before that write, which stores 0 + x (bitwise the
+// sum into a cleared slot) and need no clearing.  Likewise the final
+// half's proposal, z_propose_final = z at its start, is overwritten by its
+// first leaf before any merge reads it, so its start is not stored.
+//
+// With ls > 0 (the first level whose initial half this leaf completes if
+// the merges below it hold; spec) the tree, if it goes on, goes on from
+// this point in the same direction (the final half of level ls starts
+// here), so the same pass also takes the next leapfrog's begin half:
+// p' = p - e/2 g into NP and q' = q + e minv p' into theta, the gradient
+// request position; the final half's start then requests that gradient
+// without another pass, and the next leaf reads its point as q = theta,
+// p = NP (znext).  The arithmetic is that of the separate pass.  A spec
+// leaf's own p and q (kept in the subtree-end and proposal slots) are not
+// stored at ZP, ZQ: the point is the current one again only if the tree
+// stops, and then its trajectory ends are not read again (the transition
+// ends).  A leaf without spec stores the whole point (q, p, g) at ZQ, ZP,
+// ZG.
 __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls, double& T) {
   chain_state* s = k.s;
   double acc[1] = {0.0};
@@ -505,23 +512,19 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
     double* r = k.slot(d.rho);
     double* pb = k.slot(d.pbeg);
     double* pe = k.slot(d.pend);
-    double* zpf = k.slot(ls > 0 ? ZPF(ls) : TMP);
-    double* rf = k.slot(ls > 0 ? RF(ls) : TMP);
     const double he = 0.5 * lf_eps;
-    struct lf { double g, p, m, q, r; };
-    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{ok ? o[1 + i] : g[i], pin[i], m[i], qin[i], r[i]}; },
+    struct lf { double g, p, m, q; };
+    vpass_acc<1, lf>(k, acc, [&](int i) { return lf{ok ? o[1 + i] : g[i], pin[i], m[i], qin[i]}; },
                      [&](int i, const lf& v, double (&ac)[1]) {
                        const double gn = -v.g;
                        const double pn = v.p - he * gn;
                        g[i] = gn;
                        ac[0] += pn * (v.m * pn);
                        prop[i] = v.q;
-                       r[i] = v.r + pn;
+                       r[i] = 0.0 + pn;
                        pb[i] = pn;
                        pe[i] = pn;
                        if (spec) {
-                         zpf[i] = v.q;
-                         rf[i] = 0.0;
                          const double p2 = pn - he * gn;
                          np[i] = p2;
                          th[i] = v.q + lf_eps * (v.m * p2);
@@ -557,11 +560,8 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
   return !s->divergent;
 }
 
-// merge of the two halves of level l into d (base_nuts.hpp:321-351).
-// With clear > 0 the final half of level l + 1 starts next (its start was
-// taken by the leaf's pass): the same pass resets rho_init of levels
-// 1 .. clear (= l; this merge is the last to read them) for its subtree.
-__device__ bool merge(const ctx& k, int l, const dest& d, int clear) {
+// merge of the two halves of level l into d (base_nuts.hpp:321-351)
+__device__ bool merge(const ctx& k, int l, const dest& d) {
   chain_state* s = k.s;
   const double lsw_sub = log_sum_exp(s->lsw_init[l], s->lsw_final[l]);
   double& lw = lsw_ref(k, d.lsw);
@@ -582,7 +582,7 @@ __device__ bool merge(const ctx& k, int l, const dest& d, int clear) {
     }
   }
   // one pass: the proposal copy, rho_subtree = rho_init + rho_final,
-  // rho(d) += rho_subtree, and the six sums of the three U-turn checks
+  // rho(d) = 0 + rho_subtree (its one contribution), and the six sums of the three U-turn checks
   // (p_sharp = minv p): merged (pend, pbeg against rho_subtree), left join
   // (pfbg, pbeg against rho_init + pfbg), right join (pend, pie against
   // rho_final + pie)
@@ -598,20 +598,17 @@ __device__ bool merge(const ctx& k, int l, const dest& d, int clear) {
     const double* psrc = k.slot(prop_slot(l));
     double* pdst = k.slot(prop_slot(d.zprop));
     double* r = k.slot(d.rho);
-    double* ri1 = k.slot(RI(1));  // RI(j) = RI(1) + 5 (j - 1) slots
-    const int64_t rstep = 5 * static_cast<int64_t>(k.a->Dpad);
-    struct mg { double ri, rf, r, m, pend, pbeg, pfbg, pie, prop; };
+    struct mg { double ri, rf, m, pend, pbeg, pfbg, pie, prop; };
     vpass_acc<6, mg>(
         k, acc,
         [&](int i) {
-          return mg{ri[i], rf[i], r[i], m[i], pend[i], pbeg[i], pfbg[i], pie[i],
+          return mg{ri[i], rf[i], m[i], pend[i], pbeg[i], pfbg[i], pie[i],
                     take ? psrc[i] : 0.0};
         },
         [&](int i, const mg& v, double (&ac)[6]) {
           if (take) pdst[i] = v.prop;
           const double tv = v.ri + v.rf;
-          r[i] = v.r + tv;
-          for (int j = 0; j < clear; ++j) ri1[i + j * rstep] = 0.0;
+          r[i] = 0.0 + tv;
           ac[0] += (v.m * v.pend) * tv;
           ac[1] += (v.m * v.pbeg) * tv;
           ac[2] += (v.m * v.pfbg) * (v.ri + v.pfbg);
@@ -1036,10 +1033,11 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         }
         const bool fwd = uniform(k) > 0.5;
         __syncthreads();
-        // one pass: z = the chosen end; rho of the new subtree's side = 0,
-        // of the other side = rho; the other side's end momentum; every
-        // level's rho_init = 0 (and its frame reset); then the first leapfrog's
-        // begin half and its gradient request (leapfrog_request)
+        // one pass: z = the chosen end; rho of the other side = rho; the
+        // other side's end momentum; then the first leapfrog's begin half
+        // and its gradient request (leapfrog_request).  The new subtree's
+        // rho and every level's rho_init are written by their one
+        // contribution (see leaf), not cleared here.
         const int td = s.depth;
         const double eps = (fwd ? 1.0 : -1.0) * s.eps;
         {
@@ -1051,11 +1049,8 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
           double* zq = k.slot(ZQ);
           double* zp = k.slot(ZP);
           double* zg = k.slot(ZG);
-          double* rsame = k.slot(fwd ? RHOF : RHOB);
           double* rother = k.slot(fwd ? RHOB : RHOF);
           double* pother = k.slot(fwd ? PBF : PFB);
-          double* ri1 = k.slot(RI(1));  // RI(l) = RI(1) + 5 (l - 1) slots
-          const int64_t rstep = 5 * static_cast<int64_t>(a.Dpad);
           double* th = a.theta + static_cast<int64_t>(c) * a.D;
           const double he = 0.5 * eps;
           struct ez { double q, p, g, m, r; };
@@ -1067,10 +1062,8 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
                       zp[i] = pn;
                       zg[i] = v.g;
                       th[i] = qn;
-                      rsame[i] = 0.0;
                       rother[i] = v.r;
                       pother[i] = v.p;
-                      for (int l = 0; l < td; ++l) ri1[i + l * rstep] = 0.0;
                     });
         }
         __syncthreads();
@@ -1109,8 +1102,9 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
         if (valid && l <= s.tree_depth) {
           if (s.stage[l] == 0) {
             // initial half done (l is the leaf's ls): the final half starts
-            // from the current point, whose proposal position (ZPF(l)),
-            // rho_final = 0 and kinetic energy the leaf's pass produced
+            // from the current point (its proposal energies are set here as
+            // the reference's z_propose_final = z; the first leaf of the
+            // half overwrites them and the position, which is not stored)
             __syncthreads();
             if (lane == 0) {
               s.stage[l] = 1;
@@ -1119,10 +1113,8 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
               s.lsw_final[l] = -kInf;
             }
             __syncthreads();
-            // the leaf's pass took the begin half of the next leapfrog (NQ,
-            // NP, theta) and the last merge below l reset the lower levels'
-            // rho_init: request the gradient there.  Otherwise one pass:
-            // every lower level's rho_init = 0 (its frame reset) and the
+            // the leaf's pass took the begin half of the next leapfrog (NP,
+            // theta): request the gradient there.  Otherwise one pass: the
             // leapfrog's begin half with its gradient request
             // (leapfrog_request)
             const bool spec = s.spec != 0;
@@ -1133,8 +1125,6 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
               const double* g = k.slot(ZG);
               const double* m = k.slot(MINV);
               double* th = a.theta + static_cast<int64_t>(c) * a.D;
-              double* ri1 = k.slot(RI(1));  // RI(j) = RI(1) + 5 (j - 1) slots
-              const int64_t rstep = 5 * static_cast<int64_t>(a.Dpad);
               const double he = 0.5 * eps;
               struct pgqm { double p, g, q, m; };
               vpass<pgqm>(k, [&](int i) { return pgqm{p[i], g[i], q[i], m[i]}; },
@@ -1144,7 +1134,6 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
                             p[i] = pn;
                             q[i] = qn;
                             th[i] = qn;
-                            for (int j = 0; j < l - 1; ++j) ri1[i + j * rstep] = 0.0;
                           });
             }
             __syncthreads();
@@ -1161,9 +1150,7 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
             }
             break;
           }
-          // the last merge before the final half of level l + 1 starts
-          const int clear = l + 1 <= s.tree_depth && s.stage[l + 1] == 0 ? l : 0;
-          const bool v = merge(k, l, dest_of(k, l), clear);
+          const bool v = merge(k, l, dest_of(k, l));
           __syncthreads();
           if (lane == 0) {
             s.mvalid = v ? 1 : 0;

@@ -12,8 +12,8 @@ namespace sx {
 namespace nb {
 
 constexpr int kMaxDepth = 12;  // frames per chain (max_depth <= kMaxDepth)
-constexpr int kNumSlots = 24 + 5 * kMaxDepth;
-constexpr int kSlotMinv = 20;  // vector slot of the diagonal inverse metric
+constexpr int kNumSlots = 23 + 5 * kMaxDepth;
+constexpr int kSlotMinv = 19;  // vector slot of the diagonal inverse metric
 
 enum phase : int {
   PH_INIT_TRY = 0,
