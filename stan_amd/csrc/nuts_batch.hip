@@ -512,6 +512,13 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
     double* r = k.slot(d.rho);
     double* pb = k.slot(d.pbeg);
     double* pe = k.slot(d.pend);
+    // below the top (tree_depth >= 1) the leaf is the initial or the final
+    // half of a level-1 subtree, whose merge reads its p from the subtree's
+    // begin or end slot and forms its rho as 0 + p (see merge): the leaf
+    // stores p once and no rho
+    const bool top = s->tree_depth == 0;
+    const bool wb = top || s->stage[1] == 0;
+    const bool we = top || s->stage[1] != 0;
     const double he = 0.5 * lf_eps;
     struct lf { double g, p, m, q; };
     vpass_acc<1, lf>(k, acc, [&](int i) { return lf{ok ? o[1 + i] : g[i], pin[i], m[i], qin[i]}; },
@@ -521,9 +528,9 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
                        g[i] = gn;
                        ac[0] += pn * (v.m * pn);
                        prop[i] = v.q;
-                       r[i] = 0.0 + pn;
-                       pb[i] = pn;
-                       pe[i] = pn;
+                       if (top) r[i] = 0.0 + pn;
+                       if (wb) pb[i] = pn;
+                       if (we) pe[i] = pn;
                        if (spec) {
                          const double p2 = pn - he * gn;
                          np[i] = p2;
@@ -560,6 +567,54 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
   return !s->divergent;
 }
 
+// The vector pass of merge().  At level 1 both halves are single leaves:
+// the initial half's p (its begin and end) is at the merged subtree's begin
+// slot, the final half's p at its end slot, and their rho_init and
+// rho_final, 0 + p (the leaf's one contribution), are formed here (L1).
+template <bool L1>
+__device__ void merge_pass(const ctx& k, int l, const dest& d, bool take, double (&acc)[6]) {
+  const double* m = k.slot(MINV);
+  const double* pend = k.slot(d.pend);
+  const double* pbeg = k.slot(d.pbeg);
+  const double* psrc = k.slot(prop_slot(l));
+  double* pdst = k.slot(prop_slot(d.zprop));
+  double* r = k.slot(d.rho);
+  auto body = [&](int i, double ri, double rf, double mv, double pe, double pb, double pfbg,
+                  double pie, double prop, double (&ac)[6]) {
+    if (take) pdst[i] = prop;
+    const double tv = ri + rf;
+    r[i] = 0.0 + tv;
+    ac[0] += (mv * pe) * tv;
+    ac[1] += (mv * pb) * tv;
+    ac[2] += (mv * pfbg) * (ri + pfbg);
+    ac[3] += (mv * pb) * (ri + pfbg);
+    ac[4] += (mv * pe) * (rf + pie);
+    ac[5] += (mv * pie) * (rf + pie);
+  };
+  if constexpr (L1) {
+    struct mg { double m, pend, pbeg, prop; };
+    vpass_acc<6, mg>(
+        k, acc, [&](int i) { return mg{m[i], pend[i], pbeg[i], take ? psrc[i] : 0.0}; },
+        [&](int i, const mg& v, double (&ac)[6]) {
+          body(i, 0.0 + v.pbeg, 0.0 + v.pend, v.m, v.pend, v.pbeg, v.pend, v.pbeg, v.prop, ac);
+        });
+  } else {
+    const double* ri = k.slot(RI(l));
+    const double* rf = k.slot(RF(l));
+    const double* pfbg = k.slot(PFBG(l));
+    const double* pie = k.slot(PIE(l));
+    struct mg { double ri, rf, m, pend, pbeg, pfbg, pie, prop; };
+    vpass_acc<6, mg>(
+        k, acc,
+        [&](int i) {
+          return mg{ri[i], rf[i], m[i], pend[i], pbeg[i], pfbg[i], pie[i], take ? psrc[i] : 0.0};
+        },
+        [&](int i, const mg& v, double (&ac)[6]) {
+          body(i, v.ri, v.rf, v.m, v.pend, v.pbeg, v.pfbg, v.pie, v.prop, ac);
+        });
+  }
+}
+
 // merge of the two halves of level l into d (base_nuts.hpp:321-351)
 __device__ bool merge(const ctx& k, int l, const dest& d) {
   chain_state* s = k.s;
@@ -582,41 +637,15 @@ __device__ bool merge(const ctx& k, int l, const dest& d) {
     }
   }
   // one pass: the proposal copy, rho_subtree = rho_init + rho_final,
-  // rho(d) = 0 + rho_subtree (its one contribution), and the six sums of the three U-turn checks
-  // (p_sharp = minv p): merged (pend, pbeg against rho_subtree), left join
-  // (pfbg, pbeg against rho_init + pfbg), right join (pend, pie against
-  // rho_final + pie)
+  // rho(d) = 0 + rho_subtree (its one contribution), and the six sums of
+  // the three U-turn checks (p_sharp = minv p): merged (pend, pbeg against
+  // rho_subtree), left join (pfbg, pbeg against rho_init + pfbg), right
+  // join (pend, pie against rho_final + pie)
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  {
-    const double* ri = k.slot(RI(l));
-    const double* rf = k.slot(RF(l));
-    const double* m = k.slot(MINV);
-    const double* pend = k.slot(d.pend);
-    const double* pbeg = k.slot(d.pbeg);
-    const double* pfbg = k.slot(PFBG(l));
-    const double* pie = k.slot(PIE(l));
-    const double* psrc = k.slot(prop_slot(l));
-    double* pdst = k.slot(prop_slot(d.zprop));
-    double* r = k.slot(d.rho);
-    struct mg { double ri, rf, m, pend, pbeg, pfbg, pie, prop; };
-    vpass_acc<6, mg>(
-        k, acc,
-        [&](int i) {
-          return mg{ri[i], rf[i], m[i], pend[i], pbeg[i], pfbg[i], pie[i],
-                    take ? psrc[i] : 0.0};
-        },
-        [&](int i, const mg& v, double (&ac)[6]) {
-          if (take) pdst[i] = v.prop;
-          const double tv = v.ri + v.rf;
-          r[i] = 0.0 + tv;
-          ac[0] += (v.m * v.pend) * tv;
-          ac[1] += (v.m * v.pbeg) * tv;
-          ac[2] += (v.m * v.pfbg) * (v.ri + v.pfbg);
-          ac[3] += (v.m * v.pbeg) * (v.ri + v.pfbg);
-          ac[4] += (v.m * v.pend) * (v.rf + v.pie);
-          ac[5] += (v.m * v.pie) * (v.rf + v.pie);
-        });
-  }
+  if (l == 1)
+    merge_pass<true>(k, l, d, take, acc);
+  else
+    merge_pass<false>(k, l, d, take, acc);
   block_sum_n<6>(k, acc);
   bool persist = acc[0] > 0 && acc[1] > 0;
   persist &= acc[2] > 0 && acc[3] > 0;
