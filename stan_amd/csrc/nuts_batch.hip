@@ -488,11 +488,14 @@ __device__ dest dest_of(const ctx& k, int level) {
 // request position; the final half's start then requests that gradient
 // without another pass, and the next leaf reads its point as q = theta,
 // p = NP (znext).  The arithmetic is that of the separate pass.  A spec
-// leaf's own p and q (kept in the subtree-end and proposal slots) are not
-// stored at ZP, ZQ: the point is the current one again only if the tree
-// stops, and then its trajectory ends are not read again (the transition
-// ends).  A leaf without spec stores the whole point (q, p, g) at ZQ, ZP,
-// ZG.
+// leaf's own point is not stored at ZQ, ZP, ZG (its p and q are kept in
+// the subtree-end and proposal slots): the point is the current one again
+// only if the tree stops, and then its trajectory ends are not read again
+// (the transition ends).  Its g is read next only by the next leaf after a
+// domain error (the flipped stale g), and that leaf is divergent (V = inf):
+// the tree stops there and none of its values is read again, so a stale g
+// from an earlier leaf serves.  A leaf without spec stores the whole point
+// (q, p, g) at ZQ, ZP, ZG.
 __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls, double& T) {
   chain_state* s = k.s;
   double acc[1] = {0.0};
@@ -525,7 +528,6 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
                      [&](int i, const lf& v, double (&ac)[1]) {
                        const double gn = -v.g;
                        const double pn = v.p - he * gn;
-                       g[i] = gn;
                        ac[0] += pn * (v.m * pn);
                        prop[i] = v.q;
                        if (top) r[i] = 0.0 + pn;
@@ -536,6 +538,7 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
                          np[i] = p2;
                          th[i] = v.q + lf_eps * (v.m * p2);
                        } else {
+                         g[i] = gn;
                          p[i] = pn;
                          if (zn) zq[i] = v.q;
                        }
