@@ -334,7 +334,7 @@ int hier_autotune(sx_glm* m, const int64_t* off, int kernel) {
 // (2 y_n - 1) x_n.
 int hier_setup(sx_glm* m, const double* d_X, const double* X, const int32_t* y,
                const int64_t* off, bool do_tune, int chunk, int kernel) {
-  if (m->precision == SX_FP32 && m->N * m->K > 0) {
+  if (m->precision == SX_FP32 && m->N > 0) {
     // 32 K + 64 zero floats past the last row: the FP32 pass stages whole
     // 32-row tiles (64-float loads per lane) past a chunk's end
     const size_t n = static_cast<size_t>(m->N * m->K);
@@ -345,8 +345,44 @@ int hier_setup(sx_glm* m, const double* d_X, const double* X, const int32_t* y,
       const float v = static_cast<float>(X[i]);
       x32[i] = y[i / K] ? v : -v;
     }
-    SX_HIP(hipMalloc(&m->d_X32, total * sizeof(float)));
-    SX_HIP(hipMemcpy(m->d_X32, x32.data(), total * sizeof(float), hipMemcpyHostToDevice));
+    if (n > 0) {
+      SX_HIP(hipMalloc(&m->d_X32, total * sizeof(float)));
+      SX_HIP(hipMemcpy(m->d_X32, x32.data(), total * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (K <= 31) {
+      // the x6 pass's copy: per row and term t, columns 0 .. 8 NCH - 1 of
+      // (x'_0 .. x'_{K-1}, s, 0 ...) as term t of the exact three-term
+      // BF16 split by truncation (h = x with the low 16 bits cleared,
+      // m = (x - h) likewise, l = x - h - m), 32 zero rows past the last
+      const size_t W = 8 * static_cast<size_t>(sx::hier_x6_chunks(static_cast<int>(K)));
+      const size_t N = static_cast<size_t>(m->N);
+      std::vector<uint16_t> xs((N + 32) * 3 * W, 0);
+      auto bits = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+      };
+      auto from_bits = [](uint32_t u) {
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+      };
+      for (size_t r = 0; r < N; ++r) {
+        uint16_t* row = xs.data() + r * 3 * W;
+        for (size_t c = 0; c <= K; ++c) {
+          const float v = c < K ? x32[r * K + c] : (y[r] ? 1.0f : -1.0f);
+          const uint32_t h = bits(v) & 0xffff0000u;
+          const float r1 = v - from_bits(h);
+          const uint32_t mm = bits(r1) & 0xffff0000u;
+          const uint32_t l = bits(r1 - from_bits(mm));
+          row[c] = static_cast<uint16_t>(h >> 16);
+          row[W + c] = static_cast<uint16_t>(mm >> 16);
+          row[2 * W + c] = static_cast<uint16_t>(l >> 16);
+        }
+      }
+      SX_HIP(hipMalloc(&m->d_XS, xs.size() * sizeof(uint16_t)));
+      SX_HIP(hipMemcpy(m->d_XS, xs.data(), xs.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
   }
   m->d_X = const_cast<double*>(d_X);  // the FP64 pass reads X row-major
   // FP32 products unless asked for (or timed) otherwise
@@ -597,6 +633,7 @@ void sx_glm_destroy(sx_glm* m) {
   (void)hipFree(m->d_btheta);
   (void)hipFree(m->d_bout);
   (void)hipFree(m->d_X32);
+  (void)hipFree(m->d_XS);
   (void)hipFree(m->d_chunk_group);
   (void)hipFree(m->d_chunk_row0);
   (void)hipFree(m->d_group_chunk0);
@@ -936,6 +973,7 @@ int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, dou
   hier_args a{};
   a.X = m->d_X;
   a.X32 = m->d_X32;
+  a.XS = m->d_XS;
   a.y = static_cast<const int8_t*>(m->d_y);
   a.theta = d_theta;
   a.out = d_out;

@@ -111,6 +111,8 @@ int64_t footprint_of(const sx_job_spec& s) {
   bytes += s.family == SX_NORMAL ? 8 * N : N;                     // y
   if (s.family == SX_HIER_LOGIT) {
     bytes += 8 * (N * K + 64 * K) + (prec == SX_FP32 ? 4 * (N * K + 32 * K + 64) : 0);
+    if (prec == SX_FP32 && K <= 31)  // the split copy of X (3 BF16 terms)
+      bytes += 48 * (N + 32) * sx::hier_x6_chunks(static_cast<int>(K));
     const int64_t chunks = ceil_div(N, sx::kHierChunkMin) + s.J;  // any autotuned chunk size
     bytes += chunks * 16 + (s.J + 1) * 8;
     bytes += 8 * ((K + 3) * std::max<int64_t>(1, chunks) + (2 * K + 5) * s.J) * (C + 1);

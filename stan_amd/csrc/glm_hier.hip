@@ -388,7 +388,7 @@ __device__ __forceinline__ uint32_t pack_hi(uint32_t lo_elem, uint32_t hi_elem) 
 
 // The same pass with FP32 products formed on the BF16 matrix cores
 // (K <= 31).  Every FP32 operand is split exactly into three BF16 terms,
-// x' = xh + xm + xl (X, staged from the FP32 copy), w' = wh + wm + wl (the
+// x' = xh + xm + xl (X, split once at set-up), w' = wh + wm + wl (the
 // chain coefficients), r' = rh + rm + rl (the residuals), and each product
 // keeps the six cross terms down to 2^-14 of the leading one:
 //   forward  Q += Xh Wh + Xh Wm + Xm Wh + Xh Wl + Xl Wh + Xm Wm
@@ -402,11 +402,19 @@ __device__ __forceinline__ uint32_t pack_hi(uint32_t lo_elem, uint32_t hi_elem) 
 // pass; the forward operand (8 consecutive columns of a row) is one
 // 16-byte read per term, the adjoint's (8 rows of the lane's column,
 // matching the rows of the lane's own residuals) two transposing reads.
-// Link, padding rows and outputs as in hier_mfma_f32.
+// The terms of X (sign column included) come from the split copy XS, NCH
+// 16-byte chunks per row and term (columns 0 .. 8 NCH - 1), so a tile is
+// 32 x 3 NCH contiguous chunks that the wave copies to their permuted LDS
+// positions (16-byte loads and stores, no VALU split); rows past the
+// chunk's end are stored as zeros.  Link, padding rows and outputs as in
+// hier_mfma_f32.
 template <int K>
 __global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
   static_assert(K >= 0 && K <= 31, "K + 1 <= 32 columns");
-  constexpr int NS = (32 * K + 63) / 64;   // staged floats per lane
+  constexpr int NCH = hier_x6_chunks(K);   // 16-byte chunks per row and term
+  constexpr int RC = 3 * NCH;              // chunks per row
+  constexpr int TC = 32 * RC;              // chunks per tile
+  constexpr int NL = (TC + 63) / 64;       // chunk copies per lane and tile
   constexpr int KST = (K + 16) / 16;       // forward k-steps of 16 (columns 0 .. K)
   constexpr int ROWB = 64;                 // bytes of a tile row (32 BF16)
   constexpr int TB = 32 * ROWB;            // bytes of one term's tile
@@ -454,27 +462,25 @@ __global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
   // all three terms zero: columns past K and rows past the chunk stay zero
   for (int e = lane; e < WB / 16; e += 64)
     *reinterpret_cast<uint4*>(base + 16 * e) = make_uint4(0u, 0u, 0u, 0u);
-  // LDS byte offset of each staged float (row e / K, column e % K of the
-  // tile, its 16-byte chunk permuted); elements e >= 32 K (only in the
-  // last staging register, for odd K) are not written
-  constexpr int Kd = K > 0 ? K : 1;
-  int dst[NS > 0 ? NS : 1];
+  // LDS byte offset and tile row of the chunks this lane copies: chunk
+  // e = lane + 64 j of a tile is chunk e % NCH of term (e / NCH) % 3 of row
+  // e / RC; the same for every tile
+  int dst[NL], drow[NL];
 #pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    const int e = lane + 64 * j;
-    const int r = e / Kd, cc = e % Kd;
-    dst[j] = elem_offset<ROWB, 4>(r, cc);
+  for (int j = 0; j < NL; ++j) {
+    const int e = lane + 64 * j < TC ? lane + 64 * j : 0;  // past TC: not stored
+    dst[j] = split_chunk_offset<ROWB, 4, NCH>(e, TB);
+    drow[j] = lane + 64 * j < TC ? split_chunk_row<NCH>(e) : 32;
   }
-  // the sign column's position in row `lane` (lanes < 32)
-  const int sgn = elem_offset<ROWB, 4>(li, K);
-  float stg[NS > 0 ? NS : 1];
-  int ybyte = 0, scnt = 0;
+  const uint4* const xs = reinterpret_cast<const uint4*>(a.XS);
+  uint4 stg[NL];
+  int scnt = 0;
 #define SX_LOAD_TILE(T0)                                                          \
   {                                                                               \
     scnt = rows - (T0) < 32 ? rows - (T0) : 32;                                   \
-    const float* src_ = a.X32 + (r0 + (T0)) * K + lane;                           \
-    _Pragma("unroll") for (int j = 0; j < NS; ++j) stg[j] = src_[64 * j];         \
-    ybyte = a.y[r0 + (T0) + (lane < scnt ? lane : scnt - 1)];                     \
+    const uint4* src_ = xs + (r0 + (T0)) * RC + lane;                             \
+    _Pragma("unroll") for (int j = 0; j < NL; ++j)                                \
+      if (64 * (j + 1) <= TC || lane + 64 * j < TC) stg[j] = src_[64 * j];        \
   }
   SX_LOAD_TILE(0);
   int ln = lane;
@@ -486,18 +492,11 @@ __global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
     // are done before these writes land
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      if (64 * (j + 1) <= 32 * K || lane + 64 * j < 32 * K) {
-        const float v = lane + 64 * j < cnt * K ? stg[j] : 0.f;
-        uint32_t hh, mm, ll;
-        split3(v, hh, mm, ll);
-        *reinterpret_cast<uint16_t*>(base + dst[j]) = static_cast<uint16_t>(hh >> 16);
-        *reinterpret_cast<uint16_t*>(base + TB + dst[j]) = static_cast<uint16_t>(mm >> 16);
-        *reinterpret_cast<uint16_t*>(base + 2 * TB + dst[j]) = static_cast<uint16_t>(ll >> 16);
-      }
+    for (int j = 0; j < NL; ++j) {
+      if (64 * (j + 1) <= TC || lane + 64 * j < TC)
+        *reinterpret_cast<uint4*>(base + dst[j]) =
+            drow[j] < cnt ? stg[j] : make_uint4(0u, 0u, 0u, 0u);
     }
-    if (lane < 32)
-      *reinterpret_cast<__bf16*>(base + sgn) = to_bf16(lane < cnt ? (ybyte ? 1.0f : -1.0f) : 0.f);
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     if (t0 + 32 < rows) SX_LOAD_TILE(t0 + 32);  // in flight during this tile
@@ -788,6 +787,7 @@ __global__ __launch_bounds__(1024) void hier_final(hier_args a) {
 }
 
 hipError_t launch_hier_x6(const hier_args& a, dim3 grid, hipStream_t s) {
+  if (!a.XS && a.n_chunks > 0) return hipErrorInvalidValue;  // no split copy of X
   switch (a.K) {
 #define SX_CASE(KK) \
   case KK: hipLaunchKernelGGL((hier_mfma_x6<KK>), grid, dim3(256), 0, s, a); break;

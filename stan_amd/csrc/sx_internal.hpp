@@ -98,6 +98,7 @@ constexpr int kHierChunkMin = 128;
 struct hier_args {
   const double* X;          // N x K row-major (FP64 pass)
   const float* X32;         // N x K row-major (FP32 batched pass)
+  const uint16_t* XS;       // (N + 32) x 3 x 8 hier_x6_chunks(K) BF16 (x6 pass)
   const int8_t* y;
   const double* theta;      // chains x theta_stride
   double* out;              // chains x out_stride: lp, grad[D], status
@@ -112,6 +113,9 @@ struct hier_args {
   double alpha_sd, beta_sd, tau_sd, alpha_const, beta_const, tau_const;
 };
 hipError_t launch_hier(const hier_args& a, bool fp32, hipStream_t s);
+// 16-byte chunks of BF16 per row and term of the x6 pass's split copy of
+// X: columns 0 .. K (x' and the sign column), zero-padded to a multiple of 8
+constexpr int hier_x6_chunks(int K) { return (K + 8) / 8; }
 
 // Device workspace of batched MFMA evaluations for up to cap chains.
 struct mx_workspace {
@@ -180,6 +184,7 @@ struct sx_glm {
   int64_t bcap = 0;
   // hierarchical model
   float* d_X32 = nullptr;
+  uint16_t* d_XS = nullptr;  // split copy of the sign-folded X (FP32, K <= 31)
   int* d_chunk_group = nullptr;
   int64_t *d_chunk_row0 = nullptr, *d_group_chunk0 = nullptr;
   int64_t n_chunks = 0;

@@ -63,6 +63,19 @@ constexpr int xor_block() { return CPR % 16 == 0 ? 16 : CPR % 8 == 0 ? 8 : 4; }
 template <int ROWB, int P>
 SX_HD int elem_offset(int r, int c) { return r * ROWB + 16 * swz<P>(r, c >> 3) + 2 * (c & 7); }
 
+// Staging from a split copy whose rows hold 3 terms x NCH 16-byte chunks
+// (term-major, columns 0 .. 8 NCH - 1 of each term): chunk e of a 32-row
+// tile lies in row e / (3 NCH) of term (e % (3 NCH)) / NCH, at column
+// chunk e % NCH.  Byte offset of its destination within the three term
+// tiles (TB bytes apart), and its tile row.
+template <int ROWB, int P, int NCH>
+SX_HD int split_chunk_offset(int e, int TB) {
+  const int r = e / (3 * NCH), w = e % (3 * NCH);
+  return (w / NCH) * TB + elem_offset<ROWB, P>(r, 8 * (w % NCH));
+}
+template <int NCH>
+SX_HD int split_chunk_row(int e) { return e / (3 * NCH); }
+
 }  // namespace sx
 
 #endif

@@ -31,9 +31,11 @@ def test_lean_link_math_accuracy(sx_math_check):
 
 def test_staged_tile_layout_matches_mfma_operands(tmp_path):
     """The LDS tile layout of the BF16 MFMA passes (staging, element
-    writes, forward reads and transposing adjoint reads of
+    writes, chunk copies from the hierarchical pass's split copy of X,
+    forward reads and transposing adjoint reads of
     stan_amd/csrc/tile_layout.hpp), compiled for the host, delivers every
-    operand element the MFMA expects, for every padded K up to 256."""
+    operand element the MFMA expects, for every padded K up to 256 and
+    every hierarchical K up to 31."""
     exe = str(tmp_path / "tile_layout_check")
     subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe,
                     os.path.join(HERE, "cpp", "tile_layout_check.cpp")], check=True)
