@@ -543,18 +543,16 @@ __device__ bool leaf(const ctx& k, const dest& d, double lf_eps, bool ok, int ls
                          if (zn) zq[i] = v.q;
                        }
                      });
-    __syncthreads();
-    if (k.lane == 0) {
-      s->znext = 0;
-      s->spec = spec ? 1 : 0;
-    }
   }
   block_sum_n<1>(k, acc);
   T = 0.5 * acc[0];
   double h = T + s->V_z;
   if (isnan(h)) h = kInf;
   const double H0 = s->H0;
+  // every wave has read znext (block_sum_n's barriers, or one wave)
   if (k.lane == 0) {
+    s->znext = 0;
+    s->spec = ls > 0 ? 1 : 0;
     s->n_leapfrog++;
     if ((h - H0) > k.a->max_deltaH) s->divergent = 1;
     double& lw = lsw_ref(k, d.lsw);
@@ -851,14 +849,14 @@ __global__ __launch_bounds__(512) void nuts_step(batch_args a) {
   // one chain whose leaf closes a deep subtree or a transition while the
   // others take one leapfrog each; the arithmetic and the random stream
   // are those of an unbounded run.  A final half's start whose leapfrog
-  // the leaf's pass already took (no vector pass) is taken whatever the
-  // count and does not add to it.
-  for (int turn = 0; !s.pending && s.phase != PH_DONE && !s.xfail;) {
+  // the leaf's pass already took (no vector pass; it ends the launch with
+  // its gradient request) is taken whatever the count.
+  for (int turn = 0; !s.pending && s.phase != PH_DONE && !s.xfail; ++turn) {
     const int ph = s.phase;
-    const bool free_turn = ph == PH_MERGE && s.mvalid && s.mlevel <= s.tree_depth &&
-                           s.stage[s.mlevel] == 0 && s.spec;
-    if (!free_turn && turn >= a.phase_budget) break;
-    turn += free_turn ? 0 : 1;
+    if (turn >= a.phase_budget &&
+        !(ph == PH_MERGE && s.mvalid && s.mlevel <= s.tree_depth && s.stage[s.mlevel] == 0 &&
+          s.spec))
+      break;
     __syncthreads();
     switch (ph) {
       case PH_INIT_TRY: {
