@@ -734,6 +734,7 @@ void free_workspace(mx_workspace& w) {
 void free_workspace(hier_workspace& w) {
   (void)hipFree(w.cpart);
   (void)hipFree(w.gpart);
+  (void)hipFree(w.wsplit);
   w = hier_workspace();
 }
 
@@ -958,6 +959,11 @@ int hier_prepare(sx_glm* m, hier_workspace& w, int C) {
   const size_t E = static_cast<size_t>(m->K) + 3, G = 2 * static_cast<size_t>(m->K) + 5;
   SX_HIP(hipMalloc(&w.cpart, sizeof(double) * E * std::max<int64_t>(1, m->n_chunks) * C));
   SX_HIP(hipMalloc(&w.gpart, sizeof(double) * G * m->J * C));
+  if (m->precision == SX_FP32 && m->K <= 31) {
+    // the coefficient kernel writes columns 0 .. K; the rest stay zero
+    SX_HIP(hipMalloc(&w.wsplit, sizeof(uint16_t) * 96 * m->J * C));
+    SX_HIP(hipMemset(w.wsplit, 0, sizeof(uint16_t) * 96 * m->J * C));
+  }
   w.cap = C;
   return SX_OK;
 }
@@ -979,6 +985,7 @@ int hier_enqueue(sx_glm* m, hier_workspace& w, const double* d_theta, int C, dou
   a.out = d_out;
   a.cpart = w.cpart;
   a.gpart = w.gpart;
+  a.wsplit = w.wsplit;
   a.chunk_group = m->d_chunk_group;
   a.chunk_row0 = m->d_chunk_row0;
   a.group_chunk0 = m->d_group_chunk0;

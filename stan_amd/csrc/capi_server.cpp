@@ -116,6 +116,7 @@ int64_t footprint_of(const sx_job_spec& s) {
     const int64_t chunks = ceil_div(N, sx::kHierChunkMin) + s.J;  // any autotuned chunk size
     bytes += chunks * 16 + (s.J + 1) * 8;
     bytes += 8 * ((K + 3) * std::max<int64_t>(1, chunks) + (2 * K + 5) * s.J) * (C + 1);
+    if (prec == SX_FP32 && K <= 31) bytes += 192 * s.J * (C + 1);  // split coefficients
   } else {
     int L = 1;
     while (L < 64 && ceil_div(K, L) > 32) L <<= 1;

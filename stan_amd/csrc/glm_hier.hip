@@ -386,6 +386,42 @@ __device__ __forceinline__ uint32_t pack_hi(uint32_t lo_elem, uint32_t hi_elem) 
   return (hi_elem & 0xffff0000u) | (lo_elem >> 16);
 }
 
+// The x6 pass's chain coefficients, once per launch: for group g and chain
+// c, column k of w' = log2(e) (beta_k + tau_k z_{g,k+1}) (k < K) and
+// log2(e) (alpha + tau_0 z_{g,0}) (k = K), tau = exp(v), split into three
+// BF16 terms (split3 of the FP32 value), stored at
+// wsplit[((g C + c) 3 + term) 32 + k]; columns above K stay zero (cleared
+// at allocation).  One thread per (column k <= K, chain, 8 groups): tau is
+// computed once per thread.
+constexpr int kCoefGroups = 8;
+__global__ __launch_bounds__(256) void hier_coef_x6(hier_args a) {
+  const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int k = static_cast<int>(id & 31);
+  const int64_t cg = id >> 5;  // group block * chains + c
+  const int K = a.K;
+  const int c = static_cast<int>(cg % a.chains);
+  const int g0 = static_cast<int>(cg / a.chains) * kCoefGroups;
+  if (k > K || g0 >= a.J) return;
+  const double* th = a.theta + static_cast<int64_t>(c) * a.theta_stride;
+  const double base = k < K ? th[1 + k] : th[0];
+  const double tau = k < K ? exp(th[2 + K + k]) : exp(th[1 + K]);
+  const double* z = th + 2 + 2 * K + static_cast<int64_t>(g0) * (K + 1) + (k < K ? 1 + k : 0);
+  double zv[kCoefGroups];  // every load in flight before the first use
+#pragma unroll
+  for (int u = 0; u < kCoefGroups; ++u) zv[u] = g0 + u < a.J ? z[u * (K + 1)] : 0.0;
+#pragma unroll
+  for (int u = 0; u < kCoefGroups; ++u) {
+    if (g0 + u >= a.J) break;
+    const double v = base + tau * zv[u];
+    uint32_t hh, mm, ll;
+    split3(static_cast<float>(v * 1.4426950408889634074), hh, mm, ll);
+    uint16_t* w = a.wsplit + (static_cast<int64_t>(g0 + u) * a.chains + c) * 96 + k;
+    w[0] = static_cast<uint16_t>(hh >> 16);
+    w[32] = static_cast<uint16_t>(mm >> 16);
+    w[64] = static_cast<uint16_t>(ll >> 16);
+  }
+}
+
 // The same pass with FP32 products formed on the BF16 matrix cores
 // (K <= 31).  Every FP32 operand is split exactly into three BF16 terms,
 // x' = xh + xm + xl (X, split once at set-up), w' = wh + wm + wl (the
@@ -430,29 +466,20 @@ __global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
   const int rows = static_cast<int>(r1 - r0);
   const int cb = blockIdx.y * 32;
   char* const base = lds[w];
-  // chain coefficients (w' = log2(e) w, b' = log2(e) b at column K), the
-  // B operand of k-step s: columns 16 s + 8 h .. + 7 of chain cb + i
+  // chain coefficients (w' = log2(e) w, b' = log2(e) b at column K), split
+  // by hier_coef_x6: the B operand of k-step s is columns 16 s + 8 h .. + 7
+  // of chain cb + i, one 16-byte load per term
   bf16x8 bwh[KST], bwm[KST], bwl[KST];
   {
-    const int c = cb + li;
-    const double* th = a.theta + static_cast<int64_t>(c < a.chains ? c : 0) * a.theta_stride;
-    const double* z = th + 2 + 2 * K + static_cast<int64_t>(g) * (K + 1);
+    const int c = cb + li < a.chains ? cb + li : 0;
+    const uint4* wq =
+        reinterpret_cast<const uint4*>(a.wsplit) + (static_cast<int64_t>(g) * a.chains + c) * 12;
 #pragma unroll
-    for (int s = 0; s < KST; ++s)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = 16 * s + 8 * h + j;
-        double v = 0.0;
-        if (k < K)
-          v = th[1 + k] + exp(th[2 + K + k]) * z[1 + k];
-        else if (k == K)
-          v = th[0] + exp(th[1 + K]) * z[0];
-        uint32_t hh, mm, ll;
-        split3(static_cast<float>(v * 1.4426950408889634074), hh, mm, ll);
-        bwh[s][j] = __builtin_bit_cast(__bf16, static_cast<uint16_t>(hh >> 16));
-        bwm[s][j] = __builtin_bit_cast(__bf16, static_cast<uint16_t>(mm >> 16));
-        bwl[s][j] = __builtin_bit_cast(__bf16, static_cast<uint16_t>(ll >> 16));
-      }
+    for (int s = 0; s < KST; ++s) {
+      bwh[s] = __builtin_bit_cast(bf16x8, wq[2 * s + h]);
+      bwm[s] = __builtin_bit_cast(bf16x8, wq[4 + 2 * s + h]);
+      bwl[s] = __builtin_bit_cast(bf16x8, wq[8 + 2 * s + h]);
+    }
   }
   f32x16 sacc;
 #pragma unroll
@@ -787,7 +814,13 @@ __global__ __launch_bounds__(1024) void hier_final(hier_args a) {
 }
 
 hipError_t launch_hier_x6(const hier_args& a, dim3 grid, hipStream_t s) {
-  if (!a.XS && a.n_chunks > 0) return hipErrorInvalidValue;  // no split copy of X
+  // no split copy of X, or no coefficient buffer
+  if ((!a.XS || !a.wsplit) && a.n_chunks > 0) return hipErrorInvalidValue;
+  const int64_t nc =
+      static_cast<int64_t>((a.J + kCoefGroups - 1) / kCoefGroups) * a.chains * 32;
+  if (nc > 0)
+    hipLaunchKernelGGL(hier_coef_x6, dim3(static_cast<unsigned>((nc + 255) / 256)), dim3(256), 0,
+                       s, a);
   switch (a.K) {
 #define SX_CASE(KK) \
   case KK: hipLaunchKernelGGL((hier_mfma_x6<KK>), grid, dim3(256), 0, s, a); break;

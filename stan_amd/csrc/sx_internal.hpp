@@ -99,6 +99,7 @@ struct hier_args {
   const double* X;          // N x K row-major (FP64 pass)
   const float* X32;         // N x K row-major (FP32 batched pass)
   const uint16_t* XS;       // (N + 32) x 3 x 8 hier_x6_chunks(K) BF16 (x6 pass)
+  uint16_t* wsplit;         // J x chains x 3 x 32 BF16: split coefficients (x6 pass)
   const int8_t* y;
   const double* theta;      // chains x theta_stride
   double* out;              // chains x out_stride: lp, grad[D], status
@@ -132,6 +133,7 @@ struct hier_workspace {
   int64_t cap = 0;
   double* cpart = nullptr;  // n_chunks x chains x (K + 3)
   double* gpart = nullptr;  // chains x J x (2K + 5)
+  uint16_t* wsplit = nullptr;  // J x chains x 3 x 32 (FP32 contexts, K <= 31)
 };
 
 // Each user of batched evaluations (the context's own batch calls, every
