@@ -459,12 +459,20 @@ __global__ __launch_bounds__(256) void hier_mfma_x6(hier_args a) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 31, h = lane >> 5;
-  const int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  // block -> (group of 4 chunks, block of 32 chains): the chain blocks of a
+  // chunk group are 8 block ids apart, i.e. on one XCD and dispatched
+  // together, so all but the first read the chunk's tiles from that XCD's
+  // L2 (launch_hier_x6: a 1-D grid of chunk groups rounded up to 8 x chain
+  // blocks)
+  const int ncb = (a.chains + 31) / 32;
+  const int bq = static_cast<int>(blockIdx.x) >> 3;
+  const int cbi = bq % ncb;
+  const int64_t ch = (static_cast<int64_t>(bq / ncb) * 8 + (blockIdx.x & 7)) * 4 + w;
   if (ch >= a.n_chunks) return;  // whole wave; no block barrier follows
   const int g = a.chunk_group[ch];
   const int64_t r0 = a.chunk_row0[ch], r1 = a.chunk_row0[ch + 1];
   const int rows = static_cast<int>(r1 - r0);
-  const int cb = blockIdx.y * 32;
+  const int cb = cbi * 32;
   char* const base = lds[w];
   // chain coefficients (w' = log2(e) w, b' = log2(e) b at column K), split
   // by hier_coef_x6: the B operand of k-step s is columns 16 s + 8 h .. + 7
@@ -821,6 +829,8 @@ hipError_t launch_hier_x6(const hier_args& a, dim3 grid, hipStream_t s) {
   if (nc > 0)
     hipLaunchKernelGGL(hier_coef_x6, dim3(static_cast<unsigned>((nc + 255) / 256)), dim3(256), 0,
                        s, a);
+  const unsigned ncg8 = static_cast<unsigned>(((a.n_chunks + 3) / 4 + 7) / 8 * 8);
+  grid = dim3(ncg8 * static_cast<unsigned>((a.chains + 31) / 32), 1);
   switch (a.K) {
 #define SX_CASE(KK) \
   case KK: hipLaunchKernelGGL((hier_mfma_x6<KK>), grid, dim3(256), 0, s, a); break;
